@@ -1,0 +1,163 @@
+/*
+ * nixis.h -- C ABI of libnixis_hip.so, the MI355X (gfx950) implementation of
+ * the Nixis terrain hot path: fractal OpenSimplex noise at every icosphere
+ * vertex and the height/water/sediment erosion iteration.
+ *
+ * Conventions
+ *   - Every entry point returns an int status: NX_OK (0) or a negative
+ *     NX_E* code; nx_last_error() then returns a message (thread-local).
+ *   - Arrays marked "device" are caller-owned device pointers (for example
+ *     torch tensors' data_ptr()).  Arrays marked "host" are ordinary host
+ *     memory.  Nothing here takes ownership of caller memory.
+ *   - `stream` is a hipStream_t passed as void* (NULL = the null stream).
+ *     Device entry points are asynchronous on that stream.  Range errors the
+ *     kernels detect (NX_ERANGE, NX_EDIV0) are sticky per context and are
+ *     returned by nx_ctx_sync_check().
+ *   - Library-owned state (sticky error word, graphs, autotune results) hangs
+ *     off an opaque nx_ctx, one per device.
+ *
+ * The reference (MightyBOBcnc/nixis) is pure Python + numba and has no FFI;
+ * each entry point names the reference function (file:line) whose behaviour
+ * it provides.  The Python modules in nixis_amd/ bind these through ctypes
+ * under the reference's own function names and signatures.
+ */
+#ifndef NIXIS_H
+#define NIXIS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NX_OK 0
+#define NX_EINVAL (-1)    /* bad argument (size, null pointer, range)      */
+#define NX_ENOMEM (-2)    /* allocation failed                              */
+#define NX_EHIP (-3)      /* a HIP runtime call failed                      */
+#define NX_ERANGE (-4)    /* input outside the kernel's exact domain        */
+#define NX_EDIV0 (-5)     /* zero input range in rescale                    */
+#define NX_EBUSY (-6)     /* serving queue full / HBM budget exhausted      */
+
+#define NX_MAX_OCTAVES 32 /* octaves per sample_octaves call               */
+
+typedef struct nx_ctx nx_ctx;
+
+/* ---- library ----------------------------------------------------------- */
+
+/* Message for the last failing call on this host thread ("" if none). */
+const char *nx_last_error(void);
+/* ABI version, bumped on any signature change. */
+int nx_abi_version(void);
+
+/* Create / destroy a context bound to HIP device `device`. */
+int nx_ctx_create(int device, nx_ctx **out);
+int nx_ctx_destroy(nx_ctx *ctx);
+/* Synchronise `stream`, then return and clear the context's sticky kernel
+ * error (NX_ERANGE: a noise coordinate outside |x| < 2^30 after the octave
+ * multiplier, or not finite; NX_EDIV0: rescale of a constant array). */
+int nx_ctx_sync_check(nx_ctx *ctx, void *stream);
+
+/* ---- permutation tables (host) ----------------------------------------- */
+
+/* opensimplex.py:90-107 (init) with over() at opensimplex.py:110-112:
+ * perm/pgi are host int32[256]. */
+int nx_init_tables(int64_t seed, int32_t *perm, int32_t *pgi);
+
+/* ---- noise (device) ---------------------------------------------------- */
+
+/* terrain.py:32-59 (sample_octaves) over terrain.py:12-28 (sample_noise) and
+ * opensimplex.py:266-759 (noise3d), all octaves fused in one launch.
+ *   verts     device f64 [V,3]
+ *   perm,pgi  host int32[256] from nx_init_tables
+ *   rough     host f64 [n_octaves]: coordinate multiplier of each octave
+ *             (the reference's n_freq / world_radius)
+ *   strength  host f64 [n_octaves]: n_amp / world_radius
+ *   radius    world_radius
+ *   elev      device f64 [V]; with accumulate != 0 the octaves are added to
+ *             its contents (the reference's elevations argument), otherwise
+ *             it is overwritten.
+ *   dbg_noise device f32 [n_octaves,V] or NULL: per-octave noise3d value
+ *   dbg_sel   device int32 [n_octaves,V,5] or NULL: per-octave lattice
+ *             selection (xsb, ysb, zsb, region/extra-vertex code, checksum
+ *             of the contributing gradient indices)
+ * Lattice skew, floor, region and hash selection run in FP64 / integers and
+ * are bit-exact with the reference; the gradient and falloff arithmetic runs
+ * in FP32 (|error| <= 1e-5 of the FP64 noise value). */
+int nx_sample_octaves(nx_ctx *ctx, const double *verts, int64_t V,
+                      const int32_t *perm, const int32_t *pgi, int n_octaves,
+                      const double *rough, const double *strength,
+                      double radius, double *elev, int accumulate,
+                      float *dbg_noise, int32_t *dbg_sel, void *stream);
+
+/* opensimplex.py:257-263 (noisearr3d) / 266-759 (noise3d) at n points given
+ * as device f64 arrays x, y, z; out is device f64 [n]. */
+int nx_noise3d(nx_ctx *ctx, const double *x, const double *y, const double *z,
+               int64_t n, const int32_t *perm, const int32_t *pgi,
+               double *out, void *stream);
+
+/* ---- rescale (device) -------------------------------------------------- */
+
+/* util.py:110-140 (rescale, standard mode): out = ((x - min) / (max - min))
+ * * (upper - lower) + lower with min/max reduced on the device.  x and out
+ * are device f64 [n] and may alias.  minmax (device f64[2], nullable)
+ * receives (min, max). */
+int nx_rescale(nx_ctx *ctx, const double *x, int64_t n, double lower,
+               double upper, double *out, double *minmax, void *stream);
+
+/* ---- erosion (device) -------------------------------------------------- */
+
+/* Fields live in a caller-allocated device workspace of
+ * nx_erosion_state_bytes(V) bytes: height, water and sediment as FP32
+ * (hi, lo) pairs scaled by a per-planet power of two, double-buffered. */
+size_t nx_erosion_state_bytes(int64_t V);
+
+/* Load FP64 fields (device f64 [V]; w and s may be NULL for zeros). */
+int nx_erosion_load(nx_ctx *ctx, void *state, int64_t V, const double *h,
+                    const double *w, const double *s, void *stream);
+
+/* erosion.py:171-192 (erode_terrain3) over erosion.py:197-279
+ * (erosion_iteration3): `iters` Jacobi steps; before each step `rain` is
+ * added to every vertex's water (the reference's water += 0.3/320; pass 0
+ * for a bare erosion_iteration3).
+ *   col   device int32 [6,V]  neighbour of each slot (self for a pad slot)
+ *   elen  device f32 [6,V,2]  edge length of each slot as (hi, lo) */
+int nx_erosion_run(nx_ctx *ctx, void *state, int64_t V, const int32_t *col,
+                   const float *elen, int iters, double rain, void *stream);
+
+/* Store the current fields back as FP64 (any output may be NULL). */
+int nx_erosion_store(nx_ctx *ctx, const void *state, int64_t V, double *h,
+                     double *w, double *s, void *stream);
+
+/* FP64 mass check: out (device f64[3]) = (sum(h + s), sum(|h| + |s|),
+ * sum(w)).  In exact arithmetic every erosion step keeps h + s per vertex
+ * (erosion.py:262-267), so out[0] is constant up to rounding. */
+int nx_erosion_mass(nx_ctx *ctx, const void *state, int64_t V, double *out,
+                    void *stream);
+
+/* ---- mesh and neighbour tables (host) ---------------------------------- */
+
+/* Counts of the icosphere with `d` divisions: V = 10 d^2 + 2, T = 20 d^2
+ * (util.py:17-50 comment block). */
+int nx_icosphere_counts(int64_t d, int64_t *V, int64_t *T);
+/* util.py:17-50 (create_mesh, meshzoo.icosa_sphere): unit-sphere vertices
+ * host f64 [V,3] (12 corner vertices first) and triangles host int32 [T,3]
+ * (either may be NULL). */
+int nx_icosphere(int64_t d, double *verts, int32_t *tris);
+/* util.py:590-613 (build_adjacency): host int32 [V,6], -1 padded. */
+int nx_build_adjacency(const int32_t *tris, int64_t T, int64_t V,
+                       int32_t *adj);
+/* util.py:638-662 (sort_adjacency): rows into cyclic order, in place. */
+int nx_sort_adjacency(int32_t *adj, int64_t V);
+/* Space-filling-curve (3D Morton) vertex order: order[new] = old. */
+int nx_sfc_order(const double *verts, int64_t V, int64_t *order);
+/* Device neighbour layout from a [V,6] table (host arrays): col int32 [6,V]
+ * and elen f32 [6,V,2]; order (nullable) renumbers vertices as in
+ * nx_sfc_order.  Edge lengths follow erosion.py:38-40 in FP64. */
+int nx_ell_from_adjacency(const int32_t *adj, int64_t V, const double *verts,
+                          const int64_t *order, int32_t *col, float *elen);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NIXIS_H */

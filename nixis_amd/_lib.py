@@ -1,0 +1,166 @@
+"""ctypes binding of libnixis_hip.so (C ABI in include/nixis.h).
+
+The shared library is built in-tree by ``nixis_amd.build``.  There is no
+fallback: if the library is missing, or no HIP device is present when a
+device entry point is used, the call raises.
+"""
+import ctypes
+import os
+import threading
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnixis_hip.so")
+
+NX_OK = 0
+NX_EINVAL = -1
+NX_ENOMEM = -2
+NX_EHIP = -3
+NX_ERANGE = -4
+NX_EDIV0 = -5
+NX_EBUSY = -6
+NX_MAX_OCTAVES = 32
+
+
+class NixisError(RuntimeError):
+    """A libnixis_hip call failed; ``code`` is the NX_E* status."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"{msg} (status {code})")
+        self.code = code
+
+
+class NixisRangeError(NixisError, ValueError):
+    pass
+
+
+class NixisArgError(NixisError, ValueError):
+    pass
+
+
+class NixisBusyError(NixisError):
+    pass
+
+
+_c_void_p = ctypes.c_void_p
+_c_i64 = ctypes.c_int64
+_c_int = ctypes.c_int
+_c_double = ctypes.c_double
+_c_size = ctypes.c_size_t
+
+# name -> (restype, argtypes); one entry per declaration in include/nixis.h
+SIGNATURES = {
+    "nx_last_error": (ctypes.c_char_p, []),
+    "nx_abi_version": (_c_int, []),
+    "nx_ctx_create": (_c_int, [_c_int, ctypes.POINTER(_c_void_p)]),
+    "nx_ctx_destroy": (_c_int, [_c_void_p]),
+    "nx_ctx_sync_check": (_c_int, [_c_void_p, _c_void_p]),
+    "nx_init_tables": (_c_int, [_c_i64, _c_void_p, _c_void_p]),
+    "nx_sample_octaves": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p, _c_void_p, _c_int,
+                                   _c_void_p, _c_void_p, _c_double, _c_void_p, _c_int,
+                                   _c_void_p, _c_void_p, _c_void_p]),
+    "nx_noise3d": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_void_p, _c_i64, _c_void_p,
+                            _c_void_p, _c_void_p, _c_void_p]),
+    "nx_rescale": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_double, _c_double, _c_void_p,
+                            _c_void_p, _c_void_p]),
+    "nx_erosion_state_bytes": (_c_size, [_c_i64]),
+    "nx_erosion_load": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p, _c_void_p,
+                                 _c_void_p, _c_void_p]),
+    "nx_erosion_run": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p, _c_void_p, _c_int,
+                                _c_double, _c_void_p]),
+    "nx_erosion_store": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p, _c_void_p,
+                                  _c_void_p, _c_void_p]),
+    "nx_erosion_mass": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p, _c_void_p]),
+    "nx_icosphere_counts": (_c_int, [_c_i64, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_i64)]),
+    "nx_icosphere": (_c_int, [_c_i64, _c_void_p, _c_void_p]),
+    "nx_build_adjacency": (_c_int, [_c_void_p, _c_i64, _c_i64, _c_void_p]),
+    "nx_sort_adjacency": (_c_int, [_c_void_p, _c_i64]),
+    "nx_sfc_order": (_c_int, [_c_void_p, _c_i64, _c_void_p]),
+    "nx_ell_from_adjacency": (_c_int, [_c_void_p, _c_i64, _c_void_p, _c_void_p, _c_void_p,
+                                       _c_void_p]),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+def lib():
+    """Load libnixis_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        with _lock:
+            if _lib is None:
+                if not os.path.exists(LIB_PATH):
+                    raise ImportError(
+                        f"{LIB_PATH} is missing: build it with `python -m nixis_amd.build` "
+                        "(there is no CPU fallback)")
+                L = ctypes.CDLL(LIB_PATH)
+                for name, (res, args) in SIGNATURES.items():
+                    fn = getattr(L, name)
+                    fn.restype = res
+                    fn.argtypes = args
+                _lib = L
+    return _lib
+
+
+def check(rc):
+    """Raise the matching exception for a non-zero status."""
+    if rc == NX_OK:
+        return
+    msg = lib().nx_last_error().decode(errors="replace")
+    if rc == NX_EDIV0:
+        raise ZeroDivisionError(msg)
+    if rc == NX_ERANGE:
+        raise NixisRangeError(rc, msg)
+    if rc == NX_EINVAL:
+        raise NixisArgError(rc, msg)
+    if rc == NX_ENOMEM:
+        raise MemoryError(msg)
+    if rc == NX_EBUSY:
+        raise NixisBusyError(rc, msg)
+    raise NixisError(rc, msg)
+
+
+class Context:
+    """One nx_ctx per HIP device (created on first use)."""
+
+    _ctxs = {}
+
+    def __init__(self, device):
+        h = _c_void_p()
+        check(lib().nx_ctx_create(int(device), ctypes.byref(h)))
+        self.handle = h
+        self.device = device
+
+    @classmethod
+    def get(cls, device=None):
+        import torch
+        if device is None:
+            device = torch.cuda.current_device()
+        if isinstance(device, torch.device):
+            device = device.index if device.index is not None else torch.cuda.current_device()
+        with _lock:
+            c = cls._ctxs.get(device)
+            if c is None:
+                c = cls(device)
+                cls._ctxs[device] = c
+        return c
+
+    def sync_check(self, stream=None):
+        check(lib().nx_ctx_sync_check(self.handle, stream_ptr(stream)))
+
+
+def stream_ptr(stream=None):
+    """hipStream_t of a torch stream (default: the current stream)."""
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return _c_void_p(stream.cuda_stream)
+
+
+def ptr(t):
+    """Device (or host) address of a tensor / ndarray, None for None."""
+    if t is None:
+        return None
+    if hasattr(t, "data_ptr"):
+        return _c_void_p(t.data_ptr())
+    return _c_void_p(t.ctypes.data)

@@ -1,0 +1,209 @@
+// nx_api.cpp -- extern "C" entry points of libnixis_hip.so (include/nixis.h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "nx_ctx.h"
+
+namespace nx {
+
+static thread_local char g_err[512] = "";
+
+int set_error(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+// opensimplex.py:90-107.  over() is compiled by numba with the explicit
+// signature int32(int32) (opensimplex.py:110-112), so each LCG step keeps only
+// the low 32 bits, sign-extended: s <- int32(s * 6364136223846793005 +
+// 1442695040888963407) with the int64 arithmetic wrapping.  The swap index
+// r = (s + 31) mod (i + 1) is Python's floored modulo.  pgi is
+// int((perm % 24.0) * 3), i.e. 3 * (perm mod 24).
+int init_tables(int64_t seed, int32_t *perm, int32_t *pgi)
+{
+    if (!perm || !pgi) return set_error(NX_EINVAL, "perm/pgi must not be NULL");
+    auto step = [](int64_t s) -> int64_t {
+        const uint64_t t = (uint64_t)s * 6364136223846793005ULL + 1442695040888963407ULL;
+        return (int64_t)(int32_t)(uint32_t)t;
+    };
+    int32_t source[256];
+    for (int i = 0; i < 256; ++i) source[i] = i;
+    int64_t s = step(step(step(seed)));
+    for (int i = 255; i >= 0; --i) {
+        s = step(s);
+        const int64_t m = i + 1;
+        int64_t r = (s + 31) % m;
+        if (r < 0) r += m;
+        perm[i] = source[r];
+        pgi[i] = 3 * (perm[i] % 24);
+        source[r] = source[i];
+    }
+    return NX_OK;
+}
+
+}  // namespace nx
+
+using namespace nx;
+
+extern "C" {
+
+const char *nx_last_error(void) { return g_err; }
+
+int nx_abi_version(void) { return 1; }
+
+int nx_ctx_create(int device, nx_ctx **out)
+{
+    if (!out) return set_error(NX_EINVAL, "out must not be NULL");
+    *out = nullptr;
+    int n = 0;
+    NX_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n)
+        return set_error(NX_EINVAL, "device %d outside 0..%d", device, n - 1);
+    NX_HIP(hipSetDevice(device));
+    nx_ctx *c = new (std::nothrow) nx_ctx();
+    if (!c) return set_error(NX_ENOMEM, "context allocation failed");
+    c->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        c->num_cus = prop.multiProcessorCount;
+    hipError_t e = hipMalloc((void **)&c->d_err, 256);
+    if (e == hipSuccess) e = hipMalloc((void **)&c->d_keys, sizeof(unsigned long long) * 2 * kKeySlots);
+    if (e == hipSuccess) e = hipMemset(c->d_err, 0, 256);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        if (c->d_err) (void)hipFree(c->d_err);
+        if (c->d_keys) (void)hipFree(c->d_keys);
+        delete c;
+        return set_error(NX_EHIP, "context device allocation failed: %s", hipGetErrorString(e));
+    }
+    *out = c;
+    return NX_OK;
+}
+
+int nx_ctx_destroy(nx_ctx *ctx)
+{
+    if (!ctx) return NX_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipDeviceSynchronize();
+    if (ctx->d_err) (void)hipFree(ctx->d_err);
+    if (ctx->d_keys) (void)hipFree(ctx->d_keys);
+    delete ctx;
+    return NX_OK;
+}
+
+int nx_ctx_sync_check(nx_ctx *ctx, void *stream)
+{
+    if (!ctx) return set_error(NX_EINVAL, "null context");
+    NX_HIP(hipSetDevice(ctx->device));
+    NX_HIP(hipStreamSynchronize((hipStream_t)stream));
+    int word = 0;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    NX_HIP(hipMemcpy(&word, ctx->d_err, sizeof(int), hipMemcpyDeviceToHost));
+    if (word) {
+        const int zero = 0;
+        NX_HIP(hipMemcpy(ctx->d_err, &zero, sizeof(int), hipMemcpyHostToDevice));
+    }
+    if (word & NX_ERR_RANGE_BIT)
+        return set_error(NX_ERANGE,
+                         "noise coordinate outside the exact domain (|skewed coordinate| >= 2^30 "
+                         "or not finite)");
+    if (word & NX_ERR_DIV0_BIT) return set_error(NX_EDIV0, "rescale: input range is zero");
+    return NX_OK;
+}
+
+int nx_init_tables(int64_t seed, int32_t *perm, int32_t *pgi) { return init_tables(seed, perm, pgi); }
+
+int nx_sample_octaves(nx_ctx *ctx, const double *verts, int64_t V, const int32_t *perm,
+                      const int32_t *pgi, int n_octaves, const double *rough,
+                      const double *strength, double radius, double *elev, int accumulate,
+                      float *dbg_noise, int32_t *dbg_sel, void *stream)
+{
+    return sample_octaves(ctx, verts, V, perm, pgi, n_octaves, rough, strength, radius, elev,
+                          accumulate, dbg_noise, dbg_sel, (hipStream_t)stream);
+}
+
+int nx_noise3d(nx_ctx *ctx, const double *x, const double *y, const double *z, int64_t n,
+               const int32_t *perm, const int32_t *pgi, double *out, void *stream)
+{
+    return noise3d(ctx, x, y, z, n, perm, pgi, out, (hipStream_t)stream);
+}
+
+int nx_rescale(nx_ctx *ctx, const double *x, int64_t n, double lower, double upper, double *out,
+               double *minmax, void *stream)
+{
+    return rescale(ctx, x, n, lower, upper, out, minmax, (hipStream_t)stream);
+}
+
+size_t nx_erosion_state_bytes(int64_t V) { return erosion_state_bytes(V); }
+
+int nx_erosion_load(nx_ctx *ctx, void *state, int64_t V, const double *h, const double *w,
+                    const double *s, void *stream)
+{
+    return erosion_load(ctx, state, V, h, w, s, (hipStream_t)stream);
+}
+
+int nx_erosion_run(nx_ctx *ctx, void *state, int64_t V, const int32_t *col, const float *elen,
+                   int iters, double rain, void *stream)
+{
+    return erosion_run(ctx, state, V, col, elen, iters, rain, (hipStream_t)stream);
+}
+
+int nx_erosion_store(nx_ctx *ctx, const void *state, int64_t V, double *h, double *w, double *s,
+                     void *stream)
+{
+    return erosion_store(ctx, state, V, h, w, s, (hipStream_t)stream);
+}
+
+int nx_erosion_mass(nx_ctx *ctx, const void *state, int64_t V, double *out, void *stream)
+{
+    return erosion_mass(ctx, state, V, out, (hipStream_t)stream);
+}
+
+int nx_icosphere_counts(int64_t d, int64_t *V, int64_t *T)
+{
+    const int rc = icosphere_counts(d, V, T);
+    return rc ? set_error(rc, "divisions %lld outside 1..100000", (long long)d) : NX_OK;
+}
+
+int nx_icosphere(int64_t d, double *verts, int32_t *tris)
+{
+    const int rc = icosphere(d, verts, tris);
+    return rc ? set_error(rc, "divisions %lld outside 1..14650 (V must fit int32)", (long long)d)
+              : NX_OK;
+}
+
+int nx_build_adjacency(const int32_t *tris, int64_t T, int64_t V, int32_t *adj)
+{
+    const int rc = build_adjacency(tris, T, V, adj);
+    return rc ? set_error(rc, "build_adjacency: bad triangle list (index or degree > 6)") : NX_OK;
+}
+
+int nx_sort_adjacency(int32_t *adj, int64_t V)
+{
+    const int rc = sort_adjacency(adj, V);
+    return rc ? set_error(rc, "sort_adjacency: a row is not a closed ring") : NX_OK;
+}
+
+int nx_sfc_order(const double *verts, int64_t V, int64_t *order)
+{
+    const int rc = sfc_order(verts, V, order);
+    return rc ? set_error(rc, "sfc_order: bad arguments") : NX_OK;
+}
+
+int nx_ell_from_adjacency(const int32_t *adj, int64_t V, const double *verts, const int64_t *order,
+                          int32_t *col, float *elen)
+{
+    const int rc = ell_from_adjacency(adj, V, verts, order, col, elen);
+    return rc ? set_error(rc, "ell_from_adjacency: bad table or order") : NX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,165 @@
+// nx_noise.hip -- fused multi-octave OpenSimplex noise over icosphere vertices.
+//
+// terrain.py:32-59 (sample_octaves) calls terrain.py:12-28 (sample_noise)
+// once per octave; each call rescales all coordinates, evaluates noise3d at
+// every vertex and adds ((e + 1) * 0.5 * strength) * radius to the
+// elevations.  Here one lane owns one vertex for all octaves: coordinates
+// are read once, each octave's coordinate scale, lattice selection and the
+// weighting/accumulation stay in FP64 in the reference's order, and only the
+// final elevation is written.  Tables live in LDS (see nx_noise_dev.h).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+
+#include "nx_ctx.h"
+#include "nx_noise_dev.h"
+
+namespace nx {
+
+struct OctaveArgs {
+    NoiseTablesPacked tables;
+    double rough[NX_MAX_OCTAVES];
+    double strength[NX_MAX_OCTAVES];
+    double radius;
+    int n_oct;
+    int accumulate;
+};
+
+template <bool DBG>
+__global__ __launch_bounds__(256) void k_sample_octaves(OctaveArgs a, const double *__restrict__ verts,
+                                                        int64_t V, double *__restrict__ elev,
+                                                        float *__restrict__ dbg_noise,
+                                                        int32_t *__restrict__ dbg_sel,
+                                                        int *__restrict__ err)
+{
+    __shared__ NoiseTables T;
+    load_tables(T, a.tables);
+    __syncthreads();
+    bool bad = false;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < V; i += stride) {
+        const double x = verts[3 * i + 0], y = verts[3 * i + 1], z = verts[3 * i + 2];
+        double acc = a.accumulate ? elev[i] : 0.0;
+        for (int o = 0; o < a.n_oct; ++o) {
+            const double r = a.rough[o];
+            NoiseSel sel;
+            const float n = noise3<DBG>(x * r, y * r, z * r, T, bad, &sel);
+            // terrain.py:28 then terrain.py:43
+            acc = acc + (((double)n + 1) * 0.5 * a.strength[o]) * a.radius;
+            if (DBG) {
+                if (dbg_noise) dbg_noise[(int64_t)o * V + i] = n;
+                if (dbg_sel) {
+                    int32_t *s = dbg_sel + ((int64_t)o * V + i) * 5;
+                    for (int k = 0; k < 5; ++k) s[k] = sel.v[k];
+                }
+            }
+        }
+        elev[i] = acc;
+    }
+    if (bad) __hip_atomic_fetch_or(err, NX_ERR_RANGE_BIT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ __launch_bounds__(256) void k_noise3d(NoiseTablesPacked tp, const double *__restrict__ x,
+                                                 const double *__restrict__ y,
+                                                 const double *__restrict__ z, int64_t n,
+                                                 double *__restrict__ out, int *__restrict__ err)
+{
+    __shared__ NoiseTables T;
+    load_tables(T, tp);
+    __syncthreads();
+    bool bad = false;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        out[i] = (double)noise3<false>(x[i], y[i], z[i], T, bad, nullptr);
+    if (bad) __hip_atomic_fetch_or(err, NX_ERR_RANGE_BIT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Pack perm / pgi (host int32[256] each) into the kernel-argument form.  The
+// reference indexes both tables without bounds checks (opensimplex.py:125);
+// values outside 0..255 (perm) or outside the 24 gradient offsets (pgi)
+// would read past its arrays, so they are rejected here.
+int pack_tables(const int32_t *perm, const int32_t *pgi, NoiseTablesPacked &p)
+{
+    if (!perm || !pgi) return set_error(NX_EINVAL, "perm/pgi must not be NULL");
+    uint8_t pb[256], gb[256];
+    for (int i = 0; i < 256; ++i) {
+        if (perm[i] < 0 || perm[i] > 255)
+            return set_error(NX_EINVAL, "perm[%d]=%d outside 0..255", i, perm[i]);
+        if (pgi[i] < 0 || pgi[i] > 69 || pgi[i] % 3 != 0)
+            return set_error(NX_EINVAL, "pgi[%d]=%d is not a gradient index", i, pgi[i]);
+        pb[i] = (uint8_t)perm[i];
+        gb[i] = (uint8_t)(pgi[i] / 3);
+    }
+    std::memcpy(p.perm, pb, 256);
+    std::memcpy(p.grad, gb, 256);
+    // opensimplex.py:51-60: gradient g = 3s + k has magnitude 11 on axis k and
+    // 4 on the others; the x sign is + for odd s, y and z signs are - when
+    // bit 1 / bit 2 of s is set.
+    for (int g = 0; g < 24; ++g) {
+        const int s = g / 3, k = g % 3;
+        const int gx = (k == 0 ? 11 : 4) * ((s & 1) ? 1 : -1);
+        const int gy = (k == 1 ? 11 : 4) * ((s & 2) ? -1 : 1);
+        const int gz = (k == 2 ? 11 : 4) * ((s & 4) ? -1 : 1);
+        p.gvec[g] = (gx & 0xff) | ((gy & 0xff) << 8) | ((gz & 0xff) << 16);
+    }
+    return NX_OK;
+}
+
+static int grid_for(nx_ctx *ctx, int64_t n, int per_cu)
+{
+    const int64_t blocks = (n + 255) / 256;
+    const int64_t cap = (int64_t)ctx->num_cus * per_cu;
+    return (int)std::max<int64_t>(1, std::min(blocks, cap));
+}
+
+int sample_octaves(nx_ctx *ctx, const double *verts, int64_t V, const int32_t *perm,
+                   const int32_t *pgi, int n_oct, const double *rough, const double *strength,
+                   double radius, double *elev, int accumulate, float *dbg_noise,
+                   int32_t *dbg_sel, hipStream_t stream)
+{
+    if (!ctx) return set_error(NX_EINVAL, "null context");
+    if (V < 0 || (V > 0 && (!verts || !elev))) return set_error(NX_EINVAL, "bad verts/elev");
+    if (n_oct < 0 || n_oct > NX_MAX_OCTAVES)
+        return set_error(NX_EINVAL, "n_octaves=%d outside 0..%d", n_oct, NX_MAX_OCTAVES);
+    if (n_oct > 0 && (!rough || !strength)) return set_error(NX_EINVAL, "null octave scalars");
+    OctaveArgs a;
+    std::memset(&a, 0, sizeof(a));
+    int rc = pack_tables(perm, pgi, a.tables);
+    if (rc) return rc;
+    for (int o = 0; o < n_oct; ++o) {
+        a.rough[o] = rough[o];
+        a.strength[o] = strength[o];
+    }
+    a.radius = radius;
+    a.n_oct = n_oct;
+    a.accumulate = accumulate ? 1 : 0;
+    if (V == 0) return NX_OK;
+    NX_HIP(hipSetDevice(ctx->device));
+    const int grid = grid_for(ctx, V, 8);
+    if (dbg_noise || dbg_sel)
+        hipLaunchKernelGGL(k_sample_octaves<true>, dim3(grid), dim3(256), 0, stream, a, verts, V,
+                           elev, dbg_noise, dbg_sel, ctx->d_err);
+    else
+        hipLaunchKernelGGL(k_sample_octaves<false>, dim3(grid), dim3(256), 0, stream, a, verts, V,
+                           elev, nullptr, nullptr, ctx->d_err);
+    NX_HIP(hipGetLastError());
+    return NX_OK;
+}
+
+int noise3d(nx_ctx *ctx, const double *x, const double *y, const double *z, int64_t n,
+            const int32_t *perm, const int32_t *pgi, double *out, hipStream_t stream)
+{
+    if (!ctx) return set_error(NX_EINVAL, "null context");
+    if (n < 0 || (n > 0 && (!x || !y || !z || !out))) return set_error(NX_EINVAL, "bad arrays");
+    NoiseTablesPacked tp;
+    int rc = pack_tables(perm, pgi, tp);
+    if (rc) return rc;
+    if (n == 0) return NX_OK;
+    NX_HIP(hipSetDevice(ctx->device));
+    hipLaunchKernelGGL(k_noise3d, dim3(grid_for(ctx, n, 8)), dim3(256), 0, stream, tp, x, y, z, n,
+                       out, ctx->d_err);
+    NX_HIP(hipGetLastError());
+    return NX_OK;
+}
+
+}  // namespace nx

@@ -1,0 +1,169 @@
+"""Drop-in for the reference's erosion.py (version 3 of the erosion step).
+
+``erode_terrain3`` (erosion.py:171-192) and ``erosion_iteration3``
+(erosion.py:197-279) keep their signatures and in-place semantics; the
+Jacobi steps run on the GPU with float-float FP32 fields (see
+nixis_amd/csrc/nx_erosion.hip).  ``ErosionGraph`` / ``ErosionState`` are the
+device-resident forms used by the batched and served paths.
+"""
+import math
+
+import numpy as np
+
+from . import _lib
+
+RAIN_DEFAULT = 0.3 / 320  # erosion.py:182
+
+
+def calc_slope(v0, v1, dist):
+    """erosion.py:34-36."""
+    return (v1 - v0) / (dist + 0.00001)
+
+
+def calc_distance(v0, v1):
+    """erosion.py:38-40."""
+    return math.sqrt((v0[0] - v1[0]) ** 2 + (v0[1] - v1[1]) ** 2 + (v0[2] - v1[2]) ** 2)
+
+
+class ErosionGraph:
+    """Device neighbour table: col int32 [6,V] and float-float elen [6,V,2].
+
+    Built on the host from a reference-style neighbour table ([V,6] int32,
+    -1 padded) and FP64 vertex coordinates; `order` optionally renumbers the
+    vertices (space-filling-curve order), in which case fields passed to
+    ErosionState must use the new numbering (see ``permute``/``unpermute``).
+    """
+
+    def __init__(self, verts, neighbors, order=None, device=None):
+        import torch
+        verts = np.ascontiguousarray(verts, dtype=np.float64)
+        nb = np.ascontiguousarray(neighbors, dtype=np.int32)
+        if verts.ndim != 2 or verts.shape[1] != 3 or nb.ndim != 2 or nb.shape[1] != 6 \
+                or nb.shape[0] != verts.shape[0]:
+            raise ValueError("verts must be [V,3] float64 and neighbors [V,6] int32")
+        V = nb.shape[0]
+        col = np.empty((6, V), dtype=np.int32)
+        elen = np.empty((6, V, 2), dtype=np.float32)
+        o = None if order is None else np.ascontiguousarray(order, dtype=np.int64)
+        _lib.check(_lib.lib().nx_ell_from_adjacency(_lib.ptr(nb), V, _lib.ptr(verts),
+                                                    _lib.ptr(o), _lib.ptr(col), _lib.ptr(elen)))
+        if device is None:
+            device = torch.device("cuda", torch.cuda.current_device())
+        self.V = V
+        self.order = o
+        self.col = torch.from_numpy(col).to(device)
+        self.elen = torch.from_numpy(elen).to(device)
+        self.device = self.col.device
+
+    @property
+    def nbytes(self):
+        return self.col.numel() * 4 + self.elen.numel() * 4
+
+    def permute(self, a):
+        """Old-numbered per-vertex array -> new numbering."""
+        return a if self.order is None else a[self.order]
+
+    def unpermute(self, a):
+        """New-numbered per-vertex array -> old numbering."""
+        if self.order is None:
+            return a
+        out = np.empty_like(a)
+        out[self.order] = a
+        return out
+
+
+class ErosionState:
+    """Device-resident h/w/s (float-float, double-buffered) of one planet."""
+
+    def __init__(self, V, device=None):
+        import torch
+        if device is None:
+            device = torch.device("cuda", torch.cuda.current_device())
+        self.V = int(V)
+        nbytes = int(_lib.lib().nx_erosion_state_bytes(self.V))
+        # 256-byte alignment of the workspace: torch's caching allocator
+        # returns 512-byte aligned blocks.
+        self.buf = torch.empty(nbytes, dtype=torch.uint8, device=device)
+        if self.buf.data_ptr() % 256:
+            raise RuntimeError("erosion workspace is not 256-byte aligned")
+        self.device = self.buf.device
+        self.ctx = _lib.Context.get(self.device)
+
+    @property
+    def nbytes(self):
+        return self.buf.numel()
+
+    def load(self, h, w=None, s=None, stream=None):
+        """h, w, s: float64 [V] device tensors (w, s None for zeros)."""
+        for t in (h, w, s):
+            if t is not None and (t.dtype.itemsize != 8 or t.numel() != self.V
+                                  or not t.is_contiguous()):
+                raise ValueError("fields must be contiguous float64 [V] tensors")
+        _lib.check(_lib.lib().nx_erosion_load(self.ctx.handle, _lib.ptr(self.buf), self.V,
+                                              _lib.ptr(h), _lib.ptr(w), _lib.ptr(s),
+                                              _lib.stream_ptr(stream)))
+
+    def run(self, graph, iters, rain=RAIN_DEFAULT, stream=None):
+        if graph.V != self.V:
+            raise ValueError("graph and state sizes differ")
+        _lib.check(_lib.lib().nx_erosion_run(self.ctx.handle, _lib.ptr(self.buf), self.V,
+                                             _lib.ptr(graph.col), _lib.ptr(graph.elen),
+                                             int(iters), float(rain), _lib.stream_ptr(stream)))
+
+    def store(self, h=None, w=None, s=None, stream=None):
+        _lib.check(_lib.lib().nx_erosion_store(self.ctx.handle, _lib.ptr(self.buf), self.V,
+                                               _lib.ptr(h), _lib.ptr(w), _lib.ptr(s),
+                                               _lib.stream_ptr(stream)))
+
+    def mass(self, stream=None):
+        """(sum(h+s), sum(|h|+|s|), sum(w)) in FP64, as a device tensor [3]."""
+        import torch
+        out = torch.empty(3, dtype=torch.float64, device=self.device)
+        _lib.check(_lib.lib().nx_erosion_mass(self.ctx.handle, _lib.ptr(self.buf), self.V,
+                                              _lib.ptr(out), _lib.stream_ptr(stream)))
+        return out
+
+
+def _dev(a, graph):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(graph.permute(a), dtype=np.float64)).to(
+        graph.device)
+
+
+def erode_terrain3(nodes, neighbors, heights, num_iter=1, snapshot=False, rain=RAIN_DEFAULT):
+    """erosion.py:171-192: erode ``heights`` (float64[V]) in place.
+
+    Water and sediment start at zero and are discarded, as in the reference.
+    ``snapshot`` image export is not part of this path and is ignored.
+    ``rain`` is the per-iteration rainfall (the reference hard-codes 0.3/320).
+    """
+    if heights.dtype != np.float64:
+        raise TypeError("heights must be float64")
+    if num_iter <= 0:
+        num_iter = 1
+    graph = ErosionGraph(nodes, neighbors)
+    st = ErosionState(graph.V, graph.device)
+    st.load(_dev(heights, graph))
+    st.run(graph, num_iter, rain)
+    import torch
+    h = torch.empty(graph.V, dtype=torch.float64, device=graph.device)
+    st.store(h)
+    st.ctx.sync_check()
+    heights[...] = graph.unpermute(h.cpu().numpy())
+
+
+def erosion_iteration3(verts, neighbors, r_buff, wat, sed):
+    """erosion.py:197-279: one step; r_buff, wat, sed (float64[V]) updated in place."""
+    import torch
+    for a in (r_buff, wat, sed):
+        if a.dtype != np.float64:
+            raise TypeError("fields must be float64")
+    graph = ErosionGraph(verts, neighbors)
+    st = ErosionState(graph.V, graph.device)
+    st.load(_dev(r_buff, graph), _dev(wat, graph), _dev(sed, graph))
+    st.run(graph, 1, 0.0)
+    outs = [torch.empty(graph.V, dtype=torch.float64, device=graph.device) for _ in range(3)]
+    st.store(*outs)
+    st.ctx.sync_check()
+    for a, t in zip((r_buff, wat, sed), outs):
+        a[...] = graph.unpermute(t.cpu().numpy())
