@@ -51,7 +51,23 @@ def parse():
     ap.add_argument("--erosion-iters", type=int, default=500)
     ap.add_argument("--noise-d", type=int, default=2000)
     ap.add_argument("--erosion-d", type=int, default=1000)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run three plain 1 GiB device copies (for scaling "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
     return ap.parse_args()
+
+
+def calibrate_copy():
+    """Three device-to-device copies of 1 GiB: a kernel whose HBM traffic is
+    known (1 GiB read + 1 GiB written per launch)."""
+    import torch
+    n = (1 << 30) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_copy_bytes": n * 4, "launches": 3}), flush=True)
 
 
 def bench_noise(args):
@@ -84,7 +100,7 @@ def bench_noise(args):
     log(f"[noise] {ms:.3f} ms/step (events), wall {wall / args.steps * 1e3:.3f} ms/step, "
         f"{gvo:.1f} Gvo/s")
     return dict(V=V, vo=vo, ms=ms, gvo=gvo, pts=pts, perm=perm, pgi=pgi, n_oct=n_oct, f0=f0,
-                lac=lac, pers=pers)
+                lac=lac, pers=pers, out=out.cpu().numpy())
 
 
 def cpu_noise_baseline(nz, budget_s=25.0):
@@ -104,11 +120,19 @@ def cpu_noise_baseline(nz, budget_s=25.0):
         times.append(time.perf_counter() - t)
     med = statistics.median(times)
     val = nz["vo"] / med / 1e9
+    # the last oracle run's elevations against the GPU's on the same planet
+    ref = O.sample_octaves(pts[::101], None, nz["perm"], nz["pgi"], nz["n_oct"], nz["f0"], 0.4,
+                           nz["lac"], nz["pers"], 1.0)
+    gpu = nz["out"][::101]
+    bound = 0.5 * 0.4 * sum(nz["pers"] ** k for k in range(nz["n_oct"])) * 1e-5
+    parity = {"vertices_checked": int(ref.size), "max_abs_elev_err": float(np.abs(gpu - ref).max()),
+              "bound": bound}
     log(f"[cpu] oracle noise {val:.3f} Gvo/s over {len(times)} runs, {O.num_threads()} threads")
     return {"value": round(val, 4), "unit": "Gvertex-octaves/s", "cores": O.num_threads(),
             "kind": "port",
             "sample": f"full noise workload ({nz['V']} vertices x {nz['n_oct']} octaves), "
-                      f"median of {len(times)} warm runs, FP64 oracle with OpenMP"}
+                      f"median of {len(times)} warm runs, FP64 oracle with OpenMP",
+            "parity_spot_check": parity}
 
 
 def bench_erosion(args):
@@ -196,6 +220,9 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a HIP device", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     nz = bench_noise(args)
     achieved = NOISE_FLOPS * nz["gvo"] * 1e9 / 1e12
     line = {

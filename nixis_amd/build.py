@@ -21,7 +21,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("NX_OFFLOAD_ARCH", "gfx950")
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
-          "-Wno-unused-function", "-I", os.path.join(HERE, "..", "include")]
+          "-Wno-unused-function", "-Wno-bitwise-instead-of-logical", "-I", os.path.join(HERE, "..", "include")]
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
           "-packed-fp32-ops"]
 

@@ -11,11 +11,13 @@
 //     edge lengths elen[6][V] (coalesced per slot);
 //   * the water/sediment magnitudes grow about 2x per step (the vertex's own
 //     water is re-added each step, erosion.py:215,264), so all fields share a
-//     per-planet power-of-two exponent E: each step reads the previous
-//     step's max |value| (block max + one atomic per block), renormalises by
-//     2^-64 when it passes 2^64 (exact: the update is homogeneous in
-//     (h, w, s) and branch decisions only compare heights), and the rain
-//     is added at scale 2^-E;
+//     per-planet power-of-two exponent E.  A step whose outputs exceed 2^64
+//     anywhere raises a flag (one wave ballot, then a plain store -- no
+//     read-modify-write atomics on the hot path); the next step then scales
+//     its inputs by 2^-64 (exact: the update is homogeneous in (h, w, s)
+//     and branch decisions only compare heights) and adds the rain at scale
+//     2^-E.  Magnitudes never shrink across the planet, so no scale-up path
+//     is needed; loading normalises the largest input to [1, 2);
 //   * the next step's rain is folded into this step's water output.
 // Workspace: [meta 256 B][hw 2 x V x float4][s 2 x V x float2].
 #include <hip/hip_runtime.h>
@@ -28,7 +30,7 @@ namespace nx {
 struct ErosionMeta {
     int64_t it;           // steps taken so far (selects buffers and slots)
     int32_t E[3];         // exponent after step it-1, rotating by step % 3
-    uint32_t maxbits[3];  // max |hi| (float bits) of step outputs, same rotation
+    uint32_t big[3];      // 1 if some output of that step exceeded 2^64
     uint64_t load_max;    // FP64 max |input| bits while loading
 };
 
@@ -63,30 +65,16 @@ static ErosionBufs bufs(void *state, int64_t V)
     return b;
 }
 
-// 2^64 and 2^-64 as float bit patterns: the renormalisation thresholds.
-constexpr uint32_t kBitsHigh = 0x5f800000u;
-constexpr uint32_t kBitsLow = 0x1f800000u;
+// Outputs above this magnitude trigger a 2^-64 rescale of the next step.
+constexpr float kBig = 18446744073709551616.0f;  // 2^64
 
-__device__ __forceinline__ int rescale_shift(uint32_t maxbits)
+// Raise the step's overflow-guard flag if any lane of the wave holds a value
+// above 2^64: one ballot per wave and a coherent store of the constant 1 by
+// the first lane (concurrent stores of the same value need no atomic RMW).
+__device__ __forceinline__ void flag_big(bool over, uint32_t *flag)
 {
-    if (maxbits > kBitsHigh) return 64;
-    if (maxbits != 0 && maxbits < kBitsLow) return -64;
-    return 0;
-}
-
-// Block-wide max of a non-negative float (as bits), one atomic per block.
-__device__ __forceinline__ void block_max_atomic(float v, uint32_t *dst)
-{
-    __shared__ float red[16];
-    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) red[wave] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        float m = red[0];
-        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmaxf(m, red[w]);
-        __hip_atomic_fetch_max(dst, __float_as_uint(m), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    if (__any(over) && (threadIdx.x & 63) == 0)
+        __hip_atomic_store(flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Logical block index whose ranges are contiguous per XCD: the hardware
@@ -115,13 +103,13 @@ __global__ __launch_bounds__(256) void k_erode_step(ErosionMeta *__restrict__ me
     float4 *hw_dst = (it & 1) ? hw0 : hw1;
     float2 *s_dst = (it & 1) ? s0 : s1;
     const int cur = (int)(it % 3), prev = (int)((it + 2) % 3), nxt = (int)((it + 1) % 3);
-    const int sh = rescale_shift(meta->maxbits[prev]);
+    const int sh = meta->big[prev] ? 64 : 0;
     const int E = meta->E[prev] + sh;
     const float sc = ldexpf(1.0f, -sh);
     const int64_t tile = xcd_tile(blockIdx.x, nblocks);
     if (tile == 0 && threadIdx.x == 0) {
         meta->E[cur] = E;
-        meta->maxbits[nxt] = 0u;
+        meta->big[nxt] = 0u;
     }
     float m = 0.0f;
     const int64_t i = tile * blockDim.x + threadIdx.x;
@@ -147,7 +135,7 @@ __global__ __launch_bounds__(256) void k_erode_step(ErosionMeta *__restrict__ me
         s_dst[i] = make_float2(so.hi, so.lo);
         m = fmaxf(fabsf(ho.hi), fmaxf(fabsf(wo.hi), fabsf(so.hi)));
     }
-    block_max_atomic(m, &meta->maxbits[cur]);
+    flag_big(m > kBig, &meta->big[cur]);
 }
 
 // Add rain (scaled by the current exponent) to every vertex's water in the
@@ -171,7 +159,7 @@ __global__ __launch_bounds__(256) void k_add_rain(ErosionMeta *__restrict__ meta
         hw[i] = v;
         m = fabsf(w.hi);
     }
-    block_max_atomic(m, &meta->maxbits[prev]);
+    flag_big(m > kBig, &meta->big[prev]);
 }
 
 // Advance the step counter after `iters` launches (one thread).
@@ -212,9 +200,9 @@ __global__ __launch_bounds__(256) void k_load(ErosionMeta *__restrict__ meta,
     const int E = (mx > 0.0 && isfinite(mx)) ? ilogb(mx) : 0;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        meta->E[2] = E;  // exponent "after step -1"
-        meta->maxbits[2] = 0x3f800000u;  // max ~ 1 after scaling: no shift
-        meta->maxbits[0] = 0u;
+        meta->E[2] = E;  // exponent "after step -1"; max input now in [1, 2)
+        meta->big[2] = 0u;
+        meta->big[0] = 0u;
         meta->it = 0;
     }
     if (i < V) {
@@ -232,7 +220,7 @@ __global__ void k_meta_clear(ErosionMeta *meta)
     meta->load_max = 0;
     for (int k = 0; k < 3; ++k) {
         meta->E[k] = 0;
-        meta->maxbits[k] = 0;
+        meta->big[k] = 0;
     }
 }
 

@@ -94,13 +94,14 @@ int pack_tables(const int32_t *perm, const int32_t *pgi, NoiseTablesPacked &p)
     std::memcpy(p.grad, gb, 256);
     // opensimplex.py:51-60: gradient g = 3s + k has magnitude 11 on axis k and
     // 4 on the others; the x sign is + for odd s, y and z signs are - when
-    // bit 1 / bit 2 of s is set.
+    // bit 1 / bit 2 of s is set.  Stored as fp16 (exact for +-4, +-11).
     for (int g = 0; g < 24; ++g) {
         const int s = g / 3, k = g % 3;
         const int gx = (k == 0 ? 11 : 4) * ((s & 1) ? 1 : -1);
         const int gy = (k == 1 ? 11 : 4) * ((s & 2) ? -1 : 1);
         const int gz = (k == 2 ? 11 : 4) * ((s & 4) ? -1 : 1);
-        p.gvec[g] = (gx & 0xff) | ((gy & 0xff) << 8) | ((gz & 0xff) << 16);
+        p.g16[2 * g] = half_of_small(gx) | (half_of_small(gy) << 16);
+        p.g16[2 * g + 1] = half_of_small(gz);
     }
     return NX_OK;
 }

@@ -15,7 +15,9 @@
 //     tetrahedron at 111 keeps the 2-bit corners, region 2 = octahedron
 //     keeps all six);
 //   - one "origin" slot holds 000 (region 0) or 111 (region 1);
-//   - two slots hold the extra vertices chosen by the region logic.
+//   - two slots hold the extra vertices chosen by the region logic, whose
+//     lattice offsets and delta constants come from a small table keyed by
+//     the region's case and closest-point masks.
 // The hash prefixes perm[x], perm[perm[x]+y] are shared between corners.
 #pragma once
 
@@ -32,69 +34,204 @@
 
 namespace nx {
 
-// Sign-extended 8-bit field of a packed word.
-NX_ND int sbyte(int v, int shift)
-{
-#if defined(__HIP_DEVICE_COMPILE__)
-    return __builtin_amdgcn_sbfe(v, shift, 8);
-#else
-    return (int)(int8_t)(uint8_t)((unsigned)v >> shift);
-#endif
-}
+// Extra-vertex table.  The region logic of opensimplex.py:302-686 reduces
+// to one of seven cases and the bit masks of the closest points; `key`
+// (case * 64 + mask) selects the two extra lattice offsets (each component
+// in -1..2, packed as offset+1 in two bits: ext0 in bits 0-5, ext1 in bits
+// 6-11) and their FP32 delta constants o + (ox + oy + oz) / 3.
+//   case 0: tetrahedron at 000, (0,0,0) among the closest two (mask = c)
+//   case 1: tetrahedron at 000, otherwise (mask = a | b)
+//   case 2: tetrahedron at 111, (1,1,1) among the closest two (mask = c)
+//   case 3: tetrahedron at 111, otherwise (mask = a & b)
+//   case 4: octahedron, both closest on the (1,1,1) side (mask = a & b)
+//   case 5: octahedron, both on the (0,0,0) side (mask = a | b)
+//   case 6: octahedron, one on each side (mask = far * 8 + near)
+struct ExtEntry {
+    int32_t code;
+    float c0x, c0y, c0z;
+    int32_t pad;
+    float c1x, c1y, c1z;
+};
+constexpr int kExtKeys = 7 * 64;
 
-// Tables in LDS: perm as bytes, the gradient index (pgi / 3, 0..23) of every
-// hash value as bytes, and the 24 gradient vectors packed as signed bytes.
-struct NoiseTables {
-    uint8_t perm[256];
-    uint8_t grad[256];
-    int32_t gvec[24];
+struct ExtLut {
+    ExtEntry e[kExtKeys];
 };
 
-// Host-visible packed form passed in kernel arguments.
+constexpr int first_axis(int m) { return (m & 1) ? 0 : ((m & 2) ? 1 : 2); }
+constexpr int first_clear(int m) { return !(m & 1) ? 0 : (!(m & 2) ? 1 : 2); }
+
+constexpr ExtEntry ext_entry(int k, int m)
+{
+    int e0[3] = {0, 0, 0}, e1[3] = {0, 0, 0};
+    const int cx = m & 1, cy = (m >> 1) & 1, cz = (m >> 2) & 1;
+    switch (k) {
+    case 0:  // single-bit c (opensimplex.py:317-351)
+        e0[0] = cx ? 1 : -1;
+        e1[0] = cx ? 1 : 0;
+        e0[1] = cy ? 1 : (cx ? -1 : 0);
+        e1[1] = cy ? 1 : (cx ? 0 : -1);
+        e0[2] = cz ? 1 : 0;
+        e1[2] = cz ? 1 : -1;
+        break;
+    case 1:  // two-bit c (opensimplex.py:353-384)
+        for (int a = 0; a < 3; ++a) {
+            e0[a] = (m >> a) & 1;
+            e1[a] = ((m >> a) & 1) ? 1 : -1;
+        }
+        break;
+    case 2:  // two-bit c (opensimplex.py:434-468)
+        e0[0] = cx ? 2 : 0;
+        e1[0] = cx ? 1 : 0;
+        e0[1] = cy ? (cx ? 1 : 2) : 0;
+        e1[1] = cy ? (cx ? 2 : 1) : 0;
+        e0[2] = cz ? 1 : 0;
+        e1[2] = cz ? 2 : 0;
+        break;
+    case 3:  // single-bit c (opensimplex.py:470-501)
+        for (int a = 0; a < 3; ++a) {
+            e0[a] = (m >> a) & 1;
+            e1[a] = 2 * ((m >> a) & 1);
+        }
+        break;
+    case 4:  // (1,1,1) and twice the shared axis (opensimplex.py:579-610)
+        e0[0] = e0[1] = e0[2] = 1;
+        e1[first_axis(m)] = 2;
+        break;
+    case 5:  // (0,0,0) and -1 on the omitted axis (opensimplex.py:611-641)
+        for (int a = 0; a < 3; ++a) e1[a] = 1;
+        e1[first_clear(m)] = -1;
+        break;
+    case 6: {  // (opensimplex.py:642-686)
+        const int far = (m >> 3) & 7, near = m & 7;
+        for (int a = 0; a < 3; ++a) e0[a] = 1;
+        e0[first_clear(far)] = -1;
+        e1[first_axis(near)] = 2;
+        break;
+    }
+    default:
+        break;
+    }
+    ExtEntry r{};
+    r.code = (e0[0] + 1) | (e0[1] + 1) << 2 | (e0[2] + 1) << 4 | (e1[0] + 1) << 6 |
+             (e1[1] + 1) << 8 | (e1[2] + 1) << 10;
+    const float S = 1.0f / 3.0f;
+    const float t0 = S * (float)(e0[0] + e0[1] + e0[2]);
+    const float t1 = S * (float)(e1[0] + e1[1] + e1[2]);
+    r.c0x = (float)e0[0] + t0;
+    r.c0y = (float)e0[1] + t0;
+    r.c0z = (float)e0[2] + t0;
+    r.c1x = (float)e1[0] + t1;
+    r.c1y = (float)e1[1] + t1;
+    r.c1z = (float)e1[2] + t1;
+    return r;
+}
+
+constexpr ExtLut make_ext_lut()
+{
+    ExtLut t{};
+    for (int k = 0; k < 7; ++k)
+        for (int m = 0; m < 64; ++m) t.e[k * 64 + m] = ext_entry(k, m);
+    return t;
+}
+
+// Tables in LDS.  perm and the gradient index (pgi / 3, 0..23) of every hash
+// value are stored as bytes with wrap-around padding: entry i + 1 holds
+// table[i mod 256] for i in -1..(kWrap - 2), so every hash index used below
+// (lattice coordinate mod 256 plus an offset in -1..2, plus a table value)
+// lands in range without masking.  Then the 24 gradient vectors as fp16
+// (gx, gy | gz, 0) and the extra-vertex table.
+constexpr int kWrap = 520;
+struct NoiseTables {
+    uint8_t perm[kWrap];
+    uint8_t grad[kWrap];
+    uint32_t g16[24][2];
+    ExtEntry ext[kExtKeys];
+};
+
+// Seed-dependent part passed in kernel arguments.
 struct NoiseTablesPacked {
     uint32_t perm[64];
     uint32_t grad[64];
-    int32_t gvec[24];
+    uint32_t g16[48];
 };
+
+// fp16 bits of the gradient components used by opensimplex.py:51-60.
+inline uint32_t half_of_small(int v)
+{
+    const uint32_t sign = v < 0 ? 0x8000u : 0u;
+    const int a = v < 0 ? -v : v;
+    return sign | (a == 11 ? 0x4980u : 0x4400u);  // 11 = 1.375*2^3, 4 = 2^2
+}
 
 // Exact domain of the FP64 section: |skewed coordinate| < 2^30 keeps every
 // lattice integer representable in int32 with margin.
 constexpr double kNoiseCoordLimit = 1073741824.0;
 
+NX_ND float half_to_float(uint32_t bits)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return (float)__builtin_bit_cast(_Float16, (unsigned short)bits);
+#else
+    const uint32_t sign = (bits >> 15) & 1u, ex = (bits >> 10) & 31u, man = bits & 1023u;
+    const float v = ex ? std::ldexp(1.0f + (float)man / 1024.0f, (int)ex - 15)
+                       : std::ldexp((float)man / 1024.0f, -14);
+    return sign ? -v : v;
+#endif
+}
+
 #if defined(__HIPCC__)
+__device__ const ExtLut kExtLutDev = make_ext_lut();
+
 __device__ __forceinline__ void load_tables(NoiseTables &T, const NoiseTablesPacked &P)
 {
-    uint32_t *pw = reinterpret_cast<uint32_t *>(T.perm);
-    uint32_t *gw = reinterpret_cast<uint32_t *>(T.grad);
-    for (int i = threadIdx.x; i < 64; i += blockDim.x) {
-        pw[i] = P.perm[i];
-        gw[i] = P.grad[i];
+    const uint8_t *pb = reinterpret_cast<const uint8_t *>(P.perm);
+    const uint8_t *gb = reinterpret_cast<const uint8_t *>(P.grad);
+    uint32_t *hw = &T.g16[0][0];
+    for (int i = threadIdx.x; i < kWrap; i += blockDim.x) {
+        T.perm[i] = pb[(i + 255) & 255];
+        T.grad[i] = gb[(i + 255) & 255];
     }
-    for (int i = threadIdx.x; i < 24; i += blockDim.x) T.gvec[i] = P.gvec[i];
+    for (int i = threadIdx.x; i < 48; i += blockDim.x) hw[i] = P.g16[i];
+    const float4 *src = reinterpret_cast<const float4 *>(kExtLutDev.e);
+    float4 *dst = reinterpret_cast<float4 *>(T.ext);
+    for (int i = threadIdx.x; i < kExtKeys * 2; i += blockDim.x) dst[i] = src[i];
 }
 #endif
 
 // One lattice contribution in FP32: attn = 2 - |d|^2 clamped at 0 (the
-// reference adds only when attn > 0), times attn^4 * (g . d).
-NX_ND float contrib(float dx, float dy, float dz, int gi,
-                                         const NoiseTables &T, bool on)
+// reference adds only when attn > 0) times attn^4 * (g . d), accumulated
+// into v.  The fp16 gradient components feed the FMAs directly.
+// `cap` is 2 for a selected point (attn <= 2 always, so the clamp is the
+// reference's attn > 0 test) and 0 for a point the region excludes.
+NX_ND float clamp0(float a, float cap)
 {
-    float attn = fmaf(-dz, dz, fmaf(-dy, dy, fmaf(-dx, dx, 2.0f)));
-    attn = on ? fmaxf(attn, 0.0f) : 0.0f;
-    const float a2 = attn * attn;
-    const int gv = T.gvec[gi];
-    const float gx = (float)sbyte(gv, 0);
-    const float gy = (float)sbyte(gv, 8);
-    const float gz = (float)sbyte(gv, 16);
-    const float dot = fmaf(gz, dz, fmaf(gy, dy, gx * dx));
-    return (a2 * a2) * dot;
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_fmed3f(a, 0.0f, cap);
+#else
+    return std::fmin(std::fmax(a, 0.0f), cap);
+#endif
 }
 
+NX_ND float contrib(float v, float dx, float dy, float dz, int gi, const NoiseTables &T,
+                    float cap)
+{
+    float attn = fmaf(-dz, dz, fmaf(-dy, dy, fmaf(-dx, dx, 2.0f)));
+    attn = clamp0(attn, cap);
+    const float a2 = attn * attn;
+    const uint32_t w0 = T.g16[gi][0], w1 = T.g16[gi][1];
+    const float dot = fmaf(half_to_float(w0 & 0xffffu), dx,
+                           fmaf(half_to_float(w0 >> 16), dy, half_to_float(w1) * dz));
+    return fmaf(a2 * a2, dot, v);
+}
+
+// Hash of lattice point (x, y, z) with x, y, z in -1..257 (a coordinate mod
+// 256 plus an offset).
 NX_ND int hash_full(const NoiseTables &T, int x, int y, int z)
 {
-    const int a = T.perm[x & 255];
-    const int b = T.perm[(a + y) & 255];
-    return T.grad[(b + z) & 255];
+    const int a = T.perm[1 + x];
+    const int b = T.perm[1 + a + y];
+    return T.grad[1 + b + z];
 }
 
 // Selection record for parity tests (same encoding as the oracle):
@@ -107,18 +244,20 @@ struct NoiseSel {
 
 // Evaluate noise3d at (x, y, z) (already multiplied by the octave's
 // frequency).  Returns value / 103 in FP32.  `bad` is set when the point is
-// outside the exact domain (the value is then meaningless).
+// outside the exact domain (the value is then meaningless).  Every branch of
+// the reference's region logic is evaluated with selects, so a wave never
+// diverges.
 template <bool SEL>
-NX_ND float noise3(double x, double y, double z,
-                                        const NoiseTables &T, bool &bad, NoiseSel *sel)
+NX_ND float noise3(double x, double y, double z, const NoiseTables &T, bool &bad,
+                   NoiseSel *sel)
 {
     const double STRETCH = -1.0 / 6;
     const double SQUISH = 1.0 / 3;
     // opensimplex.py:272-299 (FP64, reference operation order)
     const double so = (x + y + z) * STRETCH;
     const double xs = x + so, ys = y + so, zs = z + so;
-    bad = bad || !(fabs(xs) < kNoiseCoordLimit && fabs(ys) < kNoiseCoordLimit &&
-                   fabs(zs) < kNoiseCoordLimit);
+    bad = bad | !((fabs(xs) < kNoiseCoordLimit) & (fabs(ys) < kNoiseCoordLimit) &
+                  (fabs(zs) < kNoiseCoordLimit));
     const int xsb = (int)floor(xs), ysb = (int)floor(ys), zsb = (int)floor(zs);
     const double fx = (double)xsb, fy = (double)ysb, fz = (double)zsb;
     const double sq = (fx + fy + fz) * SQUISH;
@@ -126,211 +265,98 @@ NX_ND float noise3(double x, double y, double z,
     const double in_sum = xins + yins + zins;
     const double dx0d = x - (fx + sq), dy0d = y - (fy + sq), dz0d = z - (fz + sq);
 
-    // Region and extra vertices (opensimplex.py:302-686).  Offsets are
-    // relative to (xsb, ysb, zsb), each in -1..2.
-    int region;
-    int e0x, e0y, e0z, e1x, e1y, e1z;
-    if (in_sum <= 1) {
-        region = 0;
-        int a_point = 1, b_point = 2;
-        double a_score = xins, b_score = yins;
-        if (a_score >= b_score && zins > b_score) {
-            b_score = zins;
-            b_point = 4;
-        } else if (a_score < b_score && zins > a_score) {
-            a_score = zins;
-            a_point = 4;
-        }
-        const double wins = 1 - in_sum;
-        if (wins > a_score || wins > b_score) {
-            const int c = (b_score > a_score) ? b_point : a_point;
-            const int cx = c & 1, cy = c & 2, cz = c & 4;
-            e0x = cx ? 1 : -1;
-            e1x = cx ? 1 : 0;
-            e0y = cy ? 1 : (cx ? -1 : 0);
-            e1y = cy ? 1 : (cx ? 0 : -1);
-            e0z = cz ? 1 : 0;
-            e1z = cz ? 1 : -1;
-        } else {
-            const int c = a_point | b_point;
-            e0x = (c & 1) ? 1 : 0;
-            e1x = (c & 1) ? 1 : -1;
-            e0y = (c & 2) ? 1 : 0;
-            e1y = (c & 2) ? 1 : -1;
-            e0z = (c & 4) ? 1 : 0;
-            e1z = (c & 4) ? 1 : -1;
-        }
-    } else if (in_sum >= 2) {
-        region = 1;
-        int a_point = 6, b_point = 5;
-        double a_score = xins, b_score = yins;
-        if (a_score <= b_score && zins < b_score) {
-            b_score = zins;
-            b_point = 3;
-        } else if (a_score > b_score && zins < a_score) {
-            a_score = zins;
-            a_point = 3;
-        }
-        const double wins = 3 - in_sum;
-        if (wins < a_score || wins < b_score) {
-            const int c = (b_score < a_score) ? b_point : a_point;
-            const int cx = c & 1, cy = c & 2, cz = c & 4;
-            e0x = cx ? 2 : 0;
-            e1x = cx ? 1 : 0;
-            e0y = cy ? (cx ? 1 : 2) : 0;
-            e1y = cy ? (cx ? 2 : 1) : 0;
-            e0z = cz ? 1 : 0;
-            e1z = cz ? 2 : 0;
-        } else {
-            const int c = a_point & b_point;
-            e0x = (c & 1) ? 1 : 0;
-            e1x = (c & 1) ? 2 : 0;
-            e0y = (c & 2) ? 1 : 0;
-            e1y = (c & 2) ? 2 : 0;
-            e0z = (c & 4) ? 1 : 0;
-            e1z = (c & 4) ? 2 : 0;
-        }
-    } else {
-        region = 2;
-        double a_score, b_score;
-        int a_point, b_point;
-        bool a_far, b_far;
-        const double p1 = xins + yins;
-        if (p1 > 1) {
-            a_score = p1 - 1;
-            a_point = 3;
-            a_far = true;
-        } else {
-            a_score = 1 - p1;
-            a_point = 4;
-            a_far = false;
-        }
-        const double p2 = xins + zins;
-        if (p2 > 1) {
-            b_score = p2 - 1;
-            b_point = 5;
-            b_far = true;
-        } else {
-            b_score = 1 - p2;
-            b_point = 2;
-            b_far = false;
-        }
-        const double p3 = yins + zins;
-        if (p3 > 1) {
-            const double score = p3 - 1;
-            if (a_score <= b_score && a_score < score) {
-                a_point = 6;
-                a_far = true;
-            } else if (a_score > b_score && b_score < score) {
-                b_point = 6;
-                b_far = true;
-            }
-        } else {
-            const double score = 1 - p3;
-            if (a_score <= b_score && a_score < score) {
-                a_point = 1;
-                a_far = false;
-            } else if (a_score > b_score && b_score < score) {
-                b_point = 1;
-                b_far = false;
-            }
-        }
-        if (a_far == b_far) {
-            if (a_far) {
-                // (1,1,1) and twice the shared axis
-                const int c = a_point & b_point;
-                e0x = 1;
-                e0y = 1;
-                e0z = 1;
-                e1x = (c & 1) ? 2 : 0;
-                e1y = (!(c & 1) && (c & 2)) ? 2 : 0;
-                e1z = (!(c & 1) && !(c & 2)) ? 2 : 0;
-            } else {
-                // (0,0,0) and a permutation of (-1,1,1) on the omitted axis
-                const int c = a_point | b_point;
-                e0x = 0;
-                e0y = 0;
-                e0z = 0;
-                const int ax = !(c & 1) ? 0 : (!(c & 2) ? 1 : 2);
-                e1x = ax == 0 ? -1 : 1;
-                e1y = ax == 1 ? -1 : 1;
-                e1z = ax == 2 ? -1 : 1;
-            }
-        } else {
-            const int c1 = a_far ? a_point : b_point;
-            const int c2 = a_far ? b_point : a_point;
-            const int ax1 = !(c1 & 1) ? 0 : (!(c1 & 2) ? 1 : 2);
-            e0x = ax1 == 0 ? -1 : 1;
-            e0y = ax1 == 1 ? -1 : 1;
-            e0z = ax1 == 2 ? -1 : 1;
-            const int ax2 = (c2 & 1) ? 0 : ((c2 & 2) ? 1 : 2);
-            e1x = ax2 == 0 ? 2 : 0;
-            e1y = ax2 == 1 ? 2 : 0;
-            e1z = ax2 == 2 ? 2 : 0;
-        }
-    }
+    // ---- region logic, FP64 comparisons in the reference's form ----------
+    // tetrahedron at (0,0,0): opensimplex.py:302-317
+    const bool a1 = (xins >= yins) & (zins > yins);
+    const bool a2 = !a1 & (xins < yins) & (zins > xins);
+    const double asA = a2 ? zins : xins, bsA = a1 ? zins : yins;
+    const int apA = a2 ? 4 : 1, bpA = a1 ? 4 : 2;
+    const double wA = 1 - in_sum;
+    const bool nearA = (wA > asA) | (wA > bsA);
+    const bool bA = bsA > asA;
+    const int keyA = nearA ? (bA ? bpA : apA) : 64 + (apA | bpA);
+    // tetrahedron at (1,1,1): opensimplex.py:417-434
+    const bool b1 = (xins <= yins) & (zins < yins);
+    const bool b2 = !b1 & (xins > yins) & (zins < xins);
+    const double asB = b2 ? zins : xins, bsB = b1 ? zins : yins;
+    const int apB = b2 ? 3 : 6, bpB = b1 ? 3 : 5;
+    const double wB = 3 - in_sum;
+    const bool nearB = (wB < asB) | (wB < bsB);
+    const bool bB = bsB < asB;
+    const int keyB = nearB ? 128 + (bB ? bpB : apB) : 192 + (apB & bpB);
+    // octahedron: opensimplex.py:537-575
+    const double p1 = xins + yins, p2 = xins + zins, p3 = yins + zins;
+    const bool f1 = p1 > 1, f2 = p2 > 1, f3 = p3 > 1;
+    // p - 1 when p > 1, else 1 - p (opensimplex.py:539-559); 1 - p equals
+    // -(p - 1) exactly in IEEE arithmetic, so this is |p - 1|.
+    const double asC = fabs(p1 - 1), bsC = fabs(p2 - 1), scC = fabs(p3 - 1);
+    const bool repA = (asC <= bsC) & (asC < scC);
+    const bool repB = !repA & (asC > bsC) & (bsC < scC);
+    const int rp = f3 ? 6 : 1;
+    const int apC = repA ? rp : (f1 ? 3 : 4), bpC = repB ? rp : (f2 ? 5 : 2);
+    const bool afC = repA ? f3 : f1, bfC = repB ? f3 : f2;
+    const int kFF = 256 + (apC & bpC), kNN = 320 + (apC | bpC);
+    const int kMix = 384 + (afC ? apC * 8 + bpC : bpC * 8 + apC);
+    const int keyC = (afC == bfC) ? (afC ? kFF : kNN) : kMix;
+    const int region = in_sum <= 1 ? 0 : (in_sum >= 2 ? 1 : 2);
+    const int key = region == 0 ? keyA : (region == 1 ? keyB : keyC);
 
-    // ---- FP32 section ---------------------------------------------------
+    // ---- FP32 section -----------------------------------------------------
     const float d0x = (float)dx0d, d0y = (float)dy0d, d0z = (float)dz0d;
     const float S = 1.0f / 3.0f;
     const int xm = xsb & 255, ym = ysb & 255, zm = zsb & 255;
 
-    // Shared hash prefixes for the unit cube.
-    const int px0 = T.perm[xm];
-    const int px1 = T.perm[(xm + 1) & 255];
-    const int p00 = T.perm[(px0 + ym) & 255];
-    const int p01 = T.perm[(px0 + ym + 1) & 255];
-    const int p10 = T.perm[(px1 + ym) & 255];
-    const int p11 = T.perm[(px1 + ym + 1) & 255];
-    const int g100 = T.grad[(p10 + zm) & 255];
-    const int g010 = T.grad[(p01 + zm) & 255];
-    const int g001 = T.grad[(p00 + zm + 1) & 255];
-    const int g110 = T.grad[(p11 + zm) & 255];
-    const int g101 = T.grad[(p10 + zm + 1) & 255];
-    const int g011 = T.grad[(p01 + zm + 1) & 255];
+    // Shared hash prefixes for the unit cube (padded tables: index + 1).
+    const int px0 = T.perm[1 + xm];
+    const int px1 = T.perm[2 + xm];
+    const int p00 = T.perm[1 + px0 + ym];
+    const int p01 = T.perm[2 + px0 + ym];
+    const int p10 = T.perm[1 + px1 + ym];
+    const int p11 = T.perm[2 + px1 + ym];
+    const int g100 = T.grad[1 + p10 + zm];
+    const int g010 = T.grad[1 + p01 + zm];
+    const int g001 = T.grad[2 + p00 + zm];
+    const int g110 = T.grad[1 + p11 + zm];
+    const int g101 = T.grad[2 + p10 + zm];
+    const int g011 = T.grad[2 + p01 + zm];
 
-    const bool oneBits = region != 1;  // 100, 010, 001
-    const bool twoBits = region != 0;  // 110, 101, 011
+    const float oneBits = region != 1 ? 2.0f : 0.0f;  // 100, 010, 001
+    const float twoBits = region != 0 ? 2.0f : 0.0f;  // 110, 101, 011
 
     // Corner c: delta = d0 - c - S * popcount(c).
-    const float ax = d0x - S, ay = d0y - S, az = d0z - S;  // shared 1-bit part
-    float v = contrib(ax - 1.0f, ay, az, g100, T, oneBits);
-    v += contrib(ax, ay - 1.0f, az, g010, T, oneBits);
-    v += contrib(ax, ay, az - 1.0f, g001, T, oneBits);
+    const float ax = d0x - S, ay = d0y - S, az = d0z - S;
+    float v = contrib(0.0f, ax - 1.0f, ay, az, g100, T, oneBits);
+    v = contrib(v, ax, ay - 1.0f, az, g010, T, oneBits);
+    v = contrib(v, ax, ay, az - 1.0f, g001, T, oneBits);
     const float bx = d0x - 2.0f * S, by = d0y - 2.0f * S, bz = d0z - 2.0f * S;
-    v += contrib(bx - 1.0f, by - 1.0f, bz, g110, T, twoBits);
-    v += contrib(bx - 1.0f, by, bz - 1.0f, g101, T, twoBits);
-    v += contrib(bx, by - 1.0f, bz - 1.0f, g011, T, twoBits);
+    v = contrib(v, bx - 1.0f, by - 1.0f, bz, g110, T, twoBits);
+    v = contrib(v, bx - 1.0f, by, bz - 1.0f, g101, T, twoBits);
+    v = contrib(v, bx, by - 1.0f, bz - 1.0f, g011, T, twoBits);
     // Origin slot: 000 in region 0, 111 in region 1.
-    {
-        const bool r1 = region == 1;
-        const int go = T.grad[((r1 ? p11 : p00) + zm + (r1 ? 1 : 0)) & 255];
-        const float k = r1 ? 2.0f : 0.0f;  // 1 + 3S
-        v += contrib(d0x - k, d0y - k, d0z - k, go, T, region != 2);
-    }
-    // Extra vertices: delta = d0 - o - S * (ox + oy + oz).
-    int ge0, ge1;
-    {
-        const float t0 = S * (float)(e0x + e0y + e0z);
-        ge0 = hash_full(T, xm + e0x, ym + e0y, zm + e0z);
-        v += contrib(d0x - ((float)e0x + t0), d0y - ((float)e0y + t0),
-                     d0z - ((float)e0z + t0), ge0, T, true);
-        const float t1 = S * (float)(e1x + e1y + e1z);
-        ge1 = hash_full(T, xm + e1x, ym + e1y, zm + e1z);
-        v += contrib(d0x - ((float)e1x + t1), d0y - ((float)e1y + t1),
-                     d0z - ((float)e1z + t1), ge1, T, true);
-    }
+    const bool r1 = region == 1;
+    const int go = T.grad[1 + (r1 ? p11 + 1 : p00) + zm];
+    const float k = r1 ? 2.0f : 0.0f;  // 1 + 3S
+    v = contrib(v, d0x - k, d0y - k, d0z - k, go, T, region != 2 ? 2.0f : 0.0f);
+    // Extra vertices from the table.
+    const ExtEntry &E = T.ext[key];
+    const int code = E.code;
+    const int ge0 = hash_full(T, xm + (code & 3) - 1, ym + ((code >> 2) & 3) - 1,
+                              zm + ((code >> 4) & 3) - 1);
+    const int ge1 = hash_full(T, xm + ((code >> 6) & 3) - 1, ym + ((code >> 8) & 3) - 1,
+                              zm + ((code >> 10) & 3) - 1);
+    v = contrib(v, d0x - E.c0x, d0y - E.c0y, d0z - E.c0z, ge0, T, 2.0f);
+    v = contrib(v, d0x - E.c1x, d0y - E.c1y, d0z - E.c1z, ge1, T, 2.0f);
 
     if (SEL) {
         // Reference evaluation order of the selected points; the checksum
         // accumulates pgi = 3 * gradient index.
         uint32_t cs = 0;
         if (region == 0) {
-            const int g000 = T.grad[(p00 + zm) & 255];
+            const int g000 = T.grad[1 + p00 + zm];
             const int order[4] = {g000, g100, g010, g001};
             for (int i = 0; i < 4; ++i) cs = cs * 31u + 3u * (uint32_t)order[i];
         } else if (region == 1) {
-            const int g111 = T.grad[(p11 + zm + 1) & 255];
+            const int g111 = T.grad[2 + p11 + zm];
             const int order[4] = {g110, g101, g011, g111};
             for (int i = 0; i < 4; ++i) cs = cs * 31u + 3u * (uint32_t)order[i];
         } else {
@@ -342,8 +368,7 @@ NX_ND float noise3(double x, double y, double z,
         sel->v[0] = xsb;
         sel->v[1] = ysb;
         sel->v[2] = zsb;
-        sel->v[3] = region | ((e0x + 1) << 2) | ((e0y + 1) << 4) | ((e0z + 1) << 6) |
-                    ((e1x + 1) << 8) | ((e1y + 1) << 10) | ((e1z + 1) << 12);
+        sel->v[3] = region | (code & 63) << 2 | ((code >> 6) & 63) << 8;
         sel->v[4] = (int)cs;
     }
     return v * (1.0f / 103.0f);
