@@ -53,6 +53,10 @@ int nx_abi_version(void);
 /* Create / destroy a context bound to HIP device `device`. */
 int nx_ctx_create(int device, nx_ctx **out);
 int nx_ctx_destroy(nx_ctx *ctx);
+/* Context options (nx_ctx_set_option). */
+#define NX_OPT_GRAPH_CHUNK 1 /* erosion steps per captured hipGraph; 0 = eager
+                                launches (default 32)                        */
+int nx_ctx_set_option(nx_ctx *ctx, int option, int64_t value);
 /* Synchronise `stream`, then return and clear the context's sticky kernel
  * error (NX_ERANGE: a noise coordinate outside |x| < 2^30 after the octave
  * multiplier, or not finite; NX_EDIV0: rescale of a constant array). */
@@ -119,7 +123,8 @@ int nx_erosion_load(nx_ctx *ctx, void *state, int64_t V, const double *h,
 /* erosion.py:171-192 (erode_terrain3) over erosion.py:197-279
  * (erosion_iteration3): `iters` Jacobi steps; before each step `rain` is
  * added to every vertex's water (the reference's water += 0.3/320; pass 0
- * for a bare erosion_iteration3).
+ * for a bare erosion_iteration3).  Multi-step runs replay captured
+ * hipGraphs of NX_OPT_GRAPH_CHUNK steps, cached per context.
  *   col   device int32 [6,V]  neighbour of each slot (self for a pad slot)
  *   elen  device f32 [6,V,2]  edge length of each slot as (hi, lo) */
 int nx_erosion_run(nx_ctx *ctx, void *state, int64_t V, const int32_t *col,
@@ -134,6 +139,55 @@ int nx_erosion_store(nx_ctx *ctx, const void *state, int64_t V, double *h,
  * (erosion.py:262-267), so out[0] is constant up to rounding. */
 int nx_erosion_mass(nx_ctx *ctx, const void *state, int64_t V, double *out,
                     void *stream);
+
+/* ---- batched planets (device) ------------------------------------------ */
+
+/* One planet job of a batch: noise (terrain.py:32-59), optional rescale
+ * (util.py:110-140) and optional erosion (erosion.py:171-192).  Host
+ * struct; the pointers it holds are device pointers unless marked host. */
+typedef struct nx_planet {
+    const double *verts;     /* device f64 [V,3] */
+    double *elev;            /* device f64 [V]: noise (then rescaled) heights;
+                                eroded heights after nx_batch_store */
+    int64_t V;
+    int n_octaves;
+    int accumulate;          /* add the octaves to elev's contents */
+    double radius;           /* world_radius */
+    const double *rough;     /* host f64 [n_octaves] */
+    const double *strength;  /* host f64 [n_octaves] */
+    const int32_t *perm;     /* host int32 [256] */
+    const int32_t *pgi;      /* host int32 [256] */
+    double lower, upper;     /* rescale bounds; lower >= upper: no rescale */
+    void *erosion_state;     /* device, nx_erosion_state_bytes(V), or NULL */
+    const int32_t *col;      /* device int32 [6,V] */
+    const float *elen;       /* device f32 [6,V,2] */
+    int64_t erosion_iters;   /* erosion steps of the whole job */
+    double rain;             /* rain per step (reference: 0.3/320) */
+    int64_t steps_done;      /* set to 0 by nx_batch_noise, advanced by
+                                nx_batch_erode */
+} nx_planet;
+
+/* Workspace (device, 256-byte aligned) for a batch of n planets; one
+ * workspace per stream (calls on that stream may reuse it). */
+size_t nx_batch_workspace_bytes(int n);
+
+/* Noise for every planet in one persistent launch (a device work queue of
+ * `tile`-vertex items, `blocks_per_cu` blocks per CU), then per-planet
+ * rescale and, for planets with an erosion state, the state load with the
+ * first step's rain added.  Results equal nx_sample_octaves / nx_rescale /
+ * nx_erosion_load on each planet alone. */
+int nx_batch_noise(nx_ctx *ctx, nx_planet *planets, int n, void *workspace,
+                   int tile, int blocks_per_cu, void *stream);
+
+/* Up to `max_steps` erosion steps for every planet with steps left; one
+ * launch per step covers all active planets (optionally replayed from a
+ * captured hipGraph).  Advances each planet's steps_done. */
+int nx_batch_erode(nx_ctx *ctx, nx_planet *planets, int n, int max_steps,
+                   void *workspace, int use_graph, void *stream);
+
+/* Write the eroded heights of planets with an erosion state into elev. */
+int nx_batch_store(nx_ctx *ctx, const nx_planet *planets, int n,
+                   void *workspace, void *stream);
 
 /* ---- mesh and neighbour tables (host) ---------------------------------- */
 

@@ -19,6 +19,7 @@ NX_ERANGE = -4
 NX_EDIV0 = -5
 NX_EBUSY = -6
 NX_MAX_OCTAVES = 32
+NX_OPT_GRAPH_CHUNK = 1
 
 
 class NixisError(RuntimeError):
@@ -54,6 +55,7 @@ SIGNATURES = {
     "nx_ctx_create": (_c_int, [_c_int, ctypes.POINTER(_c_void_p)]),
     "nx_ctx_destroy": (_c_int, [_c_void_p]),
     "nx_ctx_sync_check": (_c_int, [_c_void_p, _c_void_p]),
+    "nx_ctx_set_option": (_c_int, [_c_void_p, _c_int, _c_i64]),
     "nx_init_tables": (_c_int, [_c_i64, _c_void_p, _c_void_p]),
     "nx_sample_octaves": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p, _c_void_p, _c_int,
                                    _c_void_p, _c_void_p, _c_double, _c_void_p, _c_int,
@@ -147,6 +149,10 @@ class Context:
 
     def sync_check(self, stream=None):
         check(lib().nx_ctx_sync_check(self.handle, stream_ptr(stream)))
+
+    def set_graph_chunk(self, steps):
+        """Erosion steps per captured hipGraph (0: eager launches)."""
+        check(lib().nx_ctx_set_option(self.handle, NX_OPT_GRAPH_CHUNK, int(steps)))
 
 
 def stream_ptr(stream=None):

@@ -75,13 +75,15 @@ int nx_ctx_create(int device, nx_ctx **out)
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         c->num_cus = prop.multiProcessorCount;
-    hipError_t e = hipMalloc((void **)&c->d_err, 256);
+    hipError_t e = hipStreamCreateWithFlags(&c->capture_stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc((void **)&c->d_err, 256);
     if (e == hipSuccess) e = hipMalloc((void **)&c->d_keys, sizeof(unsigned long long) * 2 * kKeySlots);
     if (e == hipSuccess) e = hipMemset(c->d_err, 0, 256);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         if (c->d_err) (void)hipFree(c->d_err);
         if (c->d_keys) (void)hipFree(c->d_keys);
+        if (c->capture_stream) (void)hipStreamDestroy(c->capture_stream);
         delete c;
         return set_error(NX_EHIP, "context device allocation failed: %s", hipGetErrorString(e));
     }
@@ -94,10 +96,27 @@ int nx_ctx_destroy(nx_ctx *ctx)
     if (!ctx) return NX_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipDeviceSynchronize();
+    ctx->clear_graphs();
     if (ctx->d_err) (void)hipFree(ctx->d_err);
     if (ctx->d_keys) (void)hipFree(ctx->d_keys);
+    if (ctx->capture_stream) (void)hipStreamDestroy(ctx->capture_stream);
     delete ctx;
     return NX_OK;
+}
+
+int nx_ctx_set_option(nx_ctx *ctx, int option, int64_t value)
+{
+    if (!ctx) return set_error(NX_EINVAL, "null context");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    switch (option) {
+    case NX_OPT_GRAPH_CHUNK:
+        if (value < 0 || value > 4096) return set_error(NX_EINVAL, "graph chunk %lld outside 0..4096", (long long)value);
+        ctx->clear_graphs();
+        ctx->graph_chunk = (int)value;
+        return NX_OK;
+    default:
+        return set_error(NX_EINVAL, "unknown option %d", option);
+    }
 }
 
 int nx_ctx_sync_check(nx_ctx *ctx, void *stream)
@@ -166,6 +185,25 @@ int nx_erosion_store(nx_ctx *ctx, const void *state, int64_t V, double *h, doubl
 int nx_erosion_mass(nx_ctx *ctx, const void *state, int64_t V, double *out, void *stream)
 {
     return erosion_mass(ctx, state, V, out, (hipStream_t)stream);
+}
+
+size_t nx_batch_workspace_bytes(int n) { return batch_workspace_bytes(n); }
+
+int nx_batch_noise(nx_ctx *ctx, nx_planet *planets, int n, void *workspace, int tile,
+                   int blocks_per_cu, void *stream)
+{
+    return batch_noise(ctx, planets, n, workspace, tile, blocks_per_cu, (hipStream_t)stream);
+}
+
+int nx_batch_erode(nx_ctx *ctx, nx_planet *planets, int n, int max_steps, void *workspace,
+                   int use_graph, void *stream)
+{
+    return batch_erode(ctx, planets, n, max_steps, workspace, use_graph, (hipStream_t)stream);
+}
+
+int nx_batch_store(nx_ctx *ctx, const nx_planet *planets, int n, void *workspace, void *stream)
+{
+    return batch_store(ctx, planets, n, workspace, (hipStream_t)stream);
 }
 
 int nx_icosphere_counts(int64_t d, int64_t *V, int64_t *T)

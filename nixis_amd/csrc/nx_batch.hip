@@ -1,0 +1,511 @@
+// nx_batch.hip -- many planets per launch.
+//
+// Batched form of the terrain path for jobs of uneven size, seed, octave
+// count and erosion length:
+//   * noise: one persistent launch for all planets.  Work items are
+//     (planet, tile) pairs handed out by an atomic counter; planets are
+//     ordered by octave count (cost per tile) so expensive tiles go first,
+//     and a block reloads the seed tables only when it moves to another
+//     planet.  Tile size and blocks per CU come from the autotuner.
+//   * rescale + erosion load: per-planet min/max (one atomic per block per
+//     planet) then the map of util.py:139, which also writes the planet's
+//     float-float erosion state with the first rain already added.
+//   * erosion: one launch per step covers every planet that still has steps
+//     left (planets ordered by remaining steps, so the active set is a
+//     prefix); each planet keeps its own exponent, flags and step counter in
+//     its workspace, so a planet's result does not depend on its batch.
+//     The step sequence can be captured into a hipGraph.
+// Each planet's arithmetic is the single-planet kernels' arithmetic.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "nx_ctx.h"
+#include "nx_erosion_dev.h"
+#include "nx_noise_dev.h"
+#include "nx_reduce_dev.h"
+
+namespace nx {
+
+int pack_tables(const int32_t *perm, const int32_t *pgi, NoiseTablesPacked &p);
+
+struct BatchPlanetDev {
+    const double *verts;
+    double *elev;
+    int64_t V;
+    int32_t n_oct;
+    int32_t accumulate;
+    double radius;
+    double lower, upper;
+    int32_t do_rescale;
+    int32_t do_erode;
+    ErosionMeta *meta;
+    float4 *hw0, *hw1;
+    float2 *s0, *s1;
+    const int32_t *col;
+    const float2 *elen;
+    int64_t total_iters;
+    double rain;
+    int64_t nblocks;  // ceil(V / 256)
+    double rough[NX_MAX_OCTAVES];
+    double strength[NX_MAX_OCTAVES];
+    NoiseTablesPacked tables;
+};
+
+constexpr int kRsTile = 4096;  // vertices per block in the rescale / load / store passes
+
+// Workspace layout (offsets in bytes, all 256-aligned).
+struct BatchLayout {
+    size_t counter, keys, desc, noise_prefix, rs_prefix, erode_prefix, erode_order, steps, total;
+};
+
+static inline size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+static BatchLayout layout(int n)
+{
+    BatchLayout L;
+    size_t o = 0;
+    L.counter = o;
+    o += 256;
+    L.keys = o;
+    o += a256(sizeof(unsigned long long) * 2 * n);
+    L.desc = o;
+    o += a256(sizeof(BatchPlanetDev) * n);
+    L.noise_prefix = o;
+    o += a256(sizeof(int64_t) * (n + 1));
+    L.rs_prefix = o;
+    o += a256(sizeof(int64_t) * (n + 1));
+    L.erode_prefix = o;
+    o += a256(sizeof(int64_t) * (n + 1));
+    L.erode_order = o;
+    o += a256(sizeof(int32_t) * n);
+    L.steps = o;
+    o += a256(sizeof(int64_t) * n);
+    L.total = o;
+    return L;
+}
+
+size_t batch_workspace_bytes(int n) { return n > 0 ? layout(n).total : 256; }
+
+// Index r with prefix[r] <= x < prefix[r + 1] (prefix ascending, n entries).
+__device__ __forceinline__ int find_slot(const int64_t *prefix, int n, int64_t x)
+{
+    int lo = 0, hi = n;  // invariant: prefix[lo] <= x < prefix[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (prefix[mid] <= x)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__global__ void k_batch_reset(unsigned *counter, unsigned long long *keys, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *counter = 0u;
+    if (i < n) {
+        keys[2 * i] = ~0ull;
+        keys[2 * i + 1] = 0ull;
+    }
+}
+
+__device__ __forceinline__ void load_seed_tables(NoiseTables &T, const NoiseTablesPacked &P)
+{
+    const uint8_t *pb = reinterpret_cast<const uint8_t *>(P.perm);
+    const uint8_t *gb = reinterpret_cast<const uint8_t *>(P.grad);
+    for (int i = threadIdx.x; i < kWrap; i += blockDim.x) {
+        T.perm[i] = pb[(i + 255) & 255];
+        T.grad[i] = gb[(i + 255) & 255];
+    }
+    uint32_t *hw = &T.g16[0][0];
+    for (int i = threadIdx.x; i < 48; i += blockDim.x) hw[i] = P.g16[i];
+}
+
+__global__ __launch_bounds__(256) void k_noise_batch(const BatchPlanetDev *__restrict__ P,
+                                                     const int64_t *__restrict__ prefix, int np,
+                                                     int64_t total_tiles, int tile,
+                                                     unsigned *__restrict__ counter,
+                                                     int *__restrict__ err)
+{
+    __shared__ NoiseTables T;
+    __shared__ int64_t s_item;
+    __shared__ int s_p;
+    {
+        const float4 *src = reinterpret_cast<const float4 *>(kExtLutDev.e);
+        float4 *dst = reinterpret_cast<float4 *>(T.ext);
+        for (int i = threadIdx.x; i < kExtKeys * 2; i += blockDim.x) dst[i] = src[i];
+    }
+    int cur = -1;
+    bool bad = false;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            const int64_t item = (int64_t)atomicAdd(counter, 1u);
+            s_item = item;
+            s_p = item < total_tiles ? find_slot(prefix, np, item) : -1;
+        }
+        __syncthreads();
+        const int64_t item = s_item;
+        const int p = s_p;
+        if (item >= total_tiles) break;
+        if (p != cur) {
+            load_seed_tables(T, P[p].tables);
+            __syncthreads();
+            cur = p;
+        }
+        const BatchPlanetDev &D = P[p];
+        const int64_t v0 = (item - prefix[p]) * tile;
+        const int64_t v1 = v0 + tile < D.V ? v0 + tile : D.V;
+        for (int64_t i = v0 + threadIdx.x; i < v1; i += blockDim.x) {
+            const double x = D.verts[3 * i + 0], y = D.verts[3 * i + 1], z = D.verts[3 * i + 2];
+            double acc = D.accumulate ? D.elev[i] : 0.0;
+            for (int o = 0; o < D.n_oct; ++o) {
+                const double r = D.rough[o];
+                const float n = noise3<false>(x * r, y * r, z * r, T, bad, nullptr);
+                acc = acc + (((double)n + 1) * 0.5 * D.strength[o]) * D.radius;
+            }
+            D.elev[i] = acc;
+        }
+        __syncthreads();
+    }
+    if (bad) __hip_atomic_fetch_or(err, NX_ERR_RANGE_BIT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Per-planet min/max of the noise output (planets that rescale or erode).
+__global__ __launch_bounds__(256) void k_batch_minmax(const BatchPlanetDev *__restrict__ P,
+                                                      const int64_t *__restrict__ prefix, int np,
+                                                      unsigned long long *__restrict__ keys)
+{
+    const int p = find_slot(prefix, np, blockIdx.x);
+    const BatchPlanetDev &D = P[p];
+    if (!D.do_rescale && !D.do_erode) return;
+    const int64_t v0 = (blockIdx.x - prefix[p]) * (int64_t)kRsTile;
+    const int64_t v1 = v0 + kRsTile < D.V ? v0 + kRsTile : D.V;
+    unsigned long long lo = ~0ull, hi = 0ull;
+    for (int64_t i = v0 + threadIdx.x; i < v1; i += blockDim.x) {
+        const unsigned long long k = dkey(D.elev[i]);
+        lo = k < lo ? k : lo;
+        hi = k > hi ? k : hi;
+    }
+    block_minmax(lo, hi);
+    if (threadIdx.x == 0) {
+        __hip_atomic_fetch_min(&keys[2 * p], lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_max(&keys[2 * p + 1], hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// Rescale (util.py:139) and load the erosion state (exponent fixed by the
+// rescale bounds, or by the max |height| without rescale; first rain added).
+__global__ __launch_bounds__(256) void k_batch_map(const BatchPlanetDev *__restrict__ P,
+                                                   const int64_t *__restrict__ prefix, int np,
+                                                   const unsigned long long *__restrict__ keys,
+                                                   int *__restrict__ err)
+{
+    const int p = find_slot(prefix, np, blockIdx.x);
+    const BatchPlanetDev &D = P[p];
+    if (!D.do_rescale && !D.do_erode) return;
+    const double x_min = dunkey(keys[2 * p]), x_max = dunkey(keys[2 * p + 1]);
+    const double x_range = x_max - x_min, new_range = D.upper - D.lower;
+    double mx = fmax(fabs(x_min), fabs(x_max));
+    if (D.do_rescale) mx = fmax(fabs(D.lower), fabs(D.upper));
+    const int E = (mx > 0.0 && isfinite(mx)) ? ilogb(mx) : 0;
+    const int64_t v0 = (blockIdx.x - prefix[p]) * (int64_t)kRsTile;
+    if (v0 == 0 && threadIdx.x == 0) {
+        if (D.do_rescale && x_range == 0.0)
+            __hip_atomic_fetch_or(err, NX_ERR_DIV0_BIT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (D.do_erode) {
+            D.meta->it = 0;
+            D.meta->E[2] = E;
+            D.meta->big[0] = D.meta->big[1] = D.meta->big[2] = 0u;
+        }
+    }
+    const int64_t v1 = v0 + kRsTile < D.V ? v0 + kRsTile : D.V;
+    const ff rain = ff_ldexp(ff_from_double(D.total_iters > 0 ? D.rain : 0.0), -E);
+    for (int64_t i = v0 + threadIdx.x; i < v1; i += blockDim.x) {
+        double h = D.elev[i];
+        if (D.do_rescale) {
+            h = ((h - x_min) / x_range) * new_range + D.lower;
+            D.elev[i] = h;
+        }
+        if (D.do_erode) {
+            const ff hh = ff_from_double(ldexp(h, -E));
+            D.hw0[i] = make_float4(hh.hi, hh.lo, rain.hi, rain.lo);
+            D.s0[i] = make_float2(0.0f, 0.0f);
+        }
+    }
+}
+
+// One erosion step for the first `active` planets of the erosion order.
+__global__ __launch_bounds__(256) void k_erode_batch(const BatchPlanetDev *__restrict__ P,
+                                                     const int32_t *__restrict__ order,
+                                                     const int64_t *__restrict__ prefix,
+                                                     int active, int64_t nb_total, int64_t k)
+{
+    const int64_t b = xcd_tile(blockIdx.x, nb_total);
+    if (b >= nb_total) return;
+    const int r = find_slot(prefix, active, b);
+    const BatchPlanetDev &D = P[order[r]];
+    ErosionMeta *meta = D.meta;
+    const int64_t it = meta->it + k;
+    const float4 *hw_src = (it & 1) ? D.hw1 : D.hw0;
+    const float2 *s_src = (it & 1) ? D.s1 : D.s0;
+    float4 *hw_dst = (it & 1) ? D.hw0 : D.hw1;
+    float2 *s_dst = (it & 1) ? D.s0 : D.s1;
+    const int cur = (int)(it % 3), prev = (int)((it + 2) % 3), nxt = (int)((it + 1) % 3);
+    const int sh = meta->big[prev] ? 64 : 0;
+    const int E = meta->E[prev] + sh;
+    const float sc = ldexpf(1.0f, -sh);
+    const int64_t lb = b - prefix[r];
+    if (lb == 0 && threadIdx.x == 0) {
+        meta->E[cur] = E;
+        meta->big[nxt] = 0u;
+    }
+    const ff rain = ff_ldexp(ff_from_double(it + 1 < D.total_iters ? D.rain : 0.0), -E);
+    float m = 0.0f;
+    const int64_t i = lb * blockDim.x + threadIdx.x;
+    if (i < D.V)
+        m = erode_vertex_step(i, D.V, hw_src, s_src, hw_dst, s_dst, D.col, D.elen, sc, rain);
+    flag_big(m > kBig, &meta->big[cur]);
+}
+
+__global__ void k_batch_advance(const BatchPlanetDev *__restrict__ P,
+                                const int32_t *__restrict__ order,
+                                const int64_t *__restrict__ steps, int n)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) P[order[r]].meta->it += steps[r];
+}
+
+__global__ __launch_bounds__(256) void k_batch_store(const BatchPlanetDev *__restrict__ P,
+                                                     const int64_t *__restrict__ prefix, int np)
+{
+    const int p = find_slot(prefix, np, blockIdx.x);
+    const BatchPlanetDev &D = P[p];
+    if (!D.do_erode) return;
+    const int64_t it = D.meta->it;
+    const int E = D.meta->E[(int)((it + 2) % 3)];
+    const float4 *hw = (it & 1) ? D.hw1 : D.hw0;
+    const int64_t v0 = (blockIdx.x - prefix[p]) * (int64_t)kRsTile;
+    const int64_t v1 = v0 + kRsTile < D.V ? v0 + kRsTile : D.V;
+    for (int64_t i = v0 + threadIdx.x; i < v1; i += blockDim.x) {
+        const float4 v = hw[i];
+        D.elev[i] = ldexp((double)v.x + (double)v.y, E);
+    }
+}
+
+// ---- host side ---------------------------------------------------------
+
+static int fill_desc(const nx_planet &q, BatchPlanetDev &d)
+{
+    std::memset(&d, 0, sizeof(d));
+    if (q.V <= 0 || q.V > INT32_MAX || !q.verts || !q.elev)
+        return set_error(NX_EINVAL, "planet needs 0 < V < 2^31 and device verts/elev");
+    if (q.n_octaves < 0 || q.n_octaves > NX_MAX_OCTAVES || (q.n_octaves > 0 && (!q.rough || !q.strength)))
+        return set_error(NX_EINVAL, "planet octaves %d outside 0..%d", q.n_octaves, NX_MAX_OCTAVES);
+    int rc = pack_tables(q.perm, q.pgi, d.tables);
+    if (rc) return rc;
+    d.verts = q.verts;
+    d.elev = q.elev;
+    d.V = q.V;
+    d.n_oct = q.n_octaves;
+    d.accumulate = q.accumulate ? 1 : 0;
+    d.radius = q.radius;
+    for (int o = 0; o < q.n_octaves; ++o) {
+        d.rough[o] = q.rough[o];
+        d.strength[o] = q.strength[o];
+    }
+    d.lower = q.lower;
+    d.upper = q.upper;
+    d.do_rescale = q.lower < q.upper;
+    d.nblocks = (q.V + 255) / 256;
+    if (q.erosion_state) {
+        if (((uintptr_t)q.erosion_state & 255) || !q.col || !q.elen || q.erosion_iters < 0 ||
+            !(q.rain >= 0.0))
+            return set_error(NX_EINVAL, "planet erosion state must be 256-aligned with col/elen, "
+                                        "iters >= 0 and rain >= 0");
+        char *base = (char *)q.erosion_state;
+        const size_t hw = (size_t)a256((size_t)q.V * sizeof(float4)), sd = a256((size_t)q.V * 8);
+        d.do_erode = 1;
+        d.meta = (ErosionMeta *)base;
+        d.hw0 = (float4 *)(base + 256);
+        d.hw1 = (float4 *)(base + 256 + hw);
+        d.s0 = (float2 *)(base + 256 + 2 * hw);
+        d.s1 = (float2 *)(base + 256 + 2 * hw + sd);
+        d.col = q.col;
+        d.elen = (const float2 *)q.elen;
+        d.total_iters = q.erosion_iters;
+        d.rain = q.rain;
+    }
+    return NX_OK;
+}
+
+int batch_noise(nx_ctx *ctx, nx_planet *planets, int n, void *ws, int tile, int blocks_per_cu,
+                hipStream_t stream)
+{
+    if (!ctx || !planets || n <= 0 || !ws) return set_error(NX_EINVAL, "batch needs planets and a workspace");
+    if (((uintptr_t)ws & 255) != 0) return set_error(NX_EINVAL, "workspace must be 256-byte aligned");
+    if (tile < 256 || tile > (1 << 22) || (tile & 255)) return set_error(NX_EINVAL, "tile %d: multiple of 256 in 256..4M", tile);
+    if (blocks_per_cu < 1 || blocks_per_cu > 16) return set_error(NX_EINVAL, "blocks_per_cu %d outside 1..16", blocks_per_cu);
+    const BatchLayout L = layout(n);
+    // Noise order: most octaves (most work per tile) first.
+    std::vector<int> ord(n);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return planets[a].n_octaves > planets[b].n_octaves; });
+    std::vector<BatchPlanetDev> desc(n);
+    std::vector<int64_t> tpre(n + 1, 0), rpre(n + 1, 0);
+    for (int r = 0; r < n; ++r) {
+        int rc = fill_desc(planets[ord[r]], desc[r]);
+        if (rc) return rc;
+        tpre[r + 1] = tpre[r] + (desc[r].V + tile - 1) / tile;
+        rpre[r + 1] = rpre[r] + (desc[r].V + kRsTile - 1) / kRsTile;
+    }
+    char *w = (char *)ws;
+    NX_HIP(hipSetDevice(ctx->device));
+    NX_HIP(hipMemcpyAsync(w + L.desc, desc.data(), sizeof(BatchPlanetDev) * n, hipMemcpyHostToDevice, stream));
+    NX_HIP(hipMemcpyAsync(w + L.noise_prefix, tpre.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, stream));
+    NX_HIP(hipMemcpyAsync(w + L.rs_prefix, rpre.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, stream));
+    auto *counter = (unsigned *)(w + L.counter);
+    auto *keys = (unsigned long long *)(w + L.keys);
+    const auto *D = (const BatchPlanetDev *)(w + L.desc);
+    hipLaunchKernelGGL(k_batch_reset, dim3((n + 255) / 256), dim3(256), 0, stream, counter, keys, n);
+    const int64_t grid = std::min<int64_t>(tpre[n], (int64_t)ctx->num_cus * blocks_per_cu);
+    hipLaunchKernelGGL(k_noise_batch, dim3((unsigned)std::max<int64_t>(grid, 1)), dim3(256), 0, stream,
+                       D, (const int64_t *)(w + L.noise_prefix), n, tpre[n], tile, counter, ctx->d_err);
+    bool need_map = false;
+    for (int r = 0; r < n; ++r) need_map = need_map || desc[r].do_rescale || desc[r].do_erode;
+    if (need_map) {
+        const int64_t *rp = (const int64_t *)(w + L.rs_prefix);
+        hipLaunchKernelGGL(k_batch_minmax, dim3((unsigned)rpre[n]), dim3(256), 0, stream, D, rp, n, keys);
+        hipLaunchKernelGGL(k_batch_map, dim3((unsigned)rpre[n]), dim3(256), 0, stream, D, rp, n,
+                           (const unsigned long long *)keys, ctx->d_err);
+    }
+    NX_HIP(hipGetLastError());
+    for (int p = 0; p < n; ++p) planets[p].steps_done = 0;
+    return NX_OK;
+}
+
+static uint64_t fnv(const void *p, size_t n, uint64_t h)
+{
+    const unsigned char *c = (const unsigned char *)p;
+    for (size_t i = 0; i < n; ++i) h = (h ^ c[i]) * 1099511628211ull;
+    return h;
+}
+
+int batch_erode(nx_ctx *ctx, nx_planet *planets, int n, int max_steps, void *ws, int use_graph,
+                hipStream_t stream)
+{
+    if (!ctx || !planets || n <= 0 || !ws) return set_error(NX_EINVAL, "batch needs planets and a workspace");
+    if (((uintptr_t)ws & 255) != 0) return set_error(NX_EINVAL, "workspace must be 256-byte aligned");
+    if (max_steps < 0) return set_error(NX_EINVAL, "max_steps < 0");
+    const BatchLayout L = layout(n);
+    // Steps of each planet in this call; erosion order = most steps first.
+    std::vector<int64_t> steps(n, 0);
+    for (int p = 0; p < n; ++p) {
+        const nx_planet &q = planets[p];
+        if (q.erosion_state) steps[p] = std::max<int64_t>(0, std::min<int64_t>(max_steps, q.erosion_iters - q.steps_done));
+    }
+    std::vector<int> ord(n);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return steps[a] > steps[b]; });
+    int active0 = 0;
+    while (active0 < n && steps[ord[active0]] > 0) ++active0;
+    if (active0 == 0) return NX_OK;
+    const int64_t S = steps[ord[0]];
+    std::vector<BatchPlanetDev> desc(n);
+    for (int p = 0; p < n; ++p) {
+        int rc = fill_desc(planets[p], desc[p]);
+        if (rc) return rc;
+    }
+    std::vector<int64_t> bpre(active0 + 1, 0), st(active0);
+    std::vector<int32_t> order(active0);
+    for (int r = 0; r < active0; ++r) {
+        order[r] = ord[r];
+        st[r] = steps[ord[r]];
+        bpre[r + 1] = bpre[r] + desc[ord[r]].nblocks;
+    }
+    char *w = (char *)ws;
+    NX_HIP(hipSetDevice(ctx->device));
+    NX_HIP(hipMemcpyAsync(w + L.desc, desc.data(), sizeof(BatchPlanetDev) * n, hipMemcpyHostToDevice, stream));
+    NX_HIP(hipMemcpyAsync(w + L.erode_prefix, bpre.data(), sizeof(int64_t) * (active0 + 1), hipMemcpyHostToDevice, stream));
+    NX_HIP(hipMemcpyAsync(w + L.erode_order, order.data(), sizeof(int32_t) * active0, hipMemcpyHostToDevice, stream));
+    NX_HIP(hipMemcpyAsync(w + L.steps, st.data(), sizeof(int64_t) * active0, hipMemcpyHostToDevice, stream));
+    const auto *D = (const BatchPlanetDev *)(w + L.desc);
+    const auto *O = (const int32_t *)(w + L.erode_order);
+    const auto *BP = (const int64_t *)(w + L.erode_prefix);
+    const auto *ST = (const int64_t *)(w + L.steps);
+    auto launch_all = [&](hipStream_t s) {
+        int active = active0;
+        for (int64_t k = 0; k < S; ++k) {
+            while (active > 0 && st[active - 1] <= k) --active;
+            const int64_t nb = bpre[active];
+            const int64_t grid = ((nb + 7) / 8) * 8;
+            hipLaunchKernelGGL(k_erode_batch, dim3((unsigned)grid), dim3(256), 0, s, D, O, BP, active, nb, k);
+        }
+        hipLaunchKernelGGL(k_batch_advance, dim3((active0 + 255) / 256), dim3(256), 0, s, D, O, ST, active0);
+    };
+    if (!use_graph) {
+        launch_all(stream);
+        NX_HIP(hipGetLastError());
+    } else {
+        // Cache key: workspace, the descriptor contents and the schedule.
+        uint64_t h = 1469598103934665603ull;
+        h = fnv(desc.data(), sizeof(BatchPlanetDev) * n, h);
+        h = fnv(order.data(), sizeof(int32_t) * active0, h);
+        h = fnv(st.data(), sizeof(int64_t) * active0, h);
+        GraphKey key{ws, (int64_t)n, nullptr, nullptr, (int)S, 0.0, true, h};
+        hipGraphExec_t ex = nullptr;
+        {
+            std::lock_guard<std::mutex> lk(ctx->mu);
+            auto itg = ctx->graphs.find(key);
+            if (itg != ctx->graphs.end()) {
+                ex = itg->second;
+            } else {
+                if (ctx->graphs.size() >= kMaxGraphs) ctx->clear_graphs();
+                hipGraph_t g = nullptr;
+                NX_HIP(hipStreamBeginCapture(ctx->capture_stream, hipStreamCaptureModeThreadLocal));
+                launch_all(ctx->capture_stream);
+                const hipError_t le = hipGetLastError();
+                const hipError_t ce = hipStreamEndCapture(ctx->capture_stream, &g);
+                if (le != hipSuccess || ce != hipSuccess) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return set_error(NX_EHIP, "batch erosion capture failed: %s",
+                                     hipGetErrorString(le != hipSuccess ? le : ce));
+                }
+                const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (ie != hipSuccess) return set_error(NX_EHIP, "graph instantiate: %s", hipGetErrorString(ie));
+                ctx->graphs.emplace(key, ex);
+            }
+        }
+        NX_HIP(hipGraphLaunch(ex, stream));
+    }
+    for (int p = 0; p < n; ++p) planets[p].steps_done += steps[p];
+    return NX_OK;
+}
+
+int batch_store(nx_ctx *ctx, const nx_planet *planets, int n, void *ws, hipStream_t stream)
+{
+    if (!ctx || !planets || n <= 0 || !ws) return set_error(NX_EINVAL, "batch needs planets and a workspace");
+    const BatchLayout L = layout(n);
+    std::vector<BatchPlanetDev> desc(n);
+    std::vector<int64_t> rpre(n + 1, 0);
+    for (int p = 0; p < n; ++p) {
+        int rc = fill_desc(planets[p], desc[p]);
+        if (rc) return rc;
+        rpre[p + 1] = rpre[p] + (desc[p].V + kRsTile - 1) / kRsTile;
+    }
+    char *w = (char *)ws;
+    NX_HIP(hipSetDevice(ctx->device));
+    NX_HIP(hipMemcpyAsync(w + L.desc, desc.data(), sizeof(BatchPlanetDev) * n, hipMemcpyHostToDevice, stream));
+    NX_HIP(hipMemcpyAsync(w + L.rs_prefix, rpre.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, stream));
+    hipLaunchKernelGGL(k_batch_store, dim3((unsigned)rpre[n]), dim3(256), 0, stream,
+                       (const BatchPlanetDev *)(w + L.desc), (const int64_t *)(w + L.rs_prefix), n);
+    NX_HIP(hipGetLastError());
+    return NX_OK;
+}
+
+}  // namespace nx

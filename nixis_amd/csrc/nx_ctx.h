@@ -4,7 +4,9 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <map>
 #include <mutex>
+#include <tuple>
 
 #include "nx_internal.h"
 
@@ -13,6 +15,24 @@
 #define NX_ERR_DIV0_BIT 2
 
 constexpr int kKeySlots = 256;
+constexpr size_t kMaxGraphs = 64;
+
+// Identity of a captured chunk of erosion steps.
+struct GraphKey {
+    const void *state;
+    int64_t V;
+    const void *col;
+    const void *elen;
+    int n;
+    double rain;
+    bool last;
+    uint64_t tag;  // content hash for batched schedules, 0 otherwise
+    bool operator<(const GraphKey &o) const
+    {
+        return std::tie(state, V, col, elen, n, rain, last, tag) <
+               std::tie(o.state, o.V, o.col, o.elen, o.n, o.rain, o.last, o.tag);
+    }
+};
 
 struct nx_ctx {
     int device = 0;
@@ -20,7 +40,16 @@ struct nx_ctx {
     int *d_err = nullptr;                  // device word, bits NX_ERR_*
     unsigned long long *d_keys = nullptr;  // kKeySlots x 2 reduction keys
     std::atomic<unsigned> key_slot{0};
+    hipStream_t capture_stream = nullptr;  // default priority, graph capture only
+    int graph_chunk = 32;                  // erosion steps per captured graph; 0 = eager
+    std::map<GraphKey, hipGraphExec_t> graphs;
     std::mutex mu;
+
+    void clear_graphs()
+    {
+        for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        graphs.clear();
+    }
 };
 
 #define NX_HIP(call)                                                                      \
@@ -49,5 +78,11 @@ int erosion_run(nx_ctx *ctx, void *state, int64_t V, const int32_t *col, const f
 int erosion_store(nx_ctx *ctx, const void *state, int64_t V, double *h, double *w, double *s,
                   hipStream_t stream);
 int erosion_mass(nx_ctx *ctx, const void *state, int64_t V, double *out, hipStream_t stream);
+size_t batch_workspace_bytes(int n);
+int batch_noise(nx_ctx *ctx, nx_planet *planets, int n, void *ws, int tile, int blocks_per_cu,
+                hipStream_t stream);
+int batch_erode(nx_ctx *ctx, nx_planet *planets, int n, int max_steps, void *ws, int use_graph,
+                hipStream_t stream);
+int batch_store(nx_ctx *ctx, const nx_planet *planets, int n, void *ws, hipStream_t stream);
 
 }  // namespace nx

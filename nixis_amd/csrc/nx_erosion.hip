@@ -27,13 +27,6 @@
 
 namespace nx {
 
-struct ErosionMeta {
-    int64_t it;           // steps taken so far (selects buffers and slots)
-    int32_t E[3];         // exponent after step it-1, rotating by step % 3
-    uint32_t big[3];      // 1 if some output of that step exceeded 2^64
-    uint64_t load_max;    // FP64 max |input| bits while loading
-};
-
 struct ErosionBufs {
     ErosionMeta *meta;
     float4 *hw[2];
@@ -65,28 +58,6 @@ static ErosionBufs bufs(void *state, int64_t V)
     return b;
 }
 
-// Outputs above this magnitude trigger a 2^-64 rescale of the next step.
-constexpr float kBig = 18446744073709551616.0f;  // 2^64
-
-// Raise the step's overflow-guard flag if any lane of the wave holds a value
-// above 2^64: one ballot per wave and a coherent store of the constant 1 by
-// the first lane (concurrent stores of the same value need no atomic RMW).
-__device__ __forceinline__ void flag_big(bool over, uint32_t *flag)
-{
-    if (__any(over) && (threadIdx.x & 63) == 0)
-        __hip_atomic_store(flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Logical block index whose ranges are contiguous per XCD: the hardware
-// hands consecutive workgroups to the 8 XCDs round-robin, so block b runs on
-// XCD b % 8; giving each XCD one contiguous eighth of the vertices keeps the
-// neighbour rows of a tile in that XCD's L2.
-__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb)
-{
-    const int64_t per = (nb + 7) / 8;
-    return (b % 8) * per + b / 8;
-}
-
 __global__ __launch_bounds__(256) void k_erode_step(ErosionMeta *__restrict__ meta,
                                                     float4 *__restrict__ hw0,
                                                     float4 *__restrict__ hw1,
@@ -114,26 +85,8 @@ __global__ __launch_bounds__(256) void k_erode_step(ErosionMeta *__restrict__ me
     float m = 0.0f;
     const int64_t i = tile * blockDim.x + threadIdx.x;
     if (tile < nblocks && i < V) {
-        const ErosionConsts c = erosion_consts();
-        const float4 me = hw_src[i];
-        const float2 ms = s_src[i];
-        const ff h = ff_scale(ff{me.x, me.y}, sc), w = ff_scale(ff{me.z, me.w}, sc);
-        const ff s = ff_scale(ff{ms.x, ms.y}, sc);
-        ff sum_w{0.0f, 0.0f}, sum_wd{0.0f, 0.0f};
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            const int32_t n = col[(int64_t)k * V + i];
-            const float2 d = elen[(int64_t)k * V + i];
-            const float4 nb = hw_src[n];
-            erode_accumulate(h, ff_scale(ff{nb.x, nb.y}, sc), ff_scale(ff{nb.z, nb.w}, sc),
-                             ff{d.x, d.y}, sum_w, sum_wd);
-        }
         const ff rain = ff_ldexp(ff_from_double(rain_next), -E);
-        ff ho, wo, so;
-        erode_finish(c, h, w, s, sum_w, sum_wd, rain, ho, wo, so);
-        hw_dst[i] = make_float4(ho.hi, ho.lo, wo.hi, wo.lo);
-        s_dst[i] = make_float2(so.hi, so.lo);
-        m = fmaxf(fabsf(ho.hi), fmaxf(fabsf(wo.hi), fabsf(so.hi)));
+        m = erode_vertex_step(i, V, hw_src, s_src, hw_dst, s_dst, col, elen, sc, rain);
     }
     flag_big(m > kBig, &meta->big[cur]);
 }
@@ -324,6 +277,58 @@ int erosion_load(nx_ctx *ctx, void *state, int64_t V, const double *h, const dou
     return NX_OK;
 }
 
+// Launch n consecutive steps (launch k uses step it + k, where `it` is read
+// from the workspace) and advance the workspace's step counter by n.  The
+// last step's rain for a following step is `rain` unless `last`.
+static void launch_steps(const ErosionBufs &b, int64_t V, const int32_t *col, const float *elen,
+                         int n, double rain, bool last, hipStream_t stream)
+{
+    const int64_t nblocks = (V + 255) / 256;
+    const int64_t grid = ((nblocks + 7) / 8) * 8;
+    for (int k = 0; k < n; ++k) {
+        const double rain_next = (last && k + 1 == n) ? 0.0 : rain;
+        hipLaunchKernelGGL(k_erode_step, dim3((unsigned)grid), dim3(256), 0, stream, b.meta,
+                           b.hw[0], b.hw[1], b.s[0], b.s[1], V, col, (const float2 *)elen,
+                           nblocks, (int64_t)k, rain_next);
+    }
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, stream, b.meta, (int64_t)n);
+}
+
+// Captured chunk of steps, cached per (workspace, graph, length, rain, last).
+// Replays stay correct because every launch reads the step counter, buffer
+// parity and exponent state from the workspace rather than from its
+// arguments.  No memset is recorded into the graph.
+static int chunk_graph(nx_ctx *ctx, const ErosionBufs &b, void *state, int64_t V,
+                       const int32_t *col, const float *elen, int n, double rain, bool last,
+                       hipGraphExec_t *out)
+{
+    GraphKey key{state, V, col, elen, n, rain, last, 0};
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    auto it = ctx->graphs.find(key);
+    if (it != ctx->graphs.end()) {
+        *out = it->second;
+        return NX_OK;
+    }
+    if (ctx->graphs.size() >= kMaxGraphs) ctx->clear_graphs();
+    hipGraph_t g = nullptr;
+    NX_HIP(hipStreamBeginCapture(ctx->capture_stream, hipStreamCaptureModeThreadLocal));
+    launch_steps(b, V, col, elen, n, rain, last, ctx->capture_stream);
+    const hipError_t le = hipGetLastError();
+    const hipError_t ce = hipStreamEndCapture(ctx->capture_stream, &g);
+    if (le != hipSuccess || ce != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return set_error(NX_EHIP, "erosion graph capture failed: %s",
+                         hipGetErrorString(le != hipSuccess ? le : ce));
+    }
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) return set_error(NX_EHIP, "graph instantiate: %s", hipGetErrorString(ie));
+    ctx->graphs.emplace(key, ex);
+    *out = ex;
+    return NX_OK;
+}
+
 int erosion_run(nx_ctx *ctx, void *state, int64_t V, const int32_t *col, const float *elen,
                 int iters, double rain, hipStream_t stream)
 {
@@ -336,21 +341,26 @@ int erosion_run(nx_ctx *ctx, void *state, int64_t V, const int32_t *col, const f
     NX_HIP(hipSetDevice(ctx->device));
     ErosionBufs b = bufs(state, V);
     const int64_t nblocks = (V + 255) / 256;
-    const int64_t grid = ((nblocks + 7) / 8) * 8;
     if (rain != 0.0)
         hipLaunchKernelGGL(k_add_rain, dim3((unsigned)nblocks), dim3(256), 0, stream, b.meta,
                            b.hw[0], b.hw[1], V, rain);
-    // The step index is read from the workspace so a captured sequence of
-    // launches stays correct when replayed: launch k of this call uses step
-    // it0 + k; `it` itself advances once at the end.
-    for (int k = 0; k < iters; ++k) {
-        const double rain_next = (k + 1 < iters) ? rain : 0.0;
-        hipLaunchKernelGGL(k_erode_step, dim3((unsigned)grid), dim3(256), 0, stream,
-                           b.meta, b.hw[0], b.hw[1], b.s[0], b.s[1], V, col,
-                           (const float2 *)elen, nblocks, (int64_t)k, rain_next);
+    const int chunk = ctx->graph_chunk;
+    if (chunk <= 0 || iters < 2) {
+        launch_steps(b, V, col, elen, iters, rain, true, stream);
+        NX_HIP(hipGetLastError());
+        return NX_OK;
     }
-    hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, stream, b.meta, (int64_t)iters);
-    NX_HIP(hipGetLastError());
+    // Full chunks, then the remainder; the final piece does not pre-add rain.
+    int done = 0;
+    while (done < iters) {
+        const int n = std::min(chunk, iters - done);
+        const bool last = done + n == iters;
+        hipGraphExec_t ex = nullptr;
+        rc = chunk_graph(ctx, b, state, V, col, elen, n, rain, last, &ex);
+        if (rc) return rc;
+        NX_HIP(hipGraphLaunch(ex, stream));
+        done += n;
+    }
     return NX_OK;
 }
 

@@ -13,9 +13,19 @@
 // neighbouring heights differ by less than its 24-bit resolution).
 #pragma once
 
+#include <stdint.h>
+
 #include "nx_ff.h"
 
 namespace nx {
+
+// Per-planet state at the head of an erosion workspace.
+struct ErosionMeta {
+    int64_t it;           // steps taken so far (selects buffers and slots)
+    int32_t E[3];         // exponent after step it-1, rotating by step % 3
+    uint32_t big[3];      // 1 if some output of that step exceeded 2^64
+    uint64_t load_max;    // FP64 max |input| bits while loading
+};
 
 // erosion.py:205-207, rounded from the FP64 constants to float-float.
 struct ErosionConsts {
@@ -70,5 +80,58 @@ NX_HD void erode_finish(const ErosionConsts &c, ff h, ff w, ff s, ff sum_w, ff s
     s_out = sn;
     w_out = ff_add(wn, rain);
 }
+
+#if defined(__HIPCC__)
+// Outputs above this magnitude trigger a 2^-64 rescale of the next step.
+constexpr float kBig = 18446744073709551616.0f;  // 2^64
+
+// Raise the step's overflow-guard flag if any lane of the wave holds a value
+// above 2^64: one ballot per wave and a coherent store of the constant 1 by
+// the first lane (concurrent stores of the same value need no atomic RMW).
+__device__ __forceinline__ void flag_big(bool over, uint32_t *flag)
+{
+    if (__any(over) && (threadIdx.x & 63) == 0)
+        __hip_atomic_store(flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Logical block index whose ranges are contiguous per XCD: the hardware
+// hands consecutive workgroups to the 8 XCDs round-robin, so block b runs on
+// XCD b % 8; giving each XCD one contiguous eighth of the vertices keeps the
+// neighbour rows of a tile in that XCD's L2.
+__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb)
+{
+    const int64_t per = (nb + 7) / 8;
+    return (b % 8) * per + b / 8;
+}
+
+// One vertex of one step: gather the six neighbours from the source
+// buffers (scaled by `sc`), update, write the destination buffers; returns
+// the largest output magnitude for the overflow guard.
+__device__ __forceinline__ float erode_vertex_step(int64_t i, int64_t V, const float4 *hw_src,
+                                                   const float2 *s_src, float4 *hw_dst,
+                                                   float2 *s_dst, const int32_t *col,
+                                                   const float2 *elen, float sc, ff rain)
+{
+    const ErosionConsts c = erosion_consts();
+    const float4 me = hw_src[i];
+    const float2 ms = s_src[i];
+    const ff h = ff_scale(ff{me.x, me.y}, sc), w = ff_scale(ff{me.z, me.w}, sc);
+    const ff s = ff_scale(ff{ms.x, ms.y}, sc);
+    ff sum_w{0.0f, 0.0f}, sum_wd{0.0f, 0.0f};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const int32_t n = col[(int64_t)k * V + i];
+        const float2 d = elen[(int64_t)k * V + i];
+        const float4 nb = hw_src[n];
+        erode_accumulate(h, ff_scale(ff{nb.x, nb.y}, sc), ff_scale(ff{nb.z, nb.w}, sc),
+                         ff{d.x, d.y}, sum_w, sum_wd);
+    }
+    ff ho, wo, so;
+    erode_finish(c, h, w, s, sum_w, sum_wd, rain, ho, wo, so);
+    hw_dst[i] = make_float4(ho.hi, ho.lo, wo.hi, wo.lo);
+    s_dst[i] = make_float2(so.hi, so.lo);
+    return fmaxf(fabsf(ho.hi), fmaxf(fabsf(wo.hi), fabsf(so.hi)));
+}
+#endif
 
 }  // namespace nx

@@ -10,21 +10,9 @@
 #include <atomic>
 
 #include "nx_ctx.h"
+#include "nx_reduce_dev.h"
 
 namespace nx {
-
-// Monotone map double -> uint64 (negative values reversed).
-__device__ __forceinline__ unsigned long long dkey(double x)
-{
-    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
-    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
-}
-
-__device__ __forceinline__ double dunkey(unsigned long long k)
-{
-    const unsigned long long b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
-    return __longlong_as_double((long long)b);
-}
 
 __global__ void k_minmax_init(unsigned long long *keys)
 {

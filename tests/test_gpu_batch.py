@@ -1,0 +1,105 @@
+"""Batched planet jobs equal the one-planet-per-call path bit for bit.
+
+Each planet's arithmetic in the batched kernels is the single-planet
+kernels' arithmetic, and each planet keeps its own exponent and step
+counter, so results must not depend on the batch composition, the tile
+size, the blocks per CU, graph capture or chunking of the erosion steps.
+"""
+import numpy as np
+import pytest
+import torch
+
+from nixis_amd import autotune, batch
+
+pytestmark = pytest.mark.gpu
+
+
+def _jobs():
+    specs = [(8, 3, 4, 0), (12, 5, 9, 17), (16, 1, 16, 40), (20, 9, 6, 3),
+             (30, 2, 11, 0), (24, 77, 13, 25), (10, 4, 5, 40)]
+    return [batch.Job(d=d, seed=s, n_octaves=o, erosion_iters=it, rain=1.0 if k % 2 else 0.3 / 320)
+            for k, (d, s, o, it) in enumerate(specs)]
+
+
+def _runs(jobs, meshes):
+    return [batch.PlanetRun(j, meshes) for j in jobs]
+
+
+def _heights(runs):
+    torch.cuda.synchronize()
+    return [r.elev.cpu().numpy().copy() for r in runs]
+
+
+@pytest.fixture(scope="module")
+def meshes():
+    return batch.MeshCache()
+
+
+@pytest.fixture(scope="module")
+def plain(meshes):
+    runs = _runs(_jobs(), meshes)
+    batch.run_plain(runs)
+    runs[0].state  # noqa: B018
+    return _heights(runs)
+
+
+@pytest.mark.parametrize("use_graph,tile,bpc", [(True, 8192, 8), (False, 2048, 2),
+                                                (True, 256, 1)])
+def test_batch_equals_plain(meshes, plain, use_graph, tile, bpc):
+    runs = _runs(_jobs(), meshes)
+    br = batch.BatchRunner()
+    br.run(runs, batch.Tuning(tile, bpc), use_graph=use_graph)
+    br.ctx.sync_check()
+    for k, (a, b) in enumerate(zip(_heights(runs), plain)):
+        assert (a == b).all(), f"planet {k} differs from the single-planet path"
+
+
+def test_batch_graph_replays_and_chunks(meshes, plain):
+    """Replaying the captured step graph, and running the erosion in chunks
+    of 7 steps, give the same heights."""
+    runs = _runs(_jobs(), meshes)
+    br = batch.BatchRunner()
+    for rep in range(3):
+        br.run(runs, use_graph=True)
+        torch.cuda.synchronize()
+        for k, (a, b) in enumerate(zip(_heights(runs), plain)):
+            assert (a == b).all(), f"replay {rep} planet {k}"
+    br.noise(runs)
+    while any(r.state is not None and r.steps_done < r.job.erosion_iters for r in runs):
+        br.erode(runs, 7, use_graph=False)
+    br.store(runs)
+    for k, (a, b) in enumerate(zip(_heights(runs), plain)):
+        assert (a == b).all(), f"chunked planet {k}"
+
+
+def test_batch_alone_equals_in_mix(meshes, plain):
+    """A planet's result is the same alone and inside a larger batch."""
+    jobs = _jobs()
+    for k in (1, 5):
+        runs = _runs([jobs[k]], meshes)
+        batch.BatchRunner().run(runs)
+        assert (_heights(runs)[0] == plain[k]).all()
+
+
+def test_batch_matches_oracle(oracle, meshes):
+    job = batch.Job(d=14, seed=1234, n_octaves=8, erosion_iters=30, rain=1.0)
+    run = _runs([job], meshes)[0]
+    batch.BatchRunner().run([run])
+    got = _heights([run])[0]
+    from nixis_amd import util
+    pts, nb = util.icosphere_with_neighbors(14)
+    perm, pgi = oracle.init(1234)
+    h = oracle.sample_octaves(pts, None, perm, pgi, 8)
+    h = oracle.rescale(h, -9000, 9000)
+    oracle.erode_terrain3(pts, nb, h, 30, 1.0)
+    assert np.abs(got - h).max() <= 1e-4 * (h.max() - h.min())
+
+
+def test_autotune_picks_and_persists(tmp_path, meshes):
+    runs = _runs(_jobs()[:4], meshes)
+    br = batch.BatchRunner()
+    path = str(tmp_path / "tune.json")
+    best, table = autotune.tune(br, runs, reps=1, path=path, tiles=(2048, 8192), bpcs=(2, 8))
+    assert len(table) == 4 and best.tile in (2048, 8192)
+    again = autotune.lookup(runs, path=path)
+    assert again == best

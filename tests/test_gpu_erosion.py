@@ -155,3 +155,38 @@ def test_erosion_rejects_bad_args():
         st.run(g, -1)
     with pytest.raises(ValueError):
         erosion.ErosionGraph(pts, nb[:, :5])
+
+
+def test_erosion_graph_replays_match_eager():
+    """Captured chunks (16 steps) replayed several times, with eager work and
+    synchronisations in between, equal the eager launches bit for bit."""
+    pts, nb, h = _planet(20)
+    g = erosion.ErosionGraph(pts, nb)
+    ctx = _lib.Context.get(g.device)
+    dh = torch.from_numpy(h).cuda()
+
+    def run(chunk):
+        ctx.set_graph_chunk(chunk)
+        st = erosion.ErosionState(g.V)
+        st.load(dh)
+        st.run(g, 100, 1.0)
+        out = torch.empty(g.V, dtype=torch.float64, device="cuda")
+        st.store(out)
+        ctx.sync_check()
+        return out.cpu().numpy(), st
+
+    try:
+        eager, _ = run(0)
+        ctx.set_graph_chunk(16)
+        st = erosion.ErosionState(g.V)
+        for rep in range(3):
+            st.load(dh)
+            st.run(g, 100, 1.0)          # 6 replays of the 16-step graph + 4
+            out = torch.empty(g.V, dtype=torch.float64, device="cuda")
+            st.store(out)
+            # eager work between replays
+            util.rescale_device(dh, -1, 1)
+            torch.cuda.synchronize()
+            assert (out.cpu().numpy() == eager).all(), f"replay {rep} differs"
+    finally:
+        ctx.set_graph_chunk(32)
