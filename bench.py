@@ -51,6 +51,11 @@ def parse():
     ap.add_argument("--erosion-iters", type=int, default=500)
     ap.add_argument("--noise-d", type=int, default=2000)
     ap.add_argument("--erosion-d", type=int, default=1000)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--served-jobs", type=int, default=120)
+    ap.add_argument("--served-load", type=float, default=0.8,
+                    help="arrival rate as a fraction of the plain form's capacity")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run three plain 1 GiB device copies (for scaling "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -209,6 +214,106 @@ def cpu_erosion_baseline(er, iters=4):
                                       "FP64 oracle with OpenMP"}
 
 
+def _events_ms(fn, stream=None):
+    import torch
+    s = stream or torch.cuda.current_stream()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    a.record(s)
+    fn()
+    b.record(s)
+    b.synchronize()
+    return a.elapsed_time(b)
+
+
+def bench_batched(args):
+    """64 planets, d log-uniform in [64, 1024], seeds 0..63, octaves 4..16,
+    0..200 erosion steps: one batched run (persistent noise launch, batched
+    rescale/load, one graph-captured launch per erosion step) against the
+    same jobs one planet per call."""
+    from nixis_amd import autotune, batch, terrain
+    jobs = batch.batched_jobs()
+    t0 = time.perf_counter()
+    meshes = batch.MeshCache()
+    runs = [batch.PlanetRun(j, meshes) for j in jobs]
+    log(f"[batched] {len(jobs)} planets, {sum(j.V for j in jobs)} vertices, meshes in "
+        f"{time.perf_counter() - t0:.1f}s")
+    br = batch.BatchRunner()
+    tuning = autotune.lookup(runs)
+    tuned_now = tuning is None
+    if tuned_now:
+        tuning, table = autotune.tune(br, runs, reps=2)
+        log(f"[batched] autotuned tile={tuning.tile} blocks/CU={tuning.blocks_per_cu}")
+    br.run(runs, tuning)                       # warm (captures the step graph)
+    batch.run_plain(runs)
+    ms_b = min(_events_ms(lambda: br.run(runs, tuning)) for _ in range(2))
+    ms_noise = _events_ms(lambda: br.noise(runs, tuning))
+    ms_p = min(_events_ms(lambda: batch.run_plain(runs)) for _ in range(2))
+    def plain_noise():
+        for r in runs:
+            terrain.sample_octaves_device(r.verts, None, r.perm, r.pgi, r.rough, r.strength,
+                                          1.0, out=r.elev)
+    ms_pnoise = _events_ms(plain_noise)
+    vo = sum(j.vertex_octaves for j in jobs)
+    vi = sum(j.vertex_iters for j in jobs)
+    log(f"[batched] batched {ms_b:.1f} ms (noise {ms_noise:.2f} ms), plain {ms_p:.1f} ms "
+        f"(noise {ms_pnoise:.2f} ms)")
+    return {
+        "workload": "64 planets: d log-uniform in [64,1024] (even), seeds 0..63, octaves "
+                    "U{4..16}, erosion steps U{0..200}, rescale to [-9000,9000], rain 0.3/320; "
+                    "job list from numpy default_rng(20261202)",
+        "planets": len(jobs), "vertices": sum(j.V for j in jobs),
+        "vertex_octaves": vo, "vertex_iters": vi,
+        "batched_ms": round(ms_b, 3), "plain_ms": round(ms_p, 3),
+        "speedup_vs_plain": round(ms_p / ms_b, 3),
+        "noise_batched_ms": round(ms_noise, 3), "noise_plain_ms": round(ms_pnoise, 3),
+        "Gvertex_octaves_per_s_noise_phase": round(vo / (ms_noise * 1e-3) / 1e9, 2),
+        "Gvertex_iters_per_s_whole_batch": round(vi / (ms_b * 1e-3) / 1e9, 2),
+        "tuning": {"tile": tuning.tile, "blocks_per_cu": tuning.blocks_per_cu,
+                   "source": "autotuned now" if tuned_now else "autotune cache"},
+    }
+
+
+def bench_served(args):
+    """Poisson arrivals of planet jobs (d log-uniform in [32,2048], octaves
+    4..16, 30% with 100..300 erosion steps) served with continuous batching
+    under an HBM budget, against the plain one-planet-per-call form on the
+    same trace.  The arrival rate is set to `served_load` times the plain
+    form's capacity, measured by running the jobs back to back."""
+    from nixis_amd import batch, serve
+    n = args.served_jobs
+    proto = serve.served_jobs(n, rate=1.0)
+    t0 = time.perf_counter()
+    log(f"[served] preparing meshes for {len(set(j.d for j in proto))} subdivisions")
+    plain = serve.PlainServer()
+    served = serve.PlanetServer(hbm_budget=64 << 30, steps_per_tick=16)
+    serve._prepare_meshes(plain.meshes, proto)
+    served.meshes = plain.meshes   # same per-subdivision assets for both
+    log(f"[served] {n} jobs, meshes in {time.perf_counter() - t0:.1f}s")
+    # capacity of the plain form: all jobs back to back
+    b2b = [batch.Job(**{**j.__dict__, "arrival": 0.0}) for j in proto]
+    plain.serve(b2b[: min(8, n)], prepare_meshes=False)       # warm
+    outs = plain.serve(b2b, prepare_meshes=False)
+    busy = max(o.done for o in outs)
+    rate = args.served_load * n / busy
+    jobs = serve.served_jobs(n, rate=rate)
+    log(f"[served] plain back-to-back busy {busy:.2f}s -> rate {rate:.1f} jobs/s")
+    res_plain = serve.latency_summary(plain.serve(jobs, prepare_meshes=False))
+    res_served = serve.latency_summary(served.serve(jobs, prepare_meshes=False))
+    log(f"[served] served p50 {res_served['p50_ms']:.1f} ms p99 {res_served['p99_ms']:.1f} ms; "
+        f"plain p50 {res_plain['p50_ms']:.1f} ms p99 {res_plain['p99_ms']:.1f} ms")
+    return {
+        "workload": f"{n} jobs, Poisson arrivals at {rate:.2f} jobs/s ({args.served_load:.0%} of "
+                    "the plain form's back-to-back capacity), d log-uniform in [32,2048], "
+                    "octaves U{4..16}, 30% with U{100..300} erosion steps; trace from numpy "
+                    "default_rng(20261203)",
+        "rate_jobs_per_s": round(rate, 3), "hbm_budget_GB": 64,
+        "served": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in res_served.items()},
+        "plain": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in res_plain.items()},
+        "peak_job_bytes": int(served.peak_bytes),
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -258,6 +363,10 @@ def main():
         for k in [k for k in er if k.startswith("_")]:
             er.pop(k)
         line["erosion"] = er
+    if not args.no_batched:
+        line["batched"] = bench_batched(args)
+    if not args.no_served:
+        line["served"] = bench_served(args)
     print(json.dumps(line), flush=True)
 
 

@@ -91,6 +91,11 @@ def lib():
     if _lib is None:
         with _lock:
             if _lib is None:
+                # torch first: its HIP runtime (libamdhip64.so.7) then also
+                # satisfies this library.  Loaded the other way round, torch
+                # pulls in a second HIP/HSA runtime whose streams and
+                # ordering are not this library's.
+                import torch  # noqa: F401
                 if not os.path.exists(LIB_PATH):
                     raise ImportError(
                         f"{LIB_PATH} is missing: build it with `python -m nixis_amd.build` "
@@ -161,6 +166,37 @@ def stream_ptr(stream=None):
     if stream is None:
         stream = torch.cuda.current_stream()
     return _c_void_p(stream.cuda_stream)
+
+
+_side = {}
+
+
+def side_stream(device=None):
+    """A second stream on `device`: non-blocking, default priority, created
+    once per device and kept for the life of the process, wrapped as a
+    torch.cuda.ExternalStream.  torch's own stream pool is not used because
+    creating it also creates its raised-priority streams."""
+    import torch
+    if device is None:
+        device = torch.cuda.current_device()
+    if isinstance(device, torch.device):
+        device = device.index if device.index is not None else torch.cuda.current_device()
+    lib()
+    with _lock:
+        s = _side.get(device)
+        if s is None:
+            hip = ctypes.CDLL("libamdhip64.so.7")  # the runtime already loaded by torch
+            fn = hip.hipStreamCreateWithFlags
+            fn.restype = _c_int
+            fn.argtypes = [ctypes.POINTER(_c_void_p), ctypes.c_uint]
+            h = _c_void_p()
+            with torch.cuda.device(device):
+                rc = fn(ctypes.byref(h), 1)  # hipStreamNonBlocking
+            if rc != 0 or not h.value:
+                raise NixisError(NX_EHIP, f"hipStreamCreateWithFlags failed ({rc})")
+            s = torch.cuda.ExternalStream(h.value, device=torch.device("cuda", device))
+            _side[device] = s
+    return s
 
 
 def ptr(t):

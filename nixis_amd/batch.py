@@ -194,10 +194,14 @@ class BatchRunner:
         self._ws = None
 
     def _workspace(self, n):
+        """Descriptor workspace, allocated on this runner's stream so the
+        caching allocator only reuses it in that stream's order (the kernels
+        of earlier calls on the stream may still be reading it)."""
         import torch
         need = int(_L().nx_batch_workspace_bytes(n))
         if self._ws is None or self._ws.numel() < need:
-            self._ws = torch.empty(need, dtype=torch.uint8, device=self.device)
+            with torch.cuda.stream(self.stream or torch.cuda.current_stream(self.device)):
+                self._ws = torch.empty(need, dtype=torch.uint8, device=self.device)
         return self._ws
 
     @staticmethod

@@ -1,0 +1,274 @@
+"""Served planet generation: continuous batching under an HBM budget.
+
+Jobs (nixis_amd.batch.Job) arrive over time.  ``Admission`` decides, in
+arrival order, which jobs fit the HBM budget for per-job device fields: the
+FP64 heights and, for a job that erodes, its float-float erosion workspace.
+Meshes (vertices and neighbour tables) depend only on the subdivision count;
+they are shared assets provisioned before the trace starts.  A job whose
+fields exceed the whole budget is rejected; others wait, first come first
+served, until enough of the budget is released by finished jobs.
+
+``PlanetServer.serve`` runs a trace in real time.  Each tick launches one
+batched noise pass (on the noise stream) for the jobs admitted since the
+last tick and one batched chunk of erosion steps (on the erosion stream) for
+every job still eroding, so jobs that arrive while others are on the GPU
+join the next tick (continuous batching) and noise-only jobs do not wait
+behind erosion.  At most two ticks are in flight.  Completion times come
+from GPU events relative to an event recorded at the start of the trace.
+Results stay on the device.  ``PlainServer`` runs the same trace one planet
+per call, in arrival order, for comparison.
+"""
+import os
+import sys
+import time
+from collections import deque
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from . import batch as _batch
+
+
+def job_bytes(job):
+    """Device bytes of a job's own fields."""
+    from ._lib import lib
+    b = 8 * job.V
+    if job.erosion_iters > 0:
+        b += int(lib().nx_erosion_state_bytes(job.V))
+    return b
+
+
+class Admission:
+    """First-come-first-served admission against an HBM budget."""
+
+    def __init__(self, budget, job_cost=job_bytes):
+        self.budget = int(budget)
+        self.job_cost = job_cost
+        self.used = 0
+        self.peak = 0
+
+    def fits_ever(self, job):
+        return self.job_cost(job) <= self.budget
+
+    def try_admit(self, job):
+        need = self.job_cost(job)
+        if self.used + need > self.budget:
+            return False
+        self.used += need
+        self.peak = max(self.peak, self.used)
+        return True
+
+    def release(self, job):
+        self.used -= self.job_cost(job)
+        assert self.used >= 0
+
+
+@dataclass
+class Outcome:
+    job: _batch.Job
+    admitted: float = float("nan")
+    done: float = float("nan")
+    rejected: bool = False
+    heights: object = None   # device tensor (served) or None
+
+    @property
+    def latency(self):
+        return self.done - self.job.arrival
+
+
+def latency_summary(outcomes, work_start=0.0):
+    lat = np.array([o.latency for o in outcomes if not o.rejected])
+    done = max(o.done for o in outcomes if not o.rejected)
+    vo = sum(o.job.vertex_octaves for o in outcomes if not o.rejected)
+    vi = sum(o.job.vertex_iters for o in outcomes if not o.rejected)
+    span = done - work_start
+    return {
+        "jobs": int(lat.size),
+        "rejected": int(sum(o.rejected for o in outcomes)),
+        "p50_ms": float(np.percentile(lat, 50) * 1e3),
+        "p99_ms": float(np.percentile(lat, 99) * 1e3),
+        "mean_ms": float(lat.mean() * 1e3),
+        "makespan_s": float(span),
+        "jobs_per_s": float(lat.size / span),
+        "Gvertex_octaves_per_s": float(vo / span / 1e9),
+        "Gvertex_iters_per_s": float(vi / span / 1e9),
+    }
+
+
+class _Clock:
+    """Host clock and GPU event timeline sharing one origin."""
+
+    def __init__(self, stream):
+        import torch
+        self.stream = stream
+        self.t0_event = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        self.t0_event.record(stream)
+        self.t0_event.synchronize()
+        self.t0 = time.perf_counter()
+
+    def now(self):
+        return time.perf_counter() - self.t0
+
+    def event(self, stream=None):
+        import torch
+        e = torch.cuda.Event(enable_timing=True)
+        e.record(stream or self.stream)
+        return e
+
+    def gpu_time(self, e):
+        return self.t0_event.elapsed_time(e) * 1e-3
+
+
+def _prepare_meshes(meshes, jobs):
+    """Per-subdivision assets (vertices, neighbour tables) built up front."""
+    for j in jobs:
+        meshes.verts(j.d)
+        if j.erosion_iters > 0:
+            meshes.graph(j.d)
+
+
+class PlanetServer:
+    def __init__(self, hbm_budget=64 << 30, steps_per_tick=16, tuning=None, device=None):
+        import torch
+        self.device = device or torch.device("cuda", torch.cuda.current_device())
+        self.stream = torch.cuda.current_stream(self.device)
+        self.erode_stream = _lib.side_stream(self.device)
+        self.meshes = _batch.MeshCache(self.device)
+        self.noise_runner = _batch.BatchRunner(self.device, self.stream)
+        self.erode_runner = _batch.BatchRunner(self.device, self.erode_stream)
+        self.budget = int(hbm_budget)
+        self.steps_per_tick = int(steps_per_tick)
+        self.tuning = tuning
+        self.trace = bool(os.environ.get("NX_SERVE_TRACE"))
+
+    def serve(self, jobs, keep_results=False, prepare_meshes=True):
+        jobs = sorted(jobs, key=lambda j: j.arrival)
+        if prepare_meshes:
+            _prepare_meshes(self.meshes, jobs)
+        adm = Admission(self.budget)
+        outcomes = {id(j): Outcome(j) for j in jobs}
+        pending = deque(jobs)
+        waiting = deque()
+        eroding = []
+        inflight = deque()   # (event, [runs completed by it])
+        clock = _Clock(self.stream)
+        while pending or waiting or eroding or inflight:
+            now = clock.now()
+            while pending and pending[0].arrival <= now:
+                j = pending.popleft()
+                if adm.fits_ever(j):
+                    waiting.append(j)
+                else:
+                    outcomes[id(j)].rejected = True
+            admitted = []
+            while waiting and adm.try_admit(waiting[0]):
+                j = waiting.popleft()
+                outcomes[id(j)].admitted = now
+                admitted.append(_batch.PlanetRun(j, self.meshes))
+            if admitted:
+                self.noise_runner.noise(admitted, self.tuning)
+                ev = clock.event(self.stream)
+                done_now = [r for r in admitted if r.state is None]
+                inflight.append((ev, done_now))
+                new_erode = [r for r in admitted if r.state is not None]
+                if new_erode:
+                    self.erode_stream.wait_event(ev)
+                    for r in new_erode:
+                        # allocated on the noise stream, used on the erosion
+                        # stream: keep the allocator from reusing them early
+                        r.elev.record_stream(self.erode_stream)
+                        r.state.buf.record_stream(self.erode_stream)
+                    eroding += new_erode
+            if eroding:
+                self.erode_runner.erode(eroding, self.steps_per_tick, use_graph=False)
+                finished = [r for r in eroding if r.steps_done >= r.job.erosion_iters]
+                if finished:
+                    self.erode_runner.store(finished)
+                    eroding = [r for r in eroding if r.steps_done < r.job.erosion_iters]
+                inflight.append((clock.event(self.erode_stream), finished))
+            if self.trace:
+                print(f"[serve] t={now:.4f} pending={len(pending)} waiting={len(waiting)} "
+                      f"admitted={len(admitted)} eroding={len(eroding)} "
+                      f"inflight={len(inflight)} used={adm.used}", file=sys.stderr, flush=True)
+            # Keep at most two erosion ticks queued; retire what has finished.
+            while inflight and (len(inflight) > 3 or inflight[0][0].query()):
+                ev, done_runs = inflight.popleft()
+                ev.synchronize()
+                t = clock.gpu_time(ev)
+                for r in done_runs:
+                    o = outcomes[id(r.job)]
+                    o.done = t
+                    if keep_results:
+                        o.heights = r.elev
+                    adm.release(r.job)
+            if not (admitted or eroding) and pending and not waiting and not inflight:
+                time.sleep(max(0.0, min(0.002, pending[0].arrival - clock.now())))
+            elif not (admitted or eroding) and inflight:
+                time.sleep(0.0002)
+        self.peak_bytes = adm.peak
+        return [outcomes[id(j)] for j in jobs]
+
+
+class PlainServer:
+    """One planet per call, in arrival order, through the single-planet API."""
+
+    def __init__(self, device=None):
+        import torch
+        self.device = device or torch.device("cuda", torch.cuda.current_device())
+        self.stream = torch.cuda.current_stream(self.device)
+        self.meshes = _batch.MeshCache(self.device)
+
+    def serve(self, jobs, keep_results=False, prepare_meshes=True):
+        jobs = sorted(jobs, key=lambda j: j.arrival)
+        if prepare_meshes:
+            _prepare_meshes(self.meshes, jobs)
+        outcomes = []
+        marks = []
+        clock = _Clock(self.stream)
+        for j in jobs:
+            wait = j.arrival - clock.now()
+            if wait > 0:
+                time.sleep(wait)
+            o = Outcome(j, admitted=max(j.arrival, clock.now()))
+            r = _batch.PlanetRun(j, self.meshes)
+            _batch.run_plain([r], stream=self.stream)
+            marks.append((clock.event(), o, r))
+            outcomes.append(o)
+            # retire finished jobs so their buffers can be freed
+            while marks and marks[0][0].query():
+                ev, oo, rr = marks.pop(0)
+                oo.done = clock.gpu_time(ev)
+                if keep_results:
+                    oo.heights = rr.elev
+        for ev, oo, rr in marks:
+            ev.synchronize()
+            oo.done = clock.gpu_time(ev)
+            if keep_results:
+                oo.heights = rr.elev
+        return outcomes
+
+
+def served_jobs(n, rate, seed=20261203, d_range=(32, 2048), oct_range=(4, 16),
+                erosion_frac=0.3, iter_range=(100, 300)):
+    """Served-form trace: Poisson arrivals at `rate` jobs/s, d log-uniform in
+    d_range (made even), random seeds, octaves uniform in oct_range, and a
+    fraction `erosion_frac` of jobs with erosion steps uniform in iter_range.
+    Job attributes and arrival gaps come from separate generators seeded by
+    `seed`, so changing `rate` only rescales the arrival times."""
+    rng = np.random.default_rng(seed)
+    gaps = np.random.default_rng(seed + 1).exponential(1.0, size=n) / rate
+    lo, hi = np.log(d_range[0]), np.log(d_range[1])
+    t = 0.0
+    jobs = []
+    for i in range(n):
+        t += float(gaps[i])
+        d = _batch.even_divisions(int(round(float(np.exp(rng.uniform(lo, hi))))))
+        o = int(rng.integers(oct_range[0], oct_range[1] + 1))
+        erode = bool(rng.uniform() < erosion_frac)
+        it = int(rng.integers(iter_range[0], iter_range[1] + 1)) if erode else 0
+        seed_i = int(rng.integers(0, 2**31 - 1))
+        jobs.append(_batch.Job(d=d, seed=seed_i, n_octaves=o, erosion_iters=it, arrival=t,
+                               name=f"s{i:03d}"))
+    return jobs

@@ -1,0 +1,102 @@
+"""The C ABI library loads on a host without a GPU and exports every entry
+point that include/nixis.h declares; its host-only entry points work there."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from nixis_amd import _lib, batch, util
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "nixis.h")
+
+
+def _declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    names = set(re.findall(r"\b(nx_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
+
+
+def test_header_declares_the_entry_points():
+    names = _declared()
+    assert len(names) >= 20
+    for must in ("nx_sample_octaves", "nx_noise3d", "nx_rescale", "nx_erosion_run",
+                 "nx_batch_noise", "nx_batch_erode", "nx_init_tables"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    missing = [n for n in _declared() if not hasattr(L, n)]
+    assert not missing, f"declared in include/nixis.h but not exported: {missing}"
+
+
+def test_bindings_cover_every_declared_symbol():
+    bound = set(_lib.SIGNATURES) | set(batch._SIGS)
+    assert set(_declared()) == bound
+
+
+def test_abi_version_and_error_string():
+    L = _lib.lib()
+    assert L.nx_abi_version() >= 1
+    assert isinstance(L.nx_last_error(), bytes)
+
+
+def test_init_tables_host_entry_point(oracle):
+    perm = np.empty(256, np.int32)
+    pgi = np.empty(256, np.int32)
+    for seed in (0, 7, -3, 2**62):
+        _lib.check(_lib.lib().nx_init_tables(seed, _lib.ptr(perm), _lib.ptr(pgi)))
+        p2, g2 = oracle.init(seed)
+        assert (perm == p2).all() and (pgi == g2).all()
+
+
+@pytest.mark.parametrize("d", [1, 2, 5, 8])
+def test_mesh_entry_points_match_oracle(oracle, d):
+    """Icosphere counts, adjacency and its cyclic sort (host C++) equal the
+    oracle's restatement of util.py:590-662 on the same triangles."""
+    pts, cells = util.create_mesh(d)
+    V = 10 * d * d + 2
+    assert pts.shape == (V, 3) and cells.shape == (20 * d * d, 3)
+    assert np.allclose(np.linalg.norm(pts, axis=1), 1.0, atol=1e-15)
+    adj = util.build_adjacency(cells)
+    ref = oracle.build_adjacency(cells, V)
+    assert (adj == ref).all()
+    util.sort_adjacency(adj)
+    oracle.sort_adjacency(ref)
+    assert (adj == ref).all()
+
+
+def test_ell_layout_host_entry_point():
+    pts, nb = util.icosphere_with_neighbors(3)
+    V = pts.shape[0]
+    col = np.empty((6, V), np.int32)
+    elen = np.empty((6, V, 2), np.float32)
+    order = util.sfc_order(pts)
+    assert sorted(order.tolist()) == list(range(V))
+    L = _lib.lib()
+    _lib.check(L.nx_ell_from_adjacency(_lib.ptr(nb), V, _lib.ptr(pts), None,
+                                       _lib.ptr(col), _lib.ptr(elen)))
+    for i in (0, 11, 12, V - 1):
+        for k in range(6):
+            n = nb[i, k]
+            if n < 0:                      # pentagon pad: self, zero length
+                assert col[k, i] == i
+                continue
+            assert col[k, i] == n
+            d = np.linalg.norm(pts[i] - pts[n])
+            assert abs((float(elen[k, i, 0]) + float(elen[k, i, 1])) - d) <= 1e-14 * d
+
+
+def test_bad_arguments_are_rejected_without_a_gpu():
+    L = _lib.lib()
+    V = ctypes.c_int64()
+    T = ctypes.c_int64()
+    with pytest.raises(_lib.NixisArgError):
+        _lib.check(L.nx_icosphere_counts(0, ctypes.byref(V), ctypes.byref(T)))
+    with pytest.raises(ValueError):
+        util.sort_adjacency(np.zeros((4, 5), np.int32))

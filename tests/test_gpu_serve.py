@@ -1,0 +1,51 @@
+"""Served path: every job's result is the same alone and in the mix."""
+import numpy as np
+import pytest
+import torch
+
+from nixis_amd import batch, serve
+
+pytestmark = pytest.mark.gpu
+
+
+def _trace():
+    jobs = serve.served_jobs(14, rate=400.0, seed=11, d_range=(8, 48), iter_range=(20, 60),
+                             erosion_frac=0.5)
+    return jobs
+
+
+def test_served_results_equal_alone():
+    jobs = _trace()
+    srv = serve.PlanetServer(hbm_budget=64 << 20, steps_per_tick=8)
+    outs = srv.serve(jobs, keep_results=True)
+    torch.cuda.synchronize()
+    assert all(not o.rejected for o in outs)
+    assert all(o.done >= o.job.arrival for o in outs)
+    meshes = batch.MeshCache()
+    for o in outs:
+        r = batch.PlanetRun(o.job, meshes)
+        batch.run_plain([r])
+        torch.cuda.synchronize()
+        assert (r.elev.cpu().numpy() == o.heights.cpu().numpy()).all(), o.job
+    summ = serve.latency_summary(outs)
+    assert summ["jobs"] == len(jobs) and summ["p99_ms"] >= summ["p50_ms"] > 0
+
+
+def test_served_tiny_budget_queues_and_rejects():
+    jobs = _trace()
+    costs = sorted(serve.job_bytes(j) for j in jobs)
+    budget = costs[-2]  # the largest job cannot fit, others queue
+    srv = serve.PlanetServer(hbm_budget=budget, steps_per_tick=8)
+    outs = srv.serve(jobs)
+    rejected = [o for o in outs if o.rejected]
+    assert len(rejected) == 1 and serve.job_bytes(rejected[0].job) == costs[-1]
+    assert srv.peak_bytes <= budget
+    assert all(o.done >= o.admitted >= o.job.arrival - 1e-3 for o in outs if not o.rejected)
+
+
+def test_plain_server_matches_served():
+    jobs = _trace()
+    a = serve.PlainServer().serve(jobs, keep_results=True)
+    b = serve.PlanetServer(steps_per_tick=5).serve(jobs, keep_results=True)
+    for x, y in zip(a, b):
+        assert (x.heights.cpu().numpy() == y.heights.cpu().numpy()).all()

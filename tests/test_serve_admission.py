@@ -1,0 +1,62 @@
+"""Admission control of the served path (host logic, CPU only)."""
+import pytest
+
+from nixis_amd.batch import Job
+from nixis_amd.serve import Admission
+
+
+def cost(job):
+    return job.V  # one "byte" per vertex keeps the arithmetic readable
+
+
+def test_fifo_budget_never_exceeded():
+    adm = Admission(budget=10_000, job_cost=cost)
+    jobs = [Job(d=d, seed=0, n_octaves=4) for d in (20, 10, 25, 8, 30)]
+    # V: 4002, 1002, 6252, 642, 9002
+    admitted = [j for j in jobs if adm.try_admit(j)]
+    assert [j.d for j in admitted] == [20, 10, 8]  # 25 does not fit after 20+10
+    assert adm.used <= adm.budget
+    # try_admit never over-commits, and releases free the budget
+    adm.release(jobs[0])
+    adm.release(jobs[1])
+    assert adm.try_admit(jobs[2])
+    assert adm.used == 642 + 6252 and adm.peak <= adm.budget
+
+
+def test_job_larger_than_budget_is_rejected_up_front():
+    adm = Admission(budget=5_000, job_cost=cost)
+    big = Job(d=30, seed=1, n_octaves=4)
+    assert not adm.fits_ever(big)
+    assert not adm.try_admit(big)
+    assert adm.used == 0
+
+
+def test_waiting_job_admitted_once_budget_frees():
+    adm = Admission(budget=7_000, job_cost=cost)
+    a = Job(d=20, seed=0, n_octaves=4)   # 4002
+    b = Job(d=18, seed=0, n_octaves=4)   # 3242
+    assert adm.try_admit(a)
+    assert not adm.try_admit(b)          # waits
+    adm.release(a)
+    assert adm.try_admit(b)
+
+
+def test_release_below_zero_is_a_bug():
+    adm = Admission(budget=100, job_cost=cost)
+    with pytest.raises(AssertionError):
+        adm.release(Job(d=2, seed=0, n_octaves=1))
+
+
+def test_served_trace_shape():
+    from nixis_amd.serve import served_jobs
+    jobs = served_jobs(400, rate=50.0, seed=7)
+    arr = [j.arrival for j in jobs]
+    assert arr == sorted(arr)
+    assert all(32 <= j.d <= 2048 and j.d % 2 == 0 for j in jobs)
+    assert all(4 <= j.n_octaves <= 16 for j in jobs)
+    frac = sum(j.erosion_iters > 0 for j in jobs) / len(jobs)
+    assert 0.2 < frac < 0.4
+    assert all(100 <= j.erosion_iters <= 300 for j in jobs if j.erosion_iters)
+    # mean inter-arrival ~ 1/rate
+    gaps = [b - a for a, b in zip([0.0] + arr[:-1], arr)]
+    assert abs(sum(gaps) / len(gaps) - 1 / 50.0) < 0.005
