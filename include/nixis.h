@@ -112,8 +112,9 @@ int nx_rescale(nx_ctx *ctx, const double *x, int64_t n, double lower,
 /* ---- erosion (device) -------------------------------------------------- */
 
 /* Fields live in a caller-allocated device workspace of
- * nx_erosion_state_bytes(V) bytes: height, water and sediment as FP32
- * (hi, lo) pairs scaled by a per-planet power of two, double-buffered. */
+ * nx_erosion_state_bytes(V) bytes: height, water and sediment in FP64,
+ * double-buffered.  Every step follows erosion.py:209-267 operation by
+ * operation, so results equal the reference's FP64 results. */
 size_t nx_erosion_state_bytes(int64_t V);
 
 /* Load FP64 fields (device f64 [V]; w and s may be NULL for zeros). */
@@ -126,9 +127,46 @@ int nx_erosion_load(nx_ctx *ctx, void *state, int64_t V, const double *h,
  * for a bare erosion_iteration3).  Multi-step runs replay captured
  * hipGraphs of NX_OPT_GRAPH_CHUNK steps, cached per context.
  *   col   device int32 [6,V]  neighbour of each slot (self for a pad slot)
- *   elen  device f32 [6,V,2]  edge length of each slot as (hi, lo) */
+ *   elen  device f64 [6,V]    edge length of each slot (0 for a pad) */
 int nx_erosion_run(nx_ctx *ctx, void *state, int64_t V, const int32_t *col,
-                   const float *elen, int iters, double rain, void *stream);
+                   const double *elen, int iters, double rain, void *stream);
+
+/* Tile plan for running erosion steps two per launch.  The vertices (in
+ * device order) are cut into contiguous tiles of at most max_tile vertices
+ * whose neighbours (ring 1) and neighbours' neighbours (ring 2) bring the
+ * total to at most max_load; each tile carries its rings' global indices
+ * and, for tile + ring 1, the neighbour slots renumbered into that local
+ * set plus their edge lengths.  The plan is a byte image: nx_tile_plan_build
+ * fills it on the host, the caller copies it to device memory (256-byte
+ * aligned) and stores that address in `blob`.  The same results as
+ * nx_erosion_run (erosion.py:197-279 per step), with about half the HBM
+ * traffic per step. */
+typedef struct nx_tile_plan {
+    void *blob;          /* device copy of the plan image (set by the caller) */
+    int64_t bytes;       /* size of the plan image */
+    int64_t V;
+    int64_t ntiles;
+    int32_t max_tile;    /* largest tile (owned vertices), <= 1024 */
+    int32_t max_load;    /* largest tile + ring 1 + ring 2, <= 1536 */
+    int64_t off_start, off_cs, off_load, off_rbase, off_ring, off_cbase,
+            off_lcol, off_lel;   /* byte offsets of the arrays in the image */
+} nx_tile_plan;
+
+/* Build the plan from the host copies of col [6,V] and elen [6,V] (as
+ * written by nx_ell_from_adjacency).  With out == NULL or cap < the size,
+ * only fills `plan` (bytes, counts, offsets) and returns NX_OK; call again
+ * with a buffer of plan->bytes.  Returns NX_ERANGE when some tile cannot be
+ * cut small enough for its rings to fit (a graph without locality): no plan,
+ * use nx_erosion_run. */
+int nx_tile_plan_build(const int32_t *col, const double *elen, int64_t V,
+                       void *out, int64_t cap, nx_tile_plan *plan);
+
+/* nx_erosion_run with a tile plan (NULL plan: exactly nx_erosion_run).
+ * Steps go two per launch; an odd last step runs alone. */
+int nx_erosion_run_tiled(nx_ctx *ctx, void *state, int64_t V,
+                         const int32_t *col, const double *elen,
+                         const nx_tile_plan *plan, int iters, double rain,
+                         void *stream);
 
 /* Store the current fields back as FP64 (any output may be NULL). */
 int nx_erosion_store(nx_ctx *ctx, const void *state, int64_t V, double *h,
@@ -160,7 +198,7 @@ typedef struct nx_planet {
     double lower, upper;     /* rescale bounds; lower >= upper: no rescale */
     void *erosion_state;     /* device, nx_erosion_state_bytes(V), or NULL */
     const int32_t *col;      /* device int32 [6,V] */
-    const float *elen;       /* device f32 [6,V,2] */
+    const double *elen;      /* device f64 [6,V] */
     int64_t erosion_iters;   /* erosion steps of the whole job */
     double rain;             /* rain per step (reference: 0.3/320) */
     int64_t steps_done;      /* set to 0 by nx_batch_noise, advanced by
@@ -206,10 +244,10 @@ int nx_sort_adjacency(int32_t *adj, int64_t V);
 /* Space-filling-curve (3D Morton) vertex order: order[new] = old. */
 int nx_sfc_order(const double *verts, int64_t V, int64_t *order);
 /* Device neighbour layout from a [V,6] table (host arrays): col int32 [6,V]
- * and elen f32 [6,V,2]; order (nullable) renumbers vertices as in
+ * and elen f64 [6,V]; order (nullable) renumbers vertices as in
  * nx_sfc_order.  Edge lengths follow erosion.py:38-40 in FP64. */
 int nx_ell_from_adjacency(const int32_t *adj, int64_t V, const double *verts,
-                          const int64_t *order, int32_t *col, float *elen);
+                          const int64_t *order, int32_t *col, double *elen);
 
 #ifdef __cplusplus
 }

@@ -43,6 +43,17 @@ class NixisBusyError(NixisError):
 
 
 _c_void_p = ctypes.c_void_p
+
+
+class NxTilePlan(ctypes.Structure):
+    """Mirror of nx_tile_plan (include/nixis.h)."""
+    _fields_ = [("blob", ctypes.c_void_p), ("bytes", ctypes.c_int64), ("V", ctypes.c_int64),
+                ("ntiles", ctypes.c_int64), ("max_tile", ctypes.c_int32),
+                ("max_load", ctypes.c_int32)] + [
+        (f"off_{n}", ctypes.c_int64)
+        for n in ("start", "cs", "load", "rbase", "ring", "cbase", "lcol", "lel")]
+
+
 _c_i64 = ctypes.c_int64
 _c_int = ctypes.c_int
 _c_double = ctypes.c_double
@@ -69,6 +80,10 @@ SIGNATURES = {
                                  _c_void_p, _c_void_p]),
     "nx_erosion_run": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p, _c_void_p, _c_int,
                                 _c_double, _c_void_p]),
+    "nx_tile_plan_build": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p, _c_i64,
+                                    ctypes.POINTER(NxTilePlan)]),
+    "nx_erosion_run_tiled": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p, _c_void_p,
+                                      ctypes.POINTER(NxTilePlan), _c_int, _c_double, _c_void_p]),
     "nx_erosion_store": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p, _c_void_p,
                                   _c_void_p, _c_void_p]),
     "nx_erosion_mass": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p, _c_void_p]),

@@ -114,7 +114,7 @@ class MeshCache:
         g = self._graphs.get(d)
         if g is None:
             pts, nb = _util.icosphere_with_neighbors(d)
-            g = _erosion.ErosionGraph(pts, nb, device=self.device)
+            g = _erosion.ErosionGraph(pts, nb, order=None, device=self.device, tiled=False)
             self._graphs[d] = g
         return g
 

@@ -170,10 +170,23 @@ int nx_erosion_load(nx_ctx *ctx, void *state, int64_t V, const double *h, const 
     return erosion_load(ctx, state, V, h, w, s, (hipStream_t)stream);
 }
 
-int nx_erosion_run(nx_ctx *ctx, void *state, int64_t V, const int32_t *col, const float *elen,
+int nx_erosion_run(nx_ctx *ctx, void *state, int64_t V, const int32_t *col, const double *elen,
                    int iters, double rain, void *stream)
 {
-    return erosion_run(ctx, state, V, col, elen, iters, rain, (hipStream_t)stream);
+    return erosion_run(ctx, state, V, col, elen, nullptr, iters, rain, (hipStream_t)stream);
+}
+
+int nx_erosion_run_tiled(nx_ctx *ctx, void *state, int64_t V, const int32_t *col,
+                         const double *elen, const nx_tile_plan *plan, int iters, double rain,
+                         void *stream)
+{
+    return erosion_run(ctx, state, V, col, elen, plan, iters, rain, (hipStream_t)stream);
+}
+
+int nx_tile_plan_build(const int32_t *col, const double *elen, int64_t V, void *out, int64_t cap,
+                       nx_tile_plan *plan)
+{
+    return tile_plan_build(col, elen, V, out, cap, plan);
 }
 
 int nx_erosion_store(nx_ctx *ctx, const void *state, int64_t V, double *h, double *w, double *s,
@@ -238,7 +251,7 @@ int nx_sfc_order(const double *verts, int64_t V, int64_t *order)
 }
 
 int nx_ell_from_adjacency(const int32_t *adj, int64_t V, const double *verts, const int64_t *order,
-                          int32_t *col, float *elen)
+                          int32_t *col, double *elen)
 {
     const int rc = ell_from_adjacency(adj, V, verts, order, col, elen);
     return rc ? set_error(rc, "ell_from_adjacency: bad table or order") : NX_OK;

@@ -9,10 +9,10 @@
 //     planet.  Tile size and blocks per CU come from the autotuner.
 //   * rescale + erosion load: per-planet min/max (one atomic per block per
 //     planet) then the map of util.py:139, which also writes the planet's
-//     float-float erosion state with the first rain already added.
+//     FP64 erosion state with the first rain already added.
 //   * erosion: one launch per step covers every planet that still has steps
 //     left (planets ordered by remaining steps, so the active set is a
-//     prefix); each planet keeps its own exponent, flags and step counter in
+//     prefix); each planet keeps its own step counter and buffer index in
 //     its workspace, so a planet's result does not depend on its batch.
 //     The step sequence can be captured into a hipGraph.
 // Each planet's arithmetic is the single-planet kernels' arithmetic.
@@ -43,10 +43,10 @@ struct BatchPlanetDev {
     int32_t do_rescale;
     int32_t do_erode;
     ErosionMeta *meta;
-    float4 *hw0, *hw1;
-    float2 *s0, *s1;
+    double2 *hw0, *hw1;
+    double *s0, *s1;
     const int32_t *col;
-    const float2 *elen;
+    const double *elen;
     int64_t total_iters;
     double rain;
     int64_t nblocks;  // ceil(V / 256)
@@ -210,21 +210,17 @@ __global__ __launch_bounds__(256) void k_batch_map(const BatchPlanetDev *__restr
     if (!D.do_rescale && !D.do_erode) return;
     const double x_min = dunkey(keys[2 * p]), x_max = dunkey(keys[2 * p + 1]);
     const double x_range = x_max - x_min, new_range = D.upper - D.lower;
-    double mx = fmax(fabs(x_min), fabs(x_max));
-    if (D.do_rescale) mx = fmax(fabs(D.lower), fabs(D.upper));
-    const int E = (mx > 0.0 && isfinite(mx)) ? ilogb(mx) : 0;
     const int64_t v0 = (blockIdx.x - prefix[p]) * (int64_t)kRsTile;
     if (v0 == 0 && threadIdx.x == 0) {
         if (D.do_rescale && x_range == 0.0)
             __hip_atomic_fetch_or(err, NX_ERR_DIV0_BIT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (D.do_erode) {
             D.meta->it = 0;
-            D.meta->E[2] = E;
-            D.meta->big[0] = D.meta->big[1] = D.meta->big[2] = 0u;
+            D.meta->buf = 0;
         }
     }
     const int64_t v1 = v0 + kRsTile < D.V ? v0 + kRsTile : D.V;
-    const ff rain = ff_ldexp(ff_from_double(D.total_iters > 0 ? D.rain : 0.0), -E);
+    const double rain = D.total_iters > 0 ? D.rain : 0.0;
     for (int64_t i = v0 + threadIdx.x; i < v1; i += blockDim.x) {
         double h = D.elev[i];
         if (D.do_rescale) {
@@ -232,9 +228,9 @@ __global__ __launch_bounds__(256) void k_batch_map(const BatchPlanetDev *__restr
             D.elev[i] = h;
         }
         if (D.do_erode) {
-            const ff hh = ff_from_double(ldexp(h, -E));
-            D.hw0[i] = make_float4(hh.hi, hh.lo, rain.hi, rain.lo);
-            D.s0[i] = make_float2(0.0f, 0.0f);
+            // erosion.py:178-185: zero water and sediment, then the first rain
+            D.hw0[i] = make_double2(h, 0.0 + rain);
+            D.s0[i] = 0.0;
         }
     }
 }
@@ -249,27 +245,16 @@ __global__ __launch_bounds__(256) void k_erode_batch(const BatchPlanetDev *__res
     if (b >= nb_total) return;
     const int r = find_slot(prefix, active, b);
     const BatchPlanetDev &D = P[order[r]];
-    ErosionMeta *meta = D.meta;
-    const int64_t it = meta->it + k;
-    const float4 *hw_src = (it & 1) ? D.hw1 : D.hw0;
-    const float2 *s_src = (it & 1) ? D.s1 : D.s0;
-    float4 *hw_dst = (it & 1) ? D.hw0 : D.hw1;
-    float2 *s_dst = (it & 1) ? D.s0 : D.s1;
-    const int cur = (int)(it % 3), prev = (int)((it + 2) % 3), nxt = (int)((it + 1) % 3);
-    const int sh = meta->big[prev] ? 64 : 0;
-    const int E = meta->E[prev] + sh;
-    const float sc = ldexpf(1.0f, -sh);
-    const int64_t lb = b - prefix[r];
-    if (lb == 0 && threadIdx.x == 0) {
-        meta->E[cur] = E;
-        meta->big[nxt] = 0u;
-    }
-    const ff rain = ff_ldexp(ff_from_double(it + 1 < D.total_iters ? D.rain : 0.0), -E);
-    float m = 0.0f;
-    const int64_t i = lb * blockDim.x + threadIdx.x;
-    if (i < D.V)
-        m = erode_vertex_step(i, D.V, hw_src, s_src, hw_dst, s_dst, D.col, D.elen, sc, rain);
-    flag_big(m > kBig, &meta->big[cur]);
+    const ErosionMeta *meta = D.meta;
+    const int64_t t = meta->it + k;
+    const int p = (meta->buf + (int)k) & 1;
+    const double2 *hw_src = p ? D.hw1 : D.hw0;
+    const double *s_src = p ? D.s1 : D.s0;
+    double2 *hw_dst = p ? D.hw0 : D.hw1;
+    double *s_dst = p ? D.s0 : D.s1;
+    const double rain = t + 1 < D.total_iters ? D.rain : 0.0;
+    const int64_t i = (b - prefix[r]) * blockDim.x + threadIdx.x;
+    if (i < D.V) erode_vertex_step(i, D.V, hw_src, s_src, hw_dst, s_dst, D.col, D.elen, rain);
 }
 
 __global__ void k_batch_advance(const BatchPlanetDev *__restrict__ P,
@@ -277,7 +262,11 @@ __global__ void k_batch_advance(const BatchPlanetDev *__restrict__ P,
                                 const int64_t *__restrict__ steps, int n)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < n) P[order[r]].meta->it += steps[r];
+    if (r < n) {
+        ErosionMeta *m = P[order[r]].meta;
+        m->it += steps[r];
+        m->buf ^= (int)(steps[r] & 1);
+    }
 }
 
 __global__ __launch_bounds__(256) void k_batch_store(const BatchPlanetDev *__restrict__ P,
@@ -286,14 +275,11 @@ __global__ __launch_bounds__(256) void k_batch_store(const BatchPlanetDev *__res
     const int p = find_slot(prefix, np, blockIdx.x);
     const BatchPlanetDev &D = P[p];
     if (!D.do_erode) return;
-    const int64_t it = D.meta->it;
-    const int E = D.meta->E[(int)((it + 2) % 3)];
-    const float4 *hw = (it & 1) ? D.hw1 : D.hw0;
+    const double2 *hw = D.meta->buf ? D.hw1 : D.hw0;
     const int64_t v0 = (blockIdx.x - prefix[p]) * (int64_t)kRsTile;
     const int64_t v1 = v0 + kRsTile < D.V ? v0 + kRsTile : D.V;
     for (int64_t i = v0 + threadIdx.x; i < v1; i += blockDim.x) {
-        const float4 v = hw[i];
-        D.elev[i] = ldexp((double)v.x + (double)v.y, E);
+        D.elev[i] = hw[i].x;
     }
 }
 
@@ -328,15 +314,15 @@ static int fill_desc(const nx_planet &q, BatchPlanetDev &d)
             return set_error(NX_EINVAL, "planet erosion state must be 256-aligned with col/elen, "
                                         "iters >= 0 and rain >= 0");
         char *base = (char *)q.erosion_state;
-        const size_t hw = (size_t)a256((size_t)q.V * sizeof(float4)), sd = a256((size_t)q.V * 8);
+        const size_t hw = (size_t)a256((size_t)q.V * sizeof(double2)), sd = a256((size_t)q.V * 8);
         d.do_erode = 1;
         d.meta = (ErosionMeta *)base;
-        d.hw0 = (float4 *)(base + 256);
-        d.hw1 = (float4 *)(base + 256 + hw);
-        d.s0 = (float2 *)(base + 256 + 2 * hw);
-        d.s1 = (float2 *)(base + 256 + 2 * hw + sd);
+        d.hw0 = (double2 *)(base + 256);
+        d.hw1 = (double2 *)(base + 256 + hw);
+        d.s0 = (double *)(base + 256 + 2 * hw);
+        d.s1 = (double *)(base + 256 + 2 * hw + sd);
         d.col = q.col;
-        d.elen = (const float2 *)q.elen;
+        d.elen = q.elen;
         d.total_iters = q.erosion_iters;
         d.rain = q.rain;
     }

@@ -73,8 +73,8 @@ int rescale(nx_ctx *ctx, const double *x, int64_t n, double lower, double upper,
 size_t erosion_state_bytes(int64_t V);
 int erosion_load(nx_ctx *ctx, void *state, int64_t V, const double *h, const double *w,
                  const double *s, hipStream_t stream);
-int erosion_run(nx_ctx *ctx, void *state, int64_t V, const int32_t *col, const float *elen,
-                int iters, double rain, hipStream_t stream);
+int erosion_run(nx_ctx *ctx, void *state, int64_t V, const int32_t *col, const double *elen,
+                const nx_tile_plan *plan, int iters, double rain, hipStream_t stream);
 int erosion_store(nx_ctx *ctx, const void *state, int64_t V, double *h, double *w, double *s,
                   hipStream_t stream);
 int erosion_mass(nx_ctx *ctx, const void *state, int64_t V, double *out, hipStream_t stream);

@@ -3,23 +3,17 @@
 // erosion.py:171-192 (erode_terrain3) adds rain to every vertex and runs
 // erosion.py:197-279 (erosion_iteration3) once per iteration; the latter
 // copies h/w/s, gathers every vertex's neighbours, and copies the results
-// back.  Here the fields stay on the device in a double-buffered workspace
-// and one kernel launch is one Jacobi step:
-//   * fields are float-float (hi, lo) pairs: height+water packed as float4
-//     (one 16-byte gather per neighbour), sediment as float2;
-//   * neighbours come from the slot-major table col[6][V] with float-float
-//     edge lengths elen[6][V] (coalesced per slot);
-//   * the water/sediment magnitudes grow about 2x per step (the vertex's own
-//     water is re-added each step, erosion.py:215,264), so all fields share a
-//     per-planet power-of-two exponent E.  A step whose outputs exceed 2^64
-//     anywhere raises a flag (one wave ballot, then a plain store -- no
-//     read-modify-write atomics on the hot path); the next step then scales
-//     its inputs by 2^-64 (exact: the update is homogeneous in (h, w, s)
-//     and branch decisions only compare heights) and adds the rain at scale
-//     2^-E.  Magnitudes never shrink across the planet, so no scale-up path
-//     is needed; loading normalises the largest input to [1, 2);
-//   * the next step's rain is folded into this step's water output.
-// Workspace: [meta 256 B][hw 2 x V x float4][s 2 x V x float2].
+// back.  Here the fields stay on the device in a double-buffered workspace:
+//   * FP64 fields in the reference's operation order (nx_erosion_dev.h):
+//     height+water packed as double2 (one 16-byte gather per neighbour),
+//     sediment as double;
+//   * neighbours come from the slot-major table col[6][V] with FP64 edge
+//     lengths elen[6][V] (coalesced per slot);
+//   * k_erode_step runs one step per launch; with a tile plan (nx_tiles.cpp)
+//     k_erode_pair runs two steps per launch over LDS-resident tiles, which
+//     halves the HBM traffic per step;
+//   * the next iteration's rain is folded into this step's water output.
+// Workspace: [meta 256 B][hw 2 x V x double2][s 2 x V x double].
 #include <hip/hip_runtime.h>
 
 #include "nx_ctx.h"
@@ -29,8 +23,8 @@ namespace nx {
 
 struct ErosionBufs {
     ErosionMeta *meta;
-    float4 *hw[2];
-    float2 *s[2];
+    double2 *hw[2];
+    double *s[2];
 };
 
 static inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -38,7 +32,7 @@ static inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 size_t erosion_state_bytes(int64_t V)
 {
     if (V <= 0) return 256;
-    return 256 + 2 * align256((size_t)V * sizeof(float4)) + 2 * align256((size_t)V * sizeof(float2));
+    return 256 + 2 * align256((size_t)V * sizeof(double2)) + 2 * align256((size_t)V * sizeof(double));
 }
 
 static ErosionBufs bufs(void *state, int64_t V)
@@ -48,181 +42,224 @@ static ErosionBufs bufs(void *state, int64_t V)
     b.meta = (ErosionMeta *)p;
     p += 256;
     for (int k = 0; k < 2; ++k) {
-        b.hw[k] = (float4 *)p;
-        p += align256((size_t)V * sizeof(float4));
+        b.hw[k] = (double2 *)p;
+        p += align256((size_t)V * sizeof(double2));
     }
     for (int k = 0; k < 2; ++k) {
-        b.s[k] = (float2 *)p;
-        p += align256((size_t)V * sizeof(float2));
+        b.s[k] = (double *)p;
+        p += align256((size_t)V * sizeof(double));
     }
     return b;
 }
 
-__global__ __launch_bounds__(256) void k_erode_step(ErosionMeta *__restrict__ meta,
-                                                    float4 *__restrict__ hw0,
-                                                    float4 *__restrict__ hw1,
-                                                    float2 *__restrict__ s0,
-                                                    float2 *__restrict__ s1, int64_t V,
+// Launch j of a sequence reads the buffer that j earlier launches flipped
+// to; the step counter and buffer come from the workspace so that replays
+// of a captured sequence stay correct.
+__global__ __launch_bounds__(256) void k_erode_step(const ErosionMeta *__restrict__ meta,
+                                                    double2 *__restrict__ hw0,
+                                                    double2 *__restrict__ hw1,
+                                                    double *__restrict__ s0,
+                                                    double *__restrict__ s1, int64_t V,
                                                     const int32_t *__restrict__ col,
-                                                    const float2 *__restrict__ elen,
-                                                    int64_t nblocks, int64_t k, double rain_next)
+                                                    const double *__restrict__ elen,
+                                                    int64_t nblocks, int j, double rain_next)
 {
-    // Step index from the workspace, so replaying captured launches works.
-    const int64_t it = meta->it + k;
-    const float4 *hw_src = (it & 1) ? hw1 : hw0;
-    const float2 *s_src = (it & 1) ? s1 : s0;
-    float4 *hw_dst = (it & 1) ? hw0 : hw1;
-    float2 *s_dst = (it & 1) ? s0 : s1;
-    const int cur = (int)(it % 3), prev = (int)((it + 2) % 3), nxt = (int)((it + 1) % 3);
-    const int sh = meta->big[prev] ? 64 : 0;
-    const int E = meta->E[prev] + sh;
-    const float sc = ldexpf(1.0f, -sh);
+    const int p = (meta->buf + j) & 1;
     const int64_t tile = xcd_tile(blockIdx.x, nblocks);
-    if (tile == 0 && threadIdx.x == 0) {
-        meta->E[cur] = E;
-        meta->big[nxt] = 0u;
-    }
-    float m = 0.0f;
     const int64_t i = tile * blockDim.x + threadIdx.x;
-    if (tile < nblocks && i < V) {
-        const ff rain = ff_ldexp(ff_from_double(rain_next), -E);
-        m = erode_vertex_step(i, V, hw_src, s_src, hw_dst, s_dst, col, elen, sc, rain);
-    }
-    flag_big(m > kBig, &meta->big[cur]);
+    if (tile < nblocks && i < V)
+        erode_vertex_step(i, V, p ? hw1 : hw0, p ? s1 : s0, p ? hw0 : hw1, p ? s0 : s1, col, elen,
+                          rain_next);
 }
 
-// Add rain (scaled by the current exponent) to every vertex's water in the
-// current buffer: the reference's water += rain before the first step.
-__global__ __launch_bounds__(256) void k_add_rain(ErosionMeta *__restrict__ meta,
-                                                  float4 *__restrict__ hw0,
-                                                  float4 *__restrict__ hw1, int64_t V,
+// Two steps per launch over one tile of the plan (nx_tiles.cpp): the tile
+// owns vertices [start, start+T) in device order; ring 1 holds its outside
+// neighbours and ring 2 theirs.  h and w of all three are staged in LDS,
+// the first step runs on tile + ring 1 from LDS, its h and w go back to
+// LDS, and the second step runs on the tile and is written to the other
+// buffer.  Ring-1 vertices are computed by every tile that touches them,
+// with the same inputs and arithmetic, so the values agree.
+constexpr int kTileMax = 1024;   // owned vertices per tile
+constexpr int kLoadMax = 1536;   // tile + both rings
+constexpr int kTileThreads = 512;
+constexpr int kLoadPer = kLoadMax / kTileThreads;
+constexpr int kOwnPer = kTileMax / kTileThreads;
+
+struct TilePlanDev {
+    const int32_t *start;    // [ntiles + 1]
+    const int32_t *cs;       // tile + ring 1 per tile
+    const int32_t *load;     // tile + ring 1 + ring 2 per tile
+    const int64_t *rbase;    // [ntiles + 1] into ring
+    const int32_t *ring;     // global indices: ring 1 then ring 2, per tile
+    const int64_t *cbase;    // [ntiles + 1] prefix of cs
+    const uint16_t *lcol;    // local neighbour index, [6][cs] per tile
+    const double *lel;       // edge length, [6][cs] per tile
+    int64_t ntiles;
+};
+
+__global__ __launch_bounds__(kTileThreads) void k_erode_pair(const ErosionMeta *__restrict__ meta,
+                                                            double2 *__restrict__ hw0,
+                                                            double2 *__restrict__ hw1,
+                                                            double *__restrict__ s0,
+                                                            double *__restrict__ s1,
+                                                            TilePlanDev P, int j, double rain_mid,
+                                                            double rain_next)
+{
+    __shared__ double2 A[kLoadMax];
+    const int64_t tile = xcd_tile(blockIdx.x, P.ntiles);
+    if (tile >= P.ntiles) return;
+    const int p = (meta->buf + j) & 1;
+    const double2 *hw_src = p ? hw1 : hw0;
+    const double *s_src = p ? s1 : s0;
+    double2 *hw_dst = p ? hw0 : hw1;
+    double *s_dst = p ? s0 : s1;
+    const int64_t a = P.start[tile];
+    const int T = (int)(P.start[tile + 1] - a);
+    const int cs = P.cs[tile], L = P.load[tile];
+    const int32_t *ring = P.ring + P.rbase[tile];
+    const int64_t cb = P.cbase[tile];
+    const uint16_t *lcol = P.lcol + 6 * cb;
+    const double *lel = P.lel + 6 * cb;
+
+    for (int li = threadIdx.x; li < L; li += kTileThreads)
+        A[li] = hw_src[li < T ? a + li : ring[li - T]];
+    double sed[kLoadPer];
+#pragma unroll
+    for (int m = 0; m < kLoadPer; ++m) {
+        const int li = threadIdx.x + m * kTileThreads;
+        sed[m] = li < cs ? s_src[li < T ? a + li : ring[li - T]] : 0.0;
+    }
+    __syncthreads();
+
+    double2 res[kLoadPer];
+    int nbr[kOwnPer][6];
+    double len[kOwnPer][6];
+#pragma unroll
+    for (int m = 0; m < kLoadPer; ++m) {
+        const int li = threadIdx.x + m * kTileThreads;
+        res[m] = make_double2(0.0, 0.0);
+        if (li < cs) {
+            const double2 me = A[li];
+            double S = sed[m], W = me.y;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                const int n = lcol[q * cs + li];
+                const double d = lel[q * cs + li];
+                if (m < kOwnPer) {
+                    nbr[m < kOwnPer ? m : 0][q] = n;
+                    len[m < kOwnPer ? m : 0][q] = d;
+                }
+                const double2 nb = A[n];
+                erode_accumulate(me.x, nb.x, nb.y, d, S, W);
+            }
+            double ho, wo, so;
+            erode_finish(me.x, me.y, sed[m], S, W, rain_mid, ho, wo, so);
+            res[m] = make_double2(ho, wo);
+            sed[m] = so;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < kLoadPer; ++m) {
+        const int li = threadIdx.x + m * kTileThreads;
+        if (li < cs) A[li] = res[m];
+    }
+    __syncthreads();
+
+#pragma unroll
+    for (int m = 0; m < kOwnPer; ++m) {
+        const int li = threadIdx.x + m * kTileThreads;
+        if (li < T) {
+            const double2 me = A[li];
+            double S = sed[m], W = me.y;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                const double2 nb = A[nbr[m][q]];
+                erode_accumulate(me.x, nb.x, nb.y, len[m][q], S, W);
+            }
+            double ho, wo, so;
+            erode_finish(me.x, me.y, sed[m], S, W, rain_next, ho, wo, so);
+            hw_dst[a + li] = make_double2(ho, wo);
+            s_dst[a + li] = so;
+        }
+    }
+}
+
+// erosion.py:184-186: water += rain on the current fields (before a step).
+__global__ __launch_bounds__(256) void k_add_rain(const ErosionMeta *__restrict__ meta,
+                                                  double2 *__restrict__ hw0,
+                                                  double2 *__restrict__ hw1, int64_t V,
                                                   double rain)
 {
-    const int64_t it = meta->it;
-    float4 *hw = (it & 1) ? hw1 : hw0;
-    const int prev = (int)((it + 2) % 3);
-    const ff r = ff_ldexp(ff_from_double(rain), -meta->E[prev]);
-    float m = 0.0f;
+    double2 *hw = meta->buf ? hw1 : hw0;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < V) {
-        float4 v = hw[i];
-        const ff w = ff_add(ff{v.z, v.w}, r);
-        v.z = w.hi;
-        v.w = w.lo;
-        hw[i] = v;
-        m = fabsf(w.hi);
-    }
-    flag_big(m > kBig, &meta->big[prev]);
+    if (i < V) hw[i].y = hw[i].y + rain;
 }
 
-// Advance the step counter after `iters` launches (one thread).
-__global__ void k_advance(ErosionMeta *meta, int64_t iters) { meta->it += iters; }
+// After a sequence of launches: `steps` steps taken, the buffers flipped
+// once per launch (one thread).
+__global__ void k_advance(ErosionMeta *meta, int64_t steps, int launches)
+{
+    meta->it += steps;
+    meta->buf ^= launches & 1;
+}
 
 // ---- load / store --------------------------------------------------------
 
-__global__ __launch_bounds__(256) void k_load_max(ErosionMeta *__restrict__ meta,
-                                                  const double *__restrict__ h,
-                                                  const double *__restrict__ w,
-                                                  const double *__restrict__ s, int64_t V)
-{
-    double m = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < V;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        m = fmax(m, fabs(h[i]));
-        if (w) m = fmax(m, fabs(w[i]));
-        if (s) m = fmax(m, fabs(s[i]));
-    }
-    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
-    __shared__ double red[4];
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int k = 1; k < (int)(blockDim.x >> 6); ++k) m = fmax(m, red[k]);
-        __hip_atomic_fetch_max(&meta->load_max, (unsigned long long)__double_as_longlong(m),
-                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-}
-
 __global__ __launch_bounds__(256) void k_load(ErosionMeta *__restrict__ meta,
-                                              float4 *__restrict__ hw, float2 *__restrict__ sd,
+                                              double2 *__restrict__ hw, double *__restrict__ sd,
                                               const double *__restrict__ h,
                                               const double *__restrict__ w,
                                               const double *__restrict__ s, int64_t V)
 {
-    const double mx = __longlong_as_double((long long)meta->load_max);
-    const int E = (mx > 0.0 && isfinite(mx)) ? ilogb(mx) : 0;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        meta->E[2] = E;  // exponent "after step -1"; max input now in [1, 2)
-        meta->big[2] = 0u;
-        meta->big[0] = 0u;
         meta->it = 0;
+        meta->buf = 0;
     }
     if (i < V) {
-        const ff hh = ff_from_double(ldexp(h[i], -E));
-        const ff ww = w ? ff_from_double(ldexp(w[i], -E)) : ff{0.0f, 0.0f};
-        const ff ss = s ? ff_from_double(ldexp(s[i], -E)) : ff{0.0f, 0.0f};
-        hw[i] = make_float4(hh.hi, hh.lo, ww.hi, ww.lo);
-        sd[i] = make_float2(ss.hi, ss.lo);
-    }
-}
-
-__global__ void k_meta_clear(ErosionMeta *meta)
-{
-    meta->it = 0;
-    meta->load_max = 0;
-    for (int k = 0; k < 3; ++k) {
-        meta->E[k] = 0;
-        meta->big[k] = 0;
+        hw[i] = make_double2(h[i], w ? w[i] : 0.0);
+        sd[i] = s ? s[i] : 0.0;
     }
 }
 
 __global__ __launch_bounds__(256) void k_store(const ErosionMeta *__restrict__ meta,
-                                               const float4 *__restrict__ hw0,
-                                               const float4 *__restrict__ hw1,
-                                               const float2 *__restrict__ s0,
-                                               const float2 *__restrict__ s1, int64_t V,
+                                               const double2 *__restrict__ hw0,
+                                               const double2 *__restrict__ hw1,
+                                               const double *__restrict__ s0,
+                                               const double *__restrict__ s1, int64_t V,
                                                double *__restrict__ h, double *__restrict__ w,
                                                double *__restrict__ s)
 {
-    const int64_t it = meta->it;
-    const int E = meta->E[(int)((it + 2) % 3)];
-    const float4 *hw = (it & 1) ? hw1 : hw0;
-    const float2 *sd = (it & 1) ? s1 : s0;
+    const double2 *hw = meta->buf ? hw1 : hw0;
+    const double *sd = meta->buf ? s1 : s0;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < V) {
-        const float4 v = hw[i];
-        if (h) h[i] = ldexp((double)v.x + (double)v.y, E);
-        if (w) w[i] = ldexp((double)v.z + (double)v.w, E);
-        if (s) {
-            const float2 q = sd[i];
-            s[i] = ldexp((double)q.x + (double)q.y, E);
-        }
+        const double2 v = hw[i];
+        if (h) h[i] = v.x;
+        if (w) w[i] = v.y;
+        if (s) s[i] = sd[i];
     }
 }
 
 // FP64 sums for the mass check, one partial per block then atomics.
 __global__ __launch_bounds__(256) void k_mass(const ErosionMeta *__restrict__ meta,
-                                              const float4 *__restrict__ hw0,
-                                              const float4 *__restrict__ hw1,
-                                              const float2 *__restrict__ s0,
-                                              const float2 *__restrict__ s1, int64_t V,
+                                              const double2 *__restrict__ hw0,
+                                              const double2 *__restrict__ hw1,
+                                              const double *__restrict__ s0,
+                                              const double *__restrict__ s1, int64_t V,
                                               double *__restrict__ out)
 {
-    const int64_t it = meta->it;
-    const int E = meta->E[(int)((it + 2) % 3)];
-    const float4 *hw = (it & 1) ? hw1 : hw0;
-    const float2 *sd = (it & 1) ? s1 : s0;
+    const double2 *hw = meta->buf ? hw1 : hw0;
+    const double *sd = meta->buf ? s1 : s0;
     double a = 0.0, b = 0.0, c = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < V;
          i += (int64_t)gridDim.x * blockDim.x) {
-        const float4 v = hw[i];
-        const float2 q = sd[i];
-        const double hh = ldexp((double)v.x + (double)v.y, E);
-        const double ss = ldexp((double)q.x + (double)q.y, E);
-        a += hh + ss;
-        b += fabs(hh) + fabs(ss);
-        c += ldexp((double)v.z + (double)v.w, E);
+        const double2 v = hw[i];
+        const double ss = sd[i];
+        a += v.x + ss;
+        b += fabs(v.x) + fabs(ss);
+        c += v.y;
     }
     for (int off = 32; off > 0; off >>= 1) {
         a += __shfl_xor(a, off, 64);
@@ -268,41 +305,65 @@ int erosion_load(nx_ctx *ctx, void *state, int64_t V, const double *h, const dou
     NX_HIP(hipSetDevice(ctx->device));
     ErosionBufs b = bufs(state, V);
     const unsigned grid = (unsigned)((V + 255) / 256);
-    hipLaunchKernelGGL(k_meta_clear, dim3(1), dim3(1), 0, stream, b.meta);
-    hipLaunchKernelGGL(k_load_max, dim3(std::min<unsigned>(grid, 2048u)), dim3(256), 0, stream,
-                       b.meta, h, w, s, V);
     hipLaunchKernelGGL(k_load, dim3(grid), dim3(256), 0, stream, b.meta, b.hw[0], b.s[0], h, w,
                        s, V);
     NX_HIP(hipGetLastError());
     return NX_OK;
 }
 
-// Launch n consecutive steps (launch k uses step it + k, where `it` is read
-// from the workspace) and advance the workspace's step counter by n.  The
-// last step's rain for a following step is `rain` unless `last`.
-static void launch_steps(const ErosionBufs &b, int64_t V, const int32_t *col, const float *elen,
-                         int n, double rain, bool last, hipStream_t stream)
+static TilePlanDev plan_dev(const nx_tile_plan &q)
+{
+    const char *b = (const char *)q.blob;
+    TilePlanDev P;
+    P.start = (const int32_t *)(b + q.off_start);
+    P.cs = (const int32_t *)(b + q.off_cs);
+    P.load = (const int32_t *)(b + q.off_load);
+    P.rbase = (const int64_t *)(b + q.off_rbase);
+    P.ring = (const int32_t *)(b + q.off_ring);
+    P.cbase = (const int64_t *)(b + q.off_cbase);
+    P.lcol = (const uint16_t *)(b + q.off_lcol);
+    P.lel = (const double *)(b + q.off_lel);
+    P.ntiles = q.ntiles;
+    return P;
+}
+
+// Launch n consecutive steps (the launches read the step counter and the
+// current buffer from the workspace) and advance the workspace by n steps.
+// With a tile plan, steps go two per launch and an odd last step runs alone.
+// The last step's rain for a following step is `rain` unless `last`.
+static void launch_steps(const ErosionBufs &b, int64_t V, const int32_t *col, const double *elen,
+                         const nx_tile_plan *plan, int n, double rain, bool last,
+                         hipStream_t stream)
 {
     const int64_t nblocks = (V + 255) / 256;
     const int64_t grid = ((nblocks + 7) / 8) * 8;
-    for (int k = 0; k < n; ++k) {
+    int k = 0, j = 0;
+    if (plan) {
+        const TilePlanDev P = plan_dev(*plan);
+        const int64_t tgrid = ((P.ntiles + 7) / 8) * 8;
+        for (; k + 2 <= n; k += 2, ++j) {
+            const double rain_next = (last && k + 2 == n) ? 0.0 : rain;
+            hipLaunchKernelGGL(k_erode_pair, dim3((unsigned)tgrid), dim3(kTileThreads), 0, stream,
+                               b.meta, b.hw[0], b.hw[1], b.s[0], b.s[1], P, j, rain, rain_next);
+        }
+    }
+    for (; k < n; ++k, ++j) {
         const double rain_next = (last && k + 1 == n) ? 0.0 : rain;
         hipLaunchKernelGGL(k_erode_step, dim3((unsigned)grid), dim3(256), 0, stream, b.meta,
-                           b.hw[0], b.hw[1], b.s[0], b.s[1], V, col, (const float2 *)elen,
-                           nblocks, (int64_t)k, rain_next);
+                           b.hw[0], b.hw[1], b.s[0], b.s[1], V, col, elen, nblocks, j, rain_next);
     }
-    hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, stream, b.meta, (int64_t)n);
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, stream, b.meta, (int64_t)n, j);
 }
 
-// Captured chunk of steps, cached per (workspace, graph, length, rain, last).
-// Replays stay correct because every launch reads the step counter, buffer
-// parity and exponent state from the workspace rather than from its
+// Captured chunk of steps, cached per (workspace, graph, plan, length, rain,
+// last).  Replays stay correct because every launch reads the step counter,
+// buffer and exponent state from the workspace rather than from its
 // arguments.  No memset is recorded into the graph.
 static int chunk_graph(nx_ctx *ctx, const ErosionBufs &b, void *state, int64_t V,
-                       const int32_t *col, const float *elen, int n, double rain, bool last,
-                       hipGraphExec_t *out)
+                       const int32_t *col, const double *elen, const nx_tile_plan *plan, int n,
+                       double rain, bool last, hipGraphExec_t *out)
 {
-    GraphKey key{state, V, col, elen, n, rain, last, 0};
+    GraphKey key{state, V, col, elen, n, rain, last, plan ? (uint64_t)(uintptr_t)plan->blob : 0};
     std::lock_guard<std::mutex> lk(ctx->mu);
     auto it = ctx->graphs.find(key);
     if (it != ctx->graphs.end()) {
@@ -312,7 +373,7 @@ static int chunk_graph(nx_ctx *ctx, const ErosionBufs &b, void *state, int64_t V
     if (ctx->graphs.size() >= kMaxGraphs) ctx->clear_graphs();
     hipGraph_t g = nullptr;
     NX_HIP(hipStreamBeginCapture(ctx->capture_stream, hipStreamCaptureModeThreadLocal));
-    launch_steps(b, V, col, elen, n, rain, last, ctx->capture_stream);
+    launch_steps(b, V, col, elen, plan, n, rain, last, ctx->capture_stream);
     const hipError_t le = hipGetLastError();
     const hipError_t ce = hipStreamEndCapture(ctx->capture_stream, &g);
     if (le != hipSuccess || ce != hipSuccess) {
@@ -329,12 +390,27 @@ static int chunk_graph(nx_ctx *ctx, const ErosionBufs &b, void *state, int64_t V
     return NX_OK;
 }
 
-int erosion_run(nx_ctx *ctx, void *state, int64_t V, const int32_t *col, const float *elen,
-                int iters, double rain, hipStream_t stream)
+static int check_plan(const nx_tile_plan *plan, int64_t V)
+{
+    if (!plan) return NX_OK;
+    if (!plan->blob || ((uintptr_t)plan->blob & 255) != 0)
+        return set_error(NX_EINVAL, "tile plan needs a 256-byte aligned device copy");
+    if (plan->V != V || plan->ntiles <= 0 || plan->ntiles > INT32_MAX)
+        return set_error(NX_EINVAL, "tile plan is for V=%lld, not %lld", (long long)plan->V,
+                         (long long)V);
+    if (plan->max_tile > kTileMax || plan->max_load > kLoadMax)
+        return set_error(NX_EINVAL, "tile plan tiles exceed %d owned / %d loaded vertices",
+                         kTileMax, kLoadMax);
+    return NX_OK;
+}
+
+int erosion_run(nx_ctx *ctx, void *state, int64_t V, const int32_t *col, const double *elen,
+                const nx_tile_plan *plan, int iters, double rain, hipStream_t stream)
 {
     int rc = check_state(ctx, state, V);
     if (rc) return rc;
     if (!col || !elen) return set_error(NX_EINVAL, "null neighbour table");
+    if ((rc = check_plan(plan, V))) return rc;
     if (iters < 0) return set_error(NX_EINVAL, "iters=%d < 0", iters);
     if (!(rain >= 0.0) || !std::isfinite(rain)) return set_error(NX_EINVAL, "rain must be finite and >= 0");
     if (iters == 0) return NX_OK;
@@ -346,7 +422,7 @@ int erosion_run(nx_ctx *ctx, void *state, int64_t V, const int32_t *col, const f
                            b.hw[0], b.hw[1], V, rain);
     const int chunk = ctx->graph_chunk;
     if (chunk <= 0 || iters < 2) {
-        launch_steps(b, V, col, elen, iters, rain, true, stream);
+        launch_steps(b, V, col, elen, plan, iters, rain, true, stream);
         NX_HIP(hipGetLastError());
         return NX_OK;
     }
@@ -356,7 +432,7 @@ int erosion_run(nx_ctx *ctx, void *state, int64_t V, const int32_t *col, const f
         const int n = std::min(chunk, iters - done);
         const bool last = done + n == iters;
         hipGraphExec_t ex = nullptr;
-        rc = chunk_graph(ctx, b, state, V, col, elen, n, rain, last, &ex);
+        rc = chunk_graph(ctx, b, state, V, col, elen, plan, n, rain, last, &ex);
         if (rc) return rc;
         NX_HIP(hipGraphLaunch(ex, stream));
         done += n;

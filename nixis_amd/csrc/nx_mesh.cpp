@@ -284,17 +284,35 @@ int sfc_order(const double *verts, int64_t V, int64_t *order)
         key[i] = spread(q[0]) | spread(q[1]) << 1 | spread(q[2]) << 2;
       }
     });
+    // Stable LSD radix sort of (key, index) on the 63 key bits, 9 passes of
+    // 7 bits; ties keep the input order.
+    std::vector<int64_t> tmp((size_t)V);
+    std::vector<uint64_t> ktmp((size_t)V);
     std::iota(order, order + V, (int64_t)0);
-    std::stable_sort(order, order + V, [&](int64_t a, int64_t b) { return key[a] < key[b]; });
+    int64_t *src = order, *dst = tmp.data();
+    uint64_t *ksrc = key.data(), *kdst = ktmp.data();
+    for (int shift = 0; shift < 63; shift += 7) {
+        int64_t cnt[129] = {0};
+        for (int64_t i = 0; i < V; ++i) ++cnt[((ksrc[i] >> shift) & 127) + 1];
+        for (int b = 0; b < 128; ++b) cnt[b + 1] += cnt[b];
+        for (int64_t i = 0; i < V; ++i) {
+            const int64_t pos = cnt[(ksrc[i] >> shift) & 127]++;
+            dst[pos] = src[i];
+            kdst[pos] = ksrc[i];
+        }
+        std::swap(src, dst);
+        std::swap(ksrc, kdst);
+    }
+    if (src != order) std::copy(src, src + V, order);
     return NX_OK;
 }
 
 // Device neighbour layout.  order (nullable) maps new index -> old index;
-// col is slot-major [6][V] in the new numbering and elen is [6][V][2]: each
-// edge length as an FP32 (hi, lo) pair with hi + lo equal to the FP64 length
-// of the reference's calc_distance (erosion.py:38-40) to about 48 bits.
+// col is slot-major [6][V] in the new numbering and elen is [6][V]: each
+// edge length computed as the reference's calc_distance (erosion.py:38-40)
+// does, in FP64 (this file is built without FP contraction).
 int ell_from_adjacency(const int32_t *adj, int64_t V, const double *verts,
-                       const int64_t *order, int32_t *col, float *elen)
+                       const int64_t *order, int32_t *col, double *elen)
 {
     if (!adj || !verts || !col || !elen || V <= 0 || V > INT32_MAX) return NX_EINVAL;
     std::vector<int32_t> inv;
@@ -312,22 +330,19 @@ int ell_from_adjacency(const int32_t *adj, int64_t V, const double *verts,
             const double *p = verts + 3 * old;
             for (int k = 0; k < 6; ++k) {
                 const int32_t n = adj[old * 6 + k];
-                float *el = elen + 2 * ((int64_t)k * V + i);
+                double *el = elen + (int64_t)k * V + i;
                 if (n < -1 || n >= V) {
                     bad.store(1);
                     continue;
                 }
                 if (n == -1) {
                     col[(int64_t)k * V + i] = (int32_t)i;
-                    el[0] = el[1] = 0.0f;
+                    el[0] = 0.0;
                 } else {
                     const double *q = verts + 3 * (int64_t)n;
                     const double a = p[0] - q[0], b = p[1] - q[1], c = p[2] - q[2];
-                    const double dd = std::sqrt(a * a + b * b + c * c);
-                    const float hi = (float)dd;
                     col[(int64_t)k * V + i] = order ? inv[n] : n;
-                    el[0] = hi;
-                    el[1] = (float)(dd - (double)hi);
+                    el[0] = std::sqrt(a * a + b * b + c * c);
                 }
             }
         }

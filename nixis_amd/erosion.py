@@ -2,10 +2,11 @@
 
 ``erode_terrain3`` (erosion.py:171-192) and ``erosion_iteration3``
 (erosion.py:197-279) keep their signatures and in-place semantics; the
-Jacobi steps run on the GPU with float-float FP32 fields (see
-nixis_amd/csrc/nx_erosion.hip).  ``ErosionGraph`` / ``ErosionState`` are the
+Jacobi steps run on the GPU in FP64, operation for operation as the
+reference computes them (see nixis_amd/csrc/nx_erosion.hip).  ``ErosionGraph`` / ``ErosionState`` are the
 device-resident forms used by the batched and served paths.
 """
+import ctypes
 import math
 
 import numpy as np
@@ -26,15 +27,20 @@ def calc_distance(v0, v1):
 
 
 class ErosionGraph:
-    """Device neighbour table: col int32 [6,V] and float-float elen [6,V,2].
+    """Device neighbour table: col int32 [6,V] and FP64 elen [6,V],
+    plus the tile plan that lets erosion run two steps per launch.
 
     Built on the host from a reference-style neighbour table ([V,6] int32,
-    -1 padded) and FP64 vertex coordinates; `order` optionally renumbers the
-    vertices (space-filling-curve order), in which case fields passed to
-    ErosionState must use the new numbering (see ``permute``/``unpermute``).
+    -1 padded) and FP64 vertex coordinates.  `order` renumbers the vertices
+    on the device: "sfc" (default, 3D Morton order: compact tiles), None
+    (caller order) or an explicit order[new] = old.  Fields passed to
+    ErosionState use the device numbering (``permute``/``unpermute``).
+    With `tiled`, the tile plan is built when the graph cuts into tiles
+    (nx_tile_plan_build); otherwise, or for graphs without locality, every
+    step is its own launch.
     """
 
-    def __init__(self, verts, neighbors, order=None, device=None):
+    def __init__(self, verts, neighbors, order="sfc", device=None, tiled=True):
         import torch
         verts = np.ascontiguousarray(verts, dtype=np.float64)
         nb = np.ascontiguousarray(neighbors, dtype=np.int32)
@@ -42,8 +48,13 @@ class ErosionGraph:
                 or nb.shape[0] != verts.shape[0]:
             raise ValueError("verts must be [V,3] float64 and neighbors [V,6] int32")
         V = nb.shape[0]
+        if isinstance(order, str):
+            if order != "sfc":
+                raise ValueError(f"unknown order {order!r}")
+            from . import util
+            order = util.sfc_order(verts)
         col = np.empty((6, V), dtype=np.int32)
-        elen = np.empty((6, V, 2), dtype=np.float32)
+        elen = np.empty((6, V), dtype=np.float64)
         o = None if order is None else np.ascontiguousarray(order, dtype=np.int64)
         _lib.check(_lib.lib().nx_ell_from_adjacency(_lib.ptr(nb), V, _lib.ptr(verts),
                                                     _lib.ptr(o), _lib.ptr(col), _lib.ptr(elen)))
@@ -54,17 +65,60 @@ class ErosionGraph:
         self.col = torch.from_numpy(col).to(device)
         self.elen = torch.from_numpy(elen).to(device)
         self.device = self.col.device
+        self.plan = None
+        self._plan_blob = None
+        if tiled:
+            self._build_plan(col, elen)
+
+    def _build_plan(self, col, elen):
+        import torch
+        L = _lib.lib()
+        plan = _lib.NxTilePlan()
+        rc = L.nx_tile_plan_build(_lib.ptr(col), _lib.ptr(elen), self.V, None, 0,
+                                  ctypes.byref(plan))
+        if rc == _lib.NX_ERANGE:
+            return
+        _lib.check(rc)
+        img = np.zeros(plan.bytes, dtype=np.uint8)
+        _lib.check(L.nx_tile_plan_build(_lib.ptr(col), _lib.ptr(elen), self.V, _lib.ptr(img),
+                                        plan.bytes, ctypes.byref(plan)))
+        blob = torch.from_numpy(img).to(self.device)
+        if blob.data_ptr() % 256:
+            raise RuntimeError("tile plan copy is not 256-byte aligned")
+        plan.blob = blob.data_ptr()
+        self.plan = plan
+        self._plan_blob = blob
 
     @property
     def nbytes(self):
-        return self.col.numel() * 4 + self.elen.numel() * 4
+        n = self.col.numel() * 4 + self.elen.numel() * 4
+        return n + (self._plan_blob.numel() if self._plan_blob is not None else 0)
+
+    def _order_dev(self):
+        import torch
+        if getattr(self, "_odev", None) is None:
+            self._odev = torch.from_numpy(self.order).to(self.device)
+        return self._odev
+
+    def permute_device(self, t):
+        """Caller-numbered device tensor [V] -> device numbering (a copy)."""
+        return t.clone() if self.order is None else t[self._order_dev()]
+
+    def unpermute_device(self, t):
+        """Device-numbered device tensor [V] -> caller numbering (a copy)."""
+        import torch
+        if self.order is None:
+            return t.clone()
+        out = torch.empty_like(t)
+        out[self._order_dev()] = t
+        return out
 
     def permute(self, a):
-        """Old-numbered per-vertex array -> new numbering."""
+        """Caller-numbered per-vertex array -> device numbering."""
         return a if self.order is None else a[self.order]
 
     def unpermute(self, a):
-        """New-numbered per-vertex array -> old numbering."""
+        """Device-numbered per-vertex array -> caller numbering."""
         if self.order is None:
             return a
         out = np.empty_like(a)
@@ -106,9 +160,11 @@ class ErosionState:
     def run(self, graph, iters, rain=RAIN_DEFAULT, stream=None):
         if graph.V != self.V:
             raise ValueError("graph and state sizes differ")
-        _lib.check(_lib.lib().nx_erosion_run(self.ctx.handle, _lib.ptr(self.buf), self.V,
-                                             _lib.ptr(graph.col), _lib.ptr(graph.elen),
-                                             int(iters), float(rain), _lib.stream_ptr(stream)))
+        plan = ctypes.byref(graph.plan) if graph.plan is not None else None
+        _lib.check(_lib.lib().nx_erosion_run_tiled(self.ctx.handle, _lib.ptr(self.buf), self.V,
+                                                   _lib.ptr(graph.col), _lib.ptr(graph.elen),
+                                                   plan, int(iters), float(rain),
+                                                   _lib.stream_ptr(stream)))
 
     def store(self, h=None, w=None, s=None, stream=None):
         _lib.check(_lib.lib().nx_erosion_store(self.ctx.handle, _lib.ptr(self.buf), self.V,

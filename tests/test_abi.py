@@ -71,11 +71,11 @@ def test_mesh_entry_points_match_oracle(oracle, d):
     assert (adj == ref).all()
 
 
-def test_ell_layout_host_entry_point():
+def test_ell_layout_host_entry_point(oracle):
     pts, nb = util.icosphere_with_neighbors(3)
     V = pts.shape[0]
     col = np.empty((6, V), np.int32)
-    elen = np.empty((6, V, 2), np.float32)
+    elen = np.empty((6, V), np.float64)
     order = util.sfc_order(pts)
     assert sorted(order.tolist()) == list(range(V))
     L = _lib.lib()
@@ -88,8 +88,7 @@ def test_ell_layout_host_entry_point():
                 assert col[k, i] == i
                 continue
             assert col[k, i] == n
-            d = np.linalg.norm(pts[i] - pts[n])
-            assert abs((float(elen[k, i, 0]) + float(elen[k, i, 1])) - d) <= 1e-14 * d
+            assert elen[k, i] == oracle.calc_distance(pts[i], pts[n])   # bit for bit
 
 
 def test_bad_arguments_are_rejected_without_a_gpu():

@@ -1,0 +1,65 @@
+"""Why the erosion fields are float-float: a CPU emulation on a small planet.
+
+erosion.py:197-279 doubles the water (and with it the heights' spread) about
+every step, and the flow direction at each edge is decided by h[n] > h[i].
+Emulating the step with every operation rounded to K significant bits shows
+that K=24 (FP32) flips those comparisons and ends far from the FP64 result,
+while K=48 (the float-float pairs the kernels use) stays at the 1e-12 level.
+This is the evidence behind the kernels' field format; the kernels
+themselves are checked against the oracle in tests/test_gpu_erosion.py.
+"""
+import numpy as np
+import pytest
+
+EVAP, SOL, CAP = 0.1 / 320, 0.01 / 320, 0.2 / 320
+
+
+def _round(x, k):
+    m, e = np.frexp(x)
+    return np.ldexp(np.round(np.ldexp(m, k)), e - k)
+
+
+def _planet(d):
+    from nixis_amd import util
+    pts, nb = util.icosphere_with_neighbors(d)
+    V = pts.shape[0]
+    n = np.where(nb < 0, np.arange(V)[:, None], nb)          # pad -> self (a tie)
+    dist = np.linalg.norm(pts[:, None, :] - pts[n], axis=2)
+    rng = np.random.default_rng(5)
+    h = rng.uniform(-9000, 9000, V)
+    return n, dist, h
+
+
+def _run(n, dist, h, iters, rain, k=None):
+    r = (lambda x: x) if k is None else (lambda x: _round(x, k))
+    h = r(h.copy())
+    w = np.zeros_like(h)
+    s = np.zeros_like(h)
+    for _ in range(iters):
+        w = r(w + rain)
+        sg = np.sign(h[n] - h[:, None])                   # +1 higher, -1 lower, 0 tie
+        S, W = s.copy(), w.copy()
+        for j in range(6):
+            S = r(S + sg[:, j] * r(SOL * w[n[:, j]]))
+            W = r(W + sg[:, j] * r(w[n[:, j]] * dist[:, j]))
+        hn, sn = r(h - S), r(s + S)
+        wn = r(w + r(W - r(W * EVAP)))
+        cw = r(CAP * wn)
+        over = sn > cw
+        ex = r(sn - cw)
+        hn = np.where(over, r(hn + ex), hn)
+        sn = np.where(over, r(sn - ex), sn)
+        h, w, s = hn, wn, sn
+    return h
+
+
+@pytest.mark.parametrize("k,ok", [(24, False), (48, True)])
+def test_mantissa_width_decides_parity(k, ok):
+    n, dist, h0 = _planet(16)
+    ref = _run(n, dist, h0, 250, 1.0)
+    got = _run(n, dist, h0, 250, 1.0, k)
+    err = np.abs(got - ref).max() / (ref.max() - ref.min())
+    if ok:
+        assert err < 1e-9, err
+    else:
+        assert err > 1e-4, err

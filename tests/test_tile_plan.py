@@ -1,0 +1,145 @@
+"""The tile plan for two erosion steps per launch (nx_tile_plan_build, host).
+
+Checked on the CPU: tiles cover the vertices in order, each tile's rings are
+exactly its outside neighbours and their outside neighbours, every local
+neighbour index names the same vertex as the global table, edge lengths are
+carried over, and two Jacobi steps computed from the plan's local view (tile
++ ring 1 first, then the tile) equal two steps of the FP64 oracle bit for
+bit.  A numbering without locality is refused.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from nixis_amd import _lib, util
+
+EVAP, SOL, CAP = 0.1 / 320, 0.01 / 320, 0.2 / 320
+
+
+def _ell(d, order="sfc"):
+    pts, nb = util.icosphere_with_neighbors(d)
+    V = pts.shape[0]
+    o = util.sfc_order(pts) if isinstance(order, str) else order
+    col = np.empty((6, V), np.int32)
+    elen = np.empty((6, V), np.float64)
+    _lib.check(_lib.lib().nx_ell_from_adjacency(_lib.ptr(nb), V, _lib.ptr(pts), _lib.ptr(o),
+                                                _lib.ptr(col), _lib.ptr(elen)))
+    return pts, nb, o, col, elen
+
+
+def _plan(col, elen):
+    V = col.shape[1]
+    L = _lib.lib()
+    q = _lib.NxTilePlan()
+    rc = L.nx_tile_plan_build(_lib.ptr(col), _lib.ptr(elen), V, None, 0, ctypes.byref(q))
+    _lib.check(rc)
+    img = np.zeros(q.bytes, np.uint8)
+    _lib.check(L.nx_tile_plan_build(_lib.ptr(col), _lib.ptr(elen), V, _lib.ptr(img), q.bytes,
+                                    ctypes.byref(q)))
+    nt = q.ntiles
+
+    def arr(off, dt, n):
+        return np.frombuffer(img, dtype=dt, count=n, offset=off)
+
+    start = arr(q.off_start, np.int32, nt + 1)
+    cs = arr(q.off_cs, np.int32, nt)
+    load = arr(q.off_load, np.int32, nt)
+    rbase = arr(q.off_rbase, np.int64, nt + 1)
+    ring = arr(q.off_ring, np.int32, int(rbase[-1]))
+    cbase = arr(q.off_cbase, np.int64, nt + 1)
+    lcol = arr(q.off_lcol, np.uint16, 6 * int(cbase[-1]))
+    lel = arr(q.off_lel, np.float64, 6 * int(cbase[-1]))
+    return q, start, cs, load, rbase, ring, cbase, lcol, lel
+
+
+def _tiles(col, elen):
+    q, start, cs, load, rbase, ring, cbase, lcol, lel = _plan(col, elen)
+    for t in range(q.ntiles):
+        a, b = int(start[t]), int(start[t + 1])
+        T = b - a
+        rg = ring[rbase[t]:rbase[t + 1]]
+        glob = np.concatenate([np.arange(a, b), rg])          # local -> global
+        c = int(cs[t])
+        lc = lcol[6 * cbase[t]:6 * cbase[t + 1]].reshape(6, c)
+        le = lel[6 * cbase[t]:6 * cbase[t + 1]].reshape(6, c)
+        yield q, a, b, T, c, int(load[t]), glob, lc, le
+
+
+@pytest.mark.parametrize("d", [2, 8, 40])
+def test_plan_covers_and_rings_are_exact(d):
+    pts, nb, o, col, elen = _ell(d)
+    V = col.shape[1]
+    nxt = 0
+    for q, a, b, T, c, L, glob, lc, le in _tiles(col, elen):
+        assert a == nxt and 0 < T <= 1024 and L <= 1536 and q.max_tile <= 1024
+        nxt = b
+        inside = set(range(a, b))
+        r1 = sorted(set(col[:, a:b].ravel()) - inside)
+        r2 = sorted(set(col[:, r1].ravel()) - inside - set(r1)) if r1 else []
+        assert list(glob[T:c]) == r1 and list(glob[c:L]) == r2
+        # every local neighbour of tile + ring 1 names the global neighbour
+        assert (glob[lc] == col[:, glob[:c]]).all()
+        assert (le == elen[:, glob[:c]]).all()
+    assert nxt == V
+
+
+def _two_steps_local(h, w, s, glob, c, T, lc, d):
+    """Two steps of the tile from its local view: step 1 on tile + ring 1
+    from the staged fields of tile + both rings, step 2 on the tile, with
+    the plan's local neighbour indices and edge lengths."""
+    H, Wt, Sd = h[glob].copy(), w[glob].copy(), s[glob].copy()
+
+    def step(n_rows, Hs, Ws, Ss):
+        Hn, Wn, Sn = Hs[:n_rows].copy(), Ws[:n_rows].copy(), Ss[:n_rows].copy()
+        for i in range(n_rows):
+            S, W = Ss[i], Ws[i]
+            for k in range(6):
+                n = lc[k, i]
+                if Hs[n] > Hs[i]:
+                    S += SOL * Ws[n]
+                    W += Ws[n] * d[k, i]
+                elif Hs[n] < Hs[i]:
+                    S -= SOL * Ws[n]
+                    W -= Ws[n] * d[k, i]
+            Hn[i] = Hs[i] - S
+            Sn[i] = Ss[i] + S
+            Wn[i] = Ws[i] + (W - W * EVAP)
+            if Sn[i] > CAP * Wn[i]:
+                Hn[i] += Sn[i] - CAP * Wn[i]
+                Sn[i] -= Sn[i] - CAP * Wn[i]
+        return Hn, Wn, Sn
+
+    H1, W1, S1 = step(c, H, Wt, Sd)
+    H2, W2, S2 = step(T, H1, W1, S1)
+    return H2, W2, S2
+
+
+def test_two_steps_from_the_plan_equal_the_oracle(oracle):
+    d = 6
+    pts, nb, o, col, elen = _ell(d)
+    V = col.shape[1]
+    P = pts[o]
+    # oracle neighbour table in device order, -1 pads where col points at self
+    nbd = col.T.copy()
+    nbd[nbd == np.arange(V)[:, None]] = -1
+    rng = np.random.default_rng(2)
+    h = rng.uniform(-9000, 9000, V)
+    w = rng.uniform(0, 1, V)
+    s = rng.uniform(0, 0.01, V)
+    hr, wr, sr = h.copy(), w.copy(), s.copy()
+    for _ in range(2):
+        oracle.erosion_iteration3(P, nbd, hr, wr, sr)
+    for q, a, b, T, c, L, glob, lc, le in _tiles(col, elen):
+        H2, W2, S2 = _two_steps_local(h, w, s, glob, c, T, lc, le)
+        assert (H2 == hr[a:b]).all() and (W2 == wr[a:b]).all() and (S2 == sr[a:b]).all()
+
+
+def test_numbering_without_locality_is_refused():
+    pts, nb, _, _, _ = _ell(40, order=None)
+    V = pts.shape[0]
+    o = np.random.default_rng(0).permutation(V).astype(np.int64)
+    _, _, _, col, elen = _ell(40, order=o)
+    q = _lib.NxTilePlan()
+    rc = _lib.lib().nx_tile_plan_build(_lib.ptr(col), _lib.ptr(elen), V, None, 0, ctypes.byref(q))
+    assert rc == _lib.NX_ERANGE
