@@ -56,6 +56,8 @@ int nx_ctx_destroy(nx_ctx *ctx);
 /* Context options (nx_ctx_set_option). */
 #define NX_OPT_GRAPH_CHUNK 1 /* erosion steps per captured hipGraph; 0 = eager
                                 launches (default 32)                        */
+#define NX_OPT_PAIR_KERNEL 2 /* two-step erosion kernel variant, 0..6 (0 =
+                                default); a tuning knob, results identical   */
 int nx_ctx_set_option(nx_ctx *ctx, int option, int64_t value);
 /* Synchronise `stream`, then return and clear the context's sticky kernel
  * error (NX_ERANGE: a noise coordinate outside |x| < 2^30 after the octave
@@ -147,7 +149,7 @@ typedef struct nx_tile_plan {
     int64_t V;
     int64_t ntiles;
     int32_t max_tile;    /* largest tile (owned vertices), <= 1024 */
-    int32_t max_load;    /* largest tile + ring 1 + ring 2, <= 1536 */
+    int32_t max_load;    /* largest tile + ring 1 + ring 2, <= 2048 */
     int64_t off_start, off_cs, off_load, off_rbase, off_ring, off_cbase,
             off_lcol, off_lel;   /* byte offsets of the arrays in the image */
 } nx_tile_plan;

@@ -20,6 +20,7 @@ NX_EDIV0 = -5
 NX_EBUSY = -6
 NX_MAX_OCTAVES = 32
 NX_OPT_GRAPH_CHUNK = 1
+NX_OPT_PAIR_KERNEL = 2
 
 
 class NixisError(RuntimeError):
@@ -173,6 +174,10 @@ class Context:
     def set_graph_chunk(self, steps):
         """Erosion steps per captured hipGraph (0: eager launches)."""
         check(lib().nx_ctx_set_option(self.handle, NX_OPT_GRAPH_CHUNK, int(steps)))
+
+    def set_pair_kernel(self, variant):
+        """Two-step erosion kernel variant 0..6 (0: default); same results."""
+        check(lib().nx_ctx_set_option(self.handle, NX_OPT_PAIR_KERNEL, int(variant)))
 
 
 def stream_ptr(stream=None):

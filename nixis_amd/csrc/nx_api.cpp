@@ -114,6 +114,11 @@ int nx_ctx_set_option(nx_ctx *ctx, int option, int64_t value)
         ctx->clear_graphs();
         ctx->graph_chunk = (int)value;
         return NX_OK;
+    case NX_OPT_PAIR_KERNEL:
+        if (value < 0 || value > 6) return set_error(NX_EINVAL, "pair kernel variant %lld outside 0..6", (long long)value);
+        ctx->clear_graphs();
+        ctx->pair_variant = (int)value;
+        return NX_OK;
     default:
         return set_error(NX_EINVAL, "unknown option %d", option);
     }

@@ -42,6 +42,7 @@ struct nx_ctx {
     std::atomic<unsigned> key_slot{0};
     hipStream_t capture_stream = nullptr;  // default priority, graph capture only
     int graph_chunk = 32;                  // erosion steps per captured graph; 0 = eager
+    int pair_variant = 0;                  // two-step kernel variant (0 = default)
     std::map<GraphKey, hipGraphExec_t> graphs;
     std::mutex mu;
 

@@ -80,10 +80,7 @@ __global__ __launch_bounds__(256) void k_erode_step(const ErosionMeta *__restric
 // buffer.  Ring-1 vertices are computed by every tile that touches them,
 // with the same inputs and arithmetic, so the values agree.
 constexpr int kTileMax = 1024;   // owned vertices per tile
-constexpr int kLoadMax = 1536;   // tile + both rings
-constexpr int kTileThreads = 512;
-constexpr int kLoadPer = kLoadMax / kTileThreads;
-constexpr int kOwnPer = kTileMax / kTileThreads;
+constexpr int kLoadMax = 2048;   // tile + both rings
 
 struct TilePlanDev {
     const int32_t *start;    // [ntiles + 1]
@@ -97,14 +94,20 @@ struct TilePlanDev {
     int64_t ntiles;
 };
 
-__global__ __launch_bounds__(kTileThreads) void k_erode_pair(const ErosionMeta *__restrict__ meta,
-                                                            double2 *__restrict__ hw0,
-                                                            double2 *__restrict__ hw1,
-                                                            double *__restrict__ s0,
-                                                            double *__restrict__ s1,
-                                                            TilePlanDev P, int j, double rain_mid,
-                                                            double rain_next)
+// THREADS per block; KEEP holds the tile's neighbour indices and edge
+// lengths in registers between the two steps (otherwise the second step
+// reads them again).
+template <int THREADS, bool KEEP>
+__global__ __launch_bounds__(THREADS) void k_erode_pair(const ErosionMeta *__restrict__ meta,
+                                                       double2 *__restrict__ hw0,
+                                                       double2 *__restrict__ hw1,
+                                                       double *__restrict__ s0,
+                                                       double *__restrict__ s1, TilePlanDev P,
+                                                       int j, double rain_mid, double rain_next)
 {
+    constexpr int kLoadPer = (kLoadMax + THREADS - 1) / THREADS;
+    constexpr int kOwnPer = (kTileMax + THREADS - 1) / THREADS;
+    constexpr int kKeep = KEEP ? kOwnPer : 1;
     __shared__ double2 A[kLoadMax];
     const int64_t tile = xcd_tile(blockIdx.x, P.ntiles);
     if (tile >= P.ntiles) return;
@@ -121,22 +124,22 @@ __global__ __launch_bounds__(kTileThreads) void k_erode_pair(const ErosionMeta *
     const uint16_t *lcol = P.lcol + 6 * cb;
     const double *lel = P.lel + 6 * cb;
 
-    for (int li = threadIdx.x; li < L; li += kTileThreads)
+    for (int li = threadIdx.x; li < L; li += THREADS)
         A[li] = hw_src[li < T ? a + li : ring[li - T]];
     double sed[kLoadPer];
 #pragma unroll
     for (int m = 0; m < kLoadPer; ++m) {
-        const int li = threadIdx.x + m * kTileThreads;
+        const int li = threadIdx.x + m * THREADS;
         sed[m] = li < cs ? s_src[li < T ? a + li : ring[li - T]] : 0.0;
     }
     __syncthreads();
 
     double2 res[kLoadPer];
-    int nbr[kOwnPer][6];
-    double len[kOwnPer][6];
+    int nbr[kKeep][6];
+    double len[kKeep][6];
 #pragma unroll
     for (int m = 0; m < kLoadPer; ++m) {
-        const int li = threadIdx.x + m * kTileThreads;
+        const int li = threadIdx.x + m * THREADS;
         res[m] = make_double2(0.0, 0.0);
         if (li < cs) {
             const double2 me = A[li];
@@ -145,9 +148,9 @@ __global__ __launch_bounds__(kTileThreads) void k_erode_pair(const ErosionMeta *
             for (int q = 0; q < 6; ++q) {
                 const int n = lcol[q * cs + li];
                 const double d = lel[q * cs + li];
-                if (m < kOwnPer) {
-                    nbr[m < kOwnPer ? m : 0][q] = n;
-                    len[m < kOwnPer ? m : 0][q] = d;
+                if (KEEP && m < kOwnPer) {
+                    nbr[m < kKeep ? m : 0][q] = n;
+                    len[m < kKeep ? m : 0][q] = d;
                 }
                 const double2 nb = A[n];
                 erode_accumulate(me.x, nb.x, nb.y, d, S, W);
@@ -161,21 +164,23 @@ __global__ __launch_bounds__(kTileThreads) void k_erode_pair(const ErosionMeta *
     __syncthreads();
 #pragma unroll
     for (int m = 0; m < kLoadPer; ++m) {
-        const int li = threadIdx.x + m * kTileThreads;
+        const int li = threadIdx.x + m * THREADS;
         if (li < cs) A[li] = res[m];
     }
     __syncthreads();
 
 #pragma unroll
     for (int m = 0; m < kOwnPer; ++m) {
-        const int li = threadIdx.x + m * kTileThreads;
+        const int li = threadIdx.x + m * THREADS;
         if (li < T) {
             const double2 me = A[li];
             double S = sed[m], W = me.y;
 #pragma unroll
             for (int q = 0; q < 6; ++q) {
-                const double2 nb = A[nbr[m][q]];
-                erode_accumulate(me.x, nb.x, nb.y, len[m][q], S, W);
+                const int n = KEEP ? nbr[m < kKeep ? m : 0][q] : lcol[q * cs + li];
+                const double d = KEEP ? len[m < kKeep ? m : 0][q] : lel[q * cs + li];
+                const double2 nb = A[n];
+                erode_accumulate(me.x, nb.x, nb.y, d, S, W);
             }
             double ho, wo, so;
             erode_finish(me.x, me.y, sed[m], S, W, rain_next, ho, wo, so);
@@ -184,6 +189,20 @@ __global__ __launch_bounds__(kTileThreads) void k_erode_pair(const ErosionMeta *
         }
     }
 }
+
+// Kernel variants (NX_OPT_PAIR_KERNEL): block size and register use.
+struct PairVariant {
+    void (*fn)(const ErosionMeta *, double2 *, double2 *, double *, double *, TilePlanDev, int,
+               double, double);
+    int threads;
+};
+static const PairVariant kPairVariants[] = {
+    {k_erode_pair<1024, true>, 1024}, {k_erode_pair<512, true>, 512},
+    {k_erode_pair<512, false>, 512},  {k_erode_pair<1024, true>, 1024},
+    {k_erode_pair<1024, false>, 1024}, {k_erode_pair<256, true>, 256},
+    {k_erode_pair<256, false>, 256},
+};
+
 
 // erosion.py:184-186: water += rain on the current fields (before a step).
 __global__ __launch_bounds__(256) void k_add_rain(const ErosionMeta *__restrict__ meta,
@@ -332,7 +351,7 @@ static TilePlanDev plan_dev(const nx_tile_plan &q)
 // With a tile plan, steps go two per launch and an odd last step runs alone.
 // The last step's rain for a following step is `rain` unless `last`.
 static void launch_steps(const ErosionBufs &b, int64_t V, const int32_t *col, const double *elen,
-                         const nx_tile_plan *plan, int n, double rain, bool last,
+                         const nx_tile_plan *plan, int variant, int n, double rain, bool last,
                          hipStream_t stream)
 {
     const int64_t nblocks = (V + 255) / 256;
@@ -343,8 +362,9 @@ static void launch_steps(const ErosionBufs &b, int64_t V, const int32_t *col, co
         const int64_t tgrid = ((P.ntiles + 7) / 8) * 8;
         for (; k + 2 <= n; k += 2, ++j) {
             const double rain_next = (last && k + 2 == n) ? 0.0 : rain;
-            hipLaunchKernelGGL(k_erode_pair, dim3((unsigned)tgrid), dim3(kTileThreads), 0, stream,
-                               b.meta, b.hw[0], b.hw[1], b.s[0], b.s[1], P, j, rain, rain_next);
+            const PairVariant &pv = kPairVariants[variant];
+            hipLaunchKernelGGL(pv.fn, dim3((unsigned)tgrid), dim3(pv.threads), 0, stream, b.meta,
+                               b.hw[0], b.hw[1], b.s[0], b.s[1], P, j, rain, rain_next);
         }
     }
     for (; k < n; ++k, ++j) {
@@ -363,7 +383,8 @@ static int chunk_graph(nx_ctx *ctx, const ErosionBufs &b, void *state, int64_t V
                        const int32_t *col, const double *elen, const nx_tile_plan *plan, int n,
                        double rain, bool last, hipGraphExec_t *out)
 {
-    GraphKey key{state, V, col, elen, n, rain, last, plan ? (uint64_t)(uintptr_t)plan->blob : 0};
+    GraphKey key{state, V, col, elen, n, rain, last,
+                 plan ? (uint64_t)(uintptr_t)plan->blob * 16 + (uint64_t)ctx->pair_variant : 0};
     std::lock_guard<std::mutex> lk(ctx->mu);
     auto it = ctx->graphs.find(key);
     if (it != ctx->graphs.end()) {
@@ -373,7 +394,7 @@ static int chunk_graph(nx_ctx *ctx, const ErosionBufs &b, void *state, int64_t V
     if (ctx->graphs.size() >= kMaxGraphs) ctx->clear_graphs();
     hipGraph_t g = nullptr;
     NX_HIP(hipStreamBeginCapture(ctx->capture_stream, hipStreamCaptureModeThreadLocal));
-    launch_steps(b, V, col, elen, plan, n, rain, last, ctx->capture_stream);
+    launch_steps(b, V, col, elen, plan, ctx->pair_variant, n, rain, last, ctx->capture_stream);
     const hipError_t le = hipGetLastError();
     const hipError_t ce = hipStreamEndCapture(ctx->capture_stream, &g);
     if (le != hipSuccess || ce != hipSuccess) {
@@ -422,7 +443,7 @@ int erosion_run(nx_ctx *ctx, void *state, int64_t V, const int32_t *col, const d
                            b.hw[0], b.hw[1], V, rain);
     const int chunk = ctx->graph_chunk;
     if (chunk <= 0 || iters < 2) {
-        launch_steps(b, V, col, elen, plan, iters, rain, true, stream);
+        launch_steps(b, V, col, elen, plan, ctx->pair_variant, iters, rain, true, stream);
         NX_HIP(hipGetLastError());
         return NX_OK;
     }

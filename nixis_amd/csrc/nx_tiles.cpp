@@ -31,7 +31,7 @@ namespace nx {
 namespace {
 
 constexpr int kTileMax = 1024;
-constexpr int kLoadMax = 1536;
+constexpr int kLoadMax = 2048;
 constexpr int kTileMin = 32;
 
 struct Tile {

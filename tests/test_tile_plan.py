@@ -72,7 +72,7 @@ def test_plan_covers_and_rings_are_exact(d):
     V = col.shape[1]
     nxt = 0
     for q, a, b, T, c, L, glob, lc, le in _tiles(col, elen):
-        assert a == nxt and 0 < T <= 1024 and L <= 1536 and q.max_tile <= 1024
+        assert a == nxt and 0 < T <= 1024 and L <= 2048 and q.max_tile <= 1024
         nxt = b
         inside = set(range(a, b))
         r1 = sorted(set(col[:, a:b].ravel()) - inside)
