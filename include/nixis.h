@@ -201,6 +201,8 @@ typedef struct nx_planet {
     void *erosion_state;     /* device, nx_erosion_state_bytes(V), or NULL */
     const int32_t *col;      /* device int32 [6,V] */
     const double *elen;      /* device f64 [6,V] */
+    const nx_tile_plan *plan;/* host descriptor of a tile plan for col/elen
+                                (its image on the device), or NULL */
     int64_t erosion_iters;   /* erosion steps of the whole job */
     double rain;             /* rain per step (reference: 0.3/320) */
     int64_t steps_done;      /* set to 0 by nx_batch_noise, advanced by
@@ -219,9 +221,12 @@ size_t nx_batch_workspace_bytes(int n);
 int nx_batch_noise(nx_ctx *ctx, nx_planet *planets, int n, void *workspace,
                    int tile, int blocks_per_cu, void *stream);
 
-/* Up to `max_steps` erosion steps for every planet with steps left; one
- * launch per step covers all active planets (optionally replayed from a
- * captured hipGraph).  Advances each planet's steps_done. */
+/* Up to `max_steps` erosion steps for every planet with steps left.
+ * Planets with a tile plan take their steps two per launch (one launch
+ * covers the tiles of all such planets), an odd last step and planets
+ * without a plan one per launch (one launch covers all of them); the
+ * sequence may be replayed from a captured hipGraph.  Advances each
+ * planet's steps_done.  Results equal nx_erosion_run on each planet. */
 int nx_batch_erode(nx_ctx *ctx, nx_planet *planets, int n, int max_steps,
                    void *workspace, int use_graph, void *stream);
 

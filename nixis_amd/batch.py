@@ -30,7 +30,7 @@ class NxPlanet(ctypes.Structure):
         ("perm", ctypes.c_void_p), ("pgi", ctypes.c_void_p),
         ("lower", ctypes.c_double), ("upper", ctypes.c_double),
         ("erosion_state", ctypes.c_void_p), ("col", ctypes.c_void_p), ("elen", ctypes.c_void_p),
-        ("erosion_iters", ctypes.c_int64), ("rain", ctypes.c_double),
+        ("plan", ctypes.c_void_p), ("erosion_iters", ctypes.c_int64), ("rain", ctypes.c_double),
         ("steps_done", ctypes.c_int64),
     ]
 
@@ -94,19 +94,34 @@ class Job:
 
 
 class MeshCache:
-    """Device vertices (and neighbour tables when needed) per subdivision."""
+    """Device vertices (and neighbour tables with tile plans when needed) per
+    subdivision, numbered in space-filling-curve order (util.sfc_order):
+    a planet's fields are per vertex in that order, ``order(d)[k]`` being
+    the generator's index of vertex k (``PlanetRun.heights`` maps back)."""
 
     def __init__(self, device=None):
         import torch
         self.device = device or torch.device("cuda", torch.cuda.current_device())
         self._verts = {}
         self._graphs = {}
+        self._orders = {}
+
+    def order(self, d):
+        o = self._orders.get(d)
+        if o is None:
+            o = _util.sfc_order(_util.create_points(d))
+            self._orders[d] = o
+        return o
 
     def verts(self, d):
         import torch
         v = self._verts.get(d)
         if v is None:
-            v = torch.from_numpy(_util.create_points(d)).to(self.device)
+            pts = _util.create_points(d)
+            o = self._orders.get(d)
+            if o is None:
+                o = self._orders[d] = _util.sfc_order(pts)
+            v = torch.from_numpy(np.ascontiguousarray(pts[o])).to(self.device)
             self._verts[d] = v
         return v
 
@@ -114,13 +129,14 @@ class MeshCache:
         g = self._graphs.get(d)
         if g is None:
             pts, nb = _util.icosphere_with_neighbors(d)
-            g = _erosion.ErosionGraph(pts, nb, order=None, device=self.device, tiled=False)
+            g = _erosion.ErosionGraph(pts, nb, order=self.order(d), device=self.device)
             self._graphs[d] = g
         return g
 
     def evict(self, d):
         self._verts.pop(d, None)
         self._graphs.pop(d, None)
+        self._orders.pop(d, None)
 
     def nbytes(self):
         n = sum(v.numel() * 8 for v in self._verts.values())
@@ -140,11 +156,18 @@ class PlanetRun:
         self.rough, self.strength = _terrain.octave_scalars(
             job.n_octaves, job.n_init_roughness, job.n_init_strength, job.n_roughness,
             job.n_persistence, job.world_radius)
+        self.order = meshes.order(job.d)
         self.state = None
         self.graph = None
         if job.erosion_iters > 0:
             self.graph = meshes.graph(job.d)
             self.state = _erosion.ErosionState(self.V, meshes.device)
+
+    def heights(self):
+        """The planet's heights on the host, in the generator's vertex order."""
+        out = np.empty(self.V, dtype=np.float64)
+        out[self.order] = self.elev.cpu().numpy()
+        return out
 
     def nbytes(self):
         return self.elev.numel() * 8 + (self.state.nbytes if self.state is not None else 0)
@@ -167,12 +190,14 @@ class PlanetRun:
             p.erosion_state = self.state.buf.data_ptr()
             p.col = self.graph.col.data_ptr()
             p.elen = self.graph.elen.data_ptr()
+            p.plan = ctypes.addressof(self.graph.plan) if self.graph.plan is not None else None
             p.erosion_iters = int(j.erosion_iters)
             p.rain = float(j.rain)
         else:
             p.erosion_state = None
             p.col = None
             p.elen = None
+            p.plan = None
             p.erosion_iters = 0
             p.rain = 0.0
 

@@ -25,6 +25,7 @@
 
 #include "nx_ctx.h"
 #include "nx_erosion_dev.h"
+#include "nx_pair_dev.h"
 #include "nx_noise_dev.h"
 #include "nx_reduce_dev.h"
 
@@ -50,6 +51,8 @@ struct BatchPlanetDev {
     int64_t total_iters;
     double rain;
     int64_t nblocks;  // ceil(V / 256)
+    TilePlanDev plan;
+    int32_t has_plan;
     double rough[NX_MAX_OCTAVES];
     double strength[NX_MAX_OCTAVES];
     NoiseTablesPacked tables;
@@ -59,7 +62,8 @@ constexpr int kRsTile = 4096;  // vertices per block in the rescale / load / sto
 
 // Workspace layout (offsets in bytes, all 256-aligned).
 struct BatchLayout {
-    size_t counter, keys, desc, noise_prefix, rs_prefix, erode_prefix, erode_order, steps, total;
+    size_t counter, keys, desc, noise_prefix, rs_prefix, pair_prefix, pair_order, one_prefix,
+        one_order, one_base, steps, launches, total;
 };
 
 static inline size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -68,22 +72,23 @@ static BatchLayout layout(int n)
 {
     BatchLayout L;
     size_t o = 0;
-    L.counter = o;
-    o += 256;
-    L.keys = o;
-    o += a256(sizeof(unsigned long long) * 2 * n);
-    L.desc = o;
-    o += a256(sizeof(BatchPlanetDev) * n);
-    L.noise_prefix = o;
-    o += a256(sizeof(int64_t) * (n + 1));
-    L.rs_prefix = o;
-    o += a256(sizeof(int64_t) * (n + 1));
-    L.erode_prefix = o;
-    o += a256(sizeof(int64_t) * (n + 1));
-    L.erode_order = o;
-    o += a256(sizeof(int32_t) * n);
-    L.steps = o;
-    o += a256(sizeof(int64_t) * n);
+    auto take = [&](size_t bytes) {
+        const size_t at = o;
+        o += a256(bytes);
+        return at;
+    };
+    L.counter = take(256);
+    L.keys = take(sizeof(unsigned long long) * 2 * n);
+    L.desc = take(sizeof(BatchPlanetDev) * n);
+    L.noise_prefix = take(sizeof(int64_t) * (n + 1));
+    L.rs_prefix = take(sizeof(int64_t) * (n + 1));
+    L.pair_prefix = take(sizeof(int64_t) * (n + 1));
+    L.pair_order = take(sizeof(int32_t) * n);
+    L.one_prefix = take(sizeof(int64_t) * (n + 1));
+    L.one_order = take(sizeof(int32_t) * n);
+    L.one_base = take(sizeof(int64_t) * n);
+    L.steps = take(sizeof(int64_t) * n);
+    L.launches = take(sizeof(int32_t) * n);
     L.total = o;
     return L;
 }
@@ -160,15 +165,17 @@ __global__ __launch_bounds__(256) void k_noise_batch(const BatchPlanetDev *__res
         const BatchPlanetDev &D = P[p];
         const int64_t v0 = (item - prefix[p]) * tile;
         const int64_t v1 = v0 + tile < D.V ? v0 + tile : D.V;
+        const auto verts = gmem(D.verts);
+        const auto elev = gmem(D.elev);
         for (int64_t i = v0 + threadIdx.x; i < v1; i += blockDim.x) {
-            const double x = D.verts[3 * i + 0], y = D.verts[3 * i + 1], z = D.verts[3 * i + 2];
-            double acc = D.accumulate ? D.elev[i] : 0.0;
+            const double x = verts[3 * i + 0], y = verts[3 * i + 1], z = verts[3 * i + 2];
+            double acc = D.accumulate ? elev[i] : 0.0;
             for (int o = 0; o < D.n_oct; ++o) {
                 const double r = D.rough[o];
                 const float n = noise3<false>(x * r, y * r, z * r, T, bad, nullptr);
                 acc = acc + (((double)n + 1) * 0.5 * D.strength[o]) * D.radius;
             }
-            D.elev[i] = acc;
+            elev[i] = acc;
         }
         __syncthreads();
     }
@@ -235,19 +242,22 @@ __global__ __launch_bounds__(256) void k_batch_map(const BatchPlanetDev *__restr
     }
 }
 
-// One erosion step for the first `active` planets of the erosion order.
+// One erosion step for the first `active` planets of `order`: launch k of
+// the one-step phase, after base[r] two-step launches of planet order[r].
 __global__ __launch_bounds__(256) void k_erode_batch(const BatchPlanetDev *__restrict__ P,
                                                      const int32_t *__restrict__ order,
                                                      const int64_t *__restrict__ prefix,
-                                                     int active, int64_t nb_total, int64_t k)
+                                                     const int64_t *__restrict__ base, int active,
+                                                     int64_t nb_total, int64_t k)
 {
     const int64_t b = xcd_tile(blockIdx.x, nb_total);
     if (b >= nb_total) return;
     const int r = find_slot(prefix, active, b);
     const BatchPlanetDev &D = P[order[r]];
     const ErosionMeta *meta = D.meta;
-    const int64_t t = meta->it + k;
-    const int p = (meta->buf + (int)k) & 1;
+    const int64_t launch = base[r] + k;
+    const int64_t t = meta->it + 2 * base[r] + k;
+    const int p = (meta->buf + (int)(launch & 1)) & 1;
     const double2 *hw_src = p ? D.hw1 : D.hw0;
     const double *s_src = p ? D.s1 : D.s0;
     double2 *hw_dst = p ? D.hw0 : D.hw1;
@@ -257,15 +267,42 @@ __global__ __launch_bounds__(256) void k_erode_batch(const BatchPlanetDev *__res
     if (i < D.V) erode_vertex_step(i, D.V, hw_src, s_src, hw_dst, s_dst, D.col, D.elen, rain);
 }
 
-__global__ void k_batch_advance(const BatchPlanetDev *__restrict__ P,
-                                const int32_t *__restrict__ order,
-                                const int64_t *__restrict__ steps, int n)
+// Launch k of the two-step phase: steps it + 2k and it + 2k + 1 of every
+// tile of the first `active` planets of `order` (one block per tile).
+__global__ __launch_bounds__(1024) void k_erode_batch_pair(const BatchPlanetDev *__restrict__ P,
+                                                           const int32_t *__restrict__ order,
+                                                           const int64_t *__restrict__ prefix,
+                                                           int active, int64_t nt_total,
+                                                           int64_t k)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < n) {
-        ErosionMeta *m = P[order[r]].meta;
-        m->it += steps[r];
-        m->buf ^= (int)(steps[r] & 1);
+    __shared__ double2 A[kLoadMax];
+    const int64_t b = xcd_tile(blockIdx.x, nt_total);
+    if (b >= nt_total) return;
+    // The planet is the same for the whole block: keep its descriptor in
+    // scalar registers (a local copy, so stores to the fields cannot be
+    // taken to alias it).
+    const int r = __builtin_amdgcn_readfirstlane(find_slot(prefix, active, b));
+    const BatchPlanetDev &D = P[order[r]];
+    const TilePlanDev plan = D.plan;
+    double2 *hw0 = D.hw0, *hw1 = D.hw1;
+    double *s0 = D.s0, *s1 = D.s1;
+    const int64_t t = D.meta->it + 2 * k;
+    const int p = (D.meta->buf + (int)(k & 1)) & 1;
+    const double rain_mid = t + 1 < D.total_iters ? D.rain : 0.0;
+    const double rain_next = t + 2 < D.total_iters ? D.rain : 0.0;
+    erode_tile_pair<1024, true>(A, hw0, hw1, s0, s1, p, plan, b - prefix[r], rain_mid, rain_next);
+}
+
+// After both phases: planet p took steps[p] steps in launches[p] launches.
+__global__ void k_batch_advance(const BatchPlanetDev *__restrict__ P,
+                                const int64_t *__restrict__ steps,
+                                const int32_t *__restrict__ launches, int n)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n && P[q].do_erode && steps[q] > 0) {
+        ErosionMeta *m = P[q].meta;
+        m->it += steps[q];
+        m->buf ^= launches[q] & 1;
     }
 }
 
@@ -325,6 +362,15 @@ static int fill_desc(const nx_planet &q, BatchPlanetDev &d)
         d.elen = q.elen;
         d.total_iters = q.erosion_iters;
         d.rain = q.rain;
+        if (q.plan) {
+            const nx_tile_plan &pl = *q.plan;
+            if (!pl.blob || ((uintptr_t)pl.blob & 255) || pl.V != q.V || pl.ntiles <= 0 ||
+                pl.ntiles > INT32_MAX || pl.max_tile > kTileMax || pl.max_load > kLoadMax)
+                return set_error(NX_EINVAL, "planet tile plan does not match V=%lld or the kernel limits",
+                                 (long long)q.V);
+            d.plan = plan_dev(pl);
+            d.has_plan = 1;
+        }
     }
     return NX_OK;
 }
@@ -388,50 +434,74 @@ int batch_erode(nx_ctx *ctx, nx_planet *planets, int n, int max_steps, void *ws,
     if (((uintptr_t)ws & 255) != 0) return set_error(NX_EINVAL, "workspace must be 256-byte aligned");
     if (max_steps < 0) return set_error(NX_EINVAL, "max_steps < 0");
     const BatchLayout L = layout(n);
-    // Steps of each planet in this call; erosion order = most steps first.
-    std::vector<int64_t> steps(n, 0);
-    for (int p = 0; p < n; ++p) {
-        const nx_planet &q = planets[p];
-        if (q.erosion_state) steps[p] = std::max<int64_t>(0, std::min<int64_t>(max_steps, q.erosion_iters - q.steps_done));
-    }
-    std::vector<int> ord(n);
-    std::iota(ord.begin(), ord.end(), 0);
-    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return steps[a] > steps[b]; });
-    int active0 = 0;
-    while (active0 < n && steps[ord[active0]] > 0) ++active0;
-    if (active0 == 0) return NX_OK;
-    const int64_t S = steps[ord[0]];
     std::vector<BatchPlanetDev> desc(n);
     for (int p = 0; p < n; ++p) {
         int rc = fill_desc(planets[p], desc[p]);
         if (rc) return rc;
     }
-    std::vector<int64_t> bpre(active0 + 1, 0), st(active0);
-    std::vector<int32_t> order(active0);
-    for (int r = 0; r < active0; ++r) {
-        order[r] = ord[r];
-        st[r] = steps[ord[r]];
-        bpre[r + 1] = bpre[r] + desc[ord[r]].nblocks;
+    // Steps of each planet in this call: pairs over its tiles if it has a
+    // plan, then single steps (an odd last one, or all without a plan).
+    std::vector<int64_t> steps(n, 0), pairs(n, 0), ones(n, 0);
+    std::vector<int32_t> launches(n, 0);
+    for (int p = 0; p < n; ++p) {
+        const nx_planet &q = planets[p];
+        if (q.erosion_state) steps[p] = std::max<int64_t>(0, std::min<int64_t>(max_steps, q.erosion_iters - q.steps_done));
+        pairs[p] = desc[p].has_plan ? steps[p] / 2 : 0;
+        ones[p] = steps[p] - 2 * pairs[p];
+        launches[p] = (int32_t)(pairs[p] + ones[p]);
+    }
+    auto ordered = [&](const std::vector<int64_t> &cnt, std::vector<int32_t> &ord) {
+        ord.resize(n);
+        std::iota(ord.begin(), ord.end(), 0);
+        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return cnt[a] > cnt[b]; });
+        int act = 0;
+        while (act < n && cnt[ord[act]] > 0) ++act;
+        return act;
+    };
+    std::vector<int32_t> ordP, ordS;
+    const int activeP0 = ordered(pairs, ordP), activeS0 = ordered(ones, ordS);
+    if (activeP0 == 0 && activeS0 == 0) return NX_OK;
+    const int64_t KP = activeP0 ? pairs[ordP[0]] : 0, KS = activeS0 ? ones[ordS[0]] : 0;
+    std::vector<int64_t> tpre(n + 1, 0), bpre(n + 1, 0), sbase(n, 0);
+    for (int r = 0; r < n; ++r) {
+        tpre[r + 1] = tpre[r] + (r < activeP0 ? desc[ordP[r]].plan.ntiles : 0);
+        bpre[r + 1] = bpre[r] + (r < activeS0 ? desc[ordS[r]].nblocks : 0);
+        sbase[r] = pairs[ordS[r]];
     }
     char *w = (char *)ws;
     NX_HIP(hipSetDevice(ctx->device));
     NX_HIP(hipMemcpyAsync(w + L.desc, desc.data(), sizeof(BatchPlanetDev) * n, hipMemcpyHostToDevice, stream));
-    NX_HIP(hipMemcpyAsync(w + L.erode_prefix, bpre.data(), sizeof(int64_t) * (active0 + 1), hipMemcpyHostToDevice, stream));
-    NX_HIP(hipMemcpyAsync(w + L.erode_order, order.data(), sizeof(int32_t) * active0, hipMemcpyHostToDevice, stream));
-    NX_HIP(hipMemcpyAsync(w + L.steps, st.data(), sizeof(int64_t) * active0, hipMemcpyHostToDevice, stream));
+    NX_HIP(hipMemcpyAsync(w + L.pair_prefix, tpre.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, stream));
+    NX_HIP(hipMemcpyAsync(w + L.pair_order, ordP.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, stream));
+    NX_HIP(hipMemcpyAsync(w + L.one_prefix, bpre.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, stream));
+    NX_HIP(hipMemcpyAsync(w + L.one_order, ordS.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, stream));
+    NX_HIP(hipMemcpyAsync(w + L.one_base, sbase.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, stream));
+    NX_HIP(hipMemcpyAsync(w + L.steps, steps.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, stream));
+    NX_HIP(hipMemcpyAsync(w + L.launches, launches.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, stream));
     const auto *D = (const BatchPlanetDev *)(w + L.desc);
-    const auto *O = (const int32_t *)(w + L.erode_order);
-    const auto *BP = (const int64_t *)(w + L.erode_prefix);
+    const auto *OP = (const int32_t *)(w + L.pair_order);
+    const auto *TP = (const int64_t *)(w + L.pair_prefix);
+    const auto *OS = (const int32_t *)(w + L.one_order);
+    const auto *BP = (const int64_t *)(w + L.one_prefix);
+    const auto *SB = (const int64_t *)(w + L.one_base);
     const auto *ST = (const int64_t *)(w + L.steps);
+    const auto *LN = (const int32_t *)(w + L.launches);
     auto launch_all = [&](hipStream_t s) {
-        int active = active0;
-        for (int64_t k = 0; k < S; ++k) {
-            while (active > 0 && st[active - 1] <= k) --active;
-            const int64_t nb = bpre[active];
-            const int64_t grid = ((nb + 7) / 8) * 8;
-            hipLaunchKernelGGL(k_erode_batch, dim3((unsigned)grid), dim3(256), 0, s, D, O, BP, active, nb, k);
+        int active = activeP0;
+        for (int64_t k = 0; k < KP; ++k) {
+            while (active > 0 && pairs[ordP[active - 1]] <= k) --active;
+            const int64_t nt = tpre[active];
+            hipLaunchKernelGGL(k_erode_batch_pair, dim3((unsigned)(((nt + 7) / 8) * 8)), dim3(1024), 0, s,
+                               D, OP, TP, active, nt, k);
         }
-        hipLaunchKernelGGL(k_batch_advance, dim3((active0 + 255) / 256), dim3(256), 0, s, D, O, ST, active0);
+        active = activeS0;
+        for (int64_t k = 0; k < KS; ++k) {
+            while (active > 0 && ones[ordS[active - 1]] <= k) --active;
+            const int64_t nb = bpre[active];
+            hipLaunchKernelGGL(k_erode_batch, dim3((unsigned)(((nb + 7) / 8) * 8)), dim3(256), 0, s,
+                               D, OS, BP, SB, active, nb, k);
+        }
+        hipLaunchKernelGGL(k_batch_advance, dim3((n + 255) / 256), dim3(256), 0, s, D, ST, LN, n);
     };
     if (!use_graph) {
         launch_all(stream);
@@ -440,9 +510,9 @@ int batch_erode(nx_ctx *ctx, nx_planet *planets, int n, int max_steps, void *ws,
         // Cache key: workspace, the descriptor contents and the schedule.
         uint64_t h = 1469598103934665603ull;
         h = fnv(desc.data(), sizeof(BatchPlanetDev) * n, h);
-        h = fnv(order.data(), sizeof(int32_t) * active0, h);
-        h = fnv(st.data(), sizeof(int64_t) * active0, h);
-        GraphKey key{ws, (int64_t)n, nullptr, nullptr, (int)S, 0.0, true, h};
+        h = fnv(pairs.data(), sizeof(int64_t) * n, h);
+        h = fnv(ones.data(), sizeof(int64_t) * n, h);
+        GraphKey key{ws, (int64_t)n, nullptr, nullptr, (int)(KP + KS), 0.0, true, h};
         hipGraphExec_t ex = nullptr;
         {
             std::lock_guard<std::mutex> lk(ctx->mu);

@@ -18,6 +18,7 @@
 
 #include "nx_ctx.h"
 #include "nx_erosion_dev.h"
+#include "nx_pair_dev.h"
 
 namespace nx {
 
@@ -72,31 +73,7 @@ __global__ __launch_bounds__(256) void k_erode_step(const ErosionMeta *__restric
                           rain_next);
 }
 
-// Two steps per launch over one tile of the plan (nx_tiles.cpp): the tile
-// owns vertices [start, start+T) in device order; ring 1 holds its outside
-// neighbours and ring 2 theirs.  h and w of all three are staged in LDS,
-// the first step runs on tile + ring 1 from LDS, its h and w go back to
-// LDS, and the second step runs on the tile and is written to the other
-// buffer.  Ring-1 vertices are computed by every tile that touches them,
-// with the same inputs and arithmetic, so the values agree.
-constexpr int kTileMax = 1024;   // owned vertices per tile
-constexpr int kLoadMax = 2048;   // tile + both rings
-
-struct TilePlanDev {
-    const int32_t *start;    // [ntiles + 1]
-    const int32_t *cs;       // tile + ring 1 per tile
-    const int32_t *load;     // tile + ring 1 + ring 2 per tile
-    const int64_t *rbase;    // [ntiles + 1] into ring
-    const int32_t *ring;     // global indices: ring 1 then ring 2, per tile
-    const int64_t *cbase;    // [ntiles + 1] prefix of cs
-    const uint16_t *lcol;    // local neighbour index, [6][cs] per tile
-    const double *lel;       // edge length, [6][cs] per tile
-    int64_t ntiles;
-};
-
-// THREADS per block; KEEP holds the tile's neighbour indices and edge
-// lengths in registers between the two steps (otherwise the second step
-// reads them again).
+// Two steps per launch, one block per tile of the plan (nx_pair_dev.h).
 template <int THREADS, bool KEEP>
 __global__ __launch_bounds__(THREADS) void k_erode_pair(const ErosionMeta *__restrict__ meta,
                                                        double2 *__restrict__ hw0,
@@ -105,89 +82,11 @@ __global__ __launch_bounds__(THREADS) void k_erode_pair(const ErosionMeta *__res
                                                        double *__restrict__ s1, TilePlanDev P,
                                                        int j, double rain_mid, double rain_next)
 {
-    constexpr int kLoadPer = (kLoadMax + THREADS - 1) / THREADS;
-    constexpr int kOwnPer = (kTileMax + THREADS - 1) / THREADS;
-    constexpr int kKeep = KEEP ? kOwnPer : 1;
     __shared__ double2 A[kLoadMax];
     const int64_t tile = xcd_tile(blockIdx.x, P.ntiles);
     if (tile >= P.ntiles) return;
-    const int p = (meta->buf + j) & 1;
-    const double2 *hw_src = p ? hw1 : hw0;
-    const double *s_src = p ? s1 : s0;
-    double2 *hw_dst = p ? hw0 : hw1;
-    double *s_dst = p ? s0 : s1;
-    const int64_t a = P.start[tile];
-    const int T = (int)(P.start[tile + 1] - a);
-    const int cs = P.cs[tile], L = P.load[tile];
-    const int32_t *ring = P.ring + P.rbase[tile];
-    const int64_t cb = P.cbase[tile];
-    const uint16_t *lcol = P.lcol + 6 * cb;
-    const double *lel = P.lel + 6 * cb;
-
-    for (int li = threadIdx.x; li < L; li += THREADS)
-        A[li] = hw_src[li < T ? a + li : ring[li - T]];
-    double sed[kLoadPer];
-#pragma unroll
-    for (int m = 0; m < kLoadPer; ++m) {
-        const int li = threadIdx.x + m * THREADS;
-        sed[m] = li < cs ? s_src[li < T ? a + li : ring[li - T]] : 0.0;
-    }
-    __syncthreads();
-
-    double2 res[kLoadPer];
-    int nbr[kKeep][6];
-    double len[kKeep][6];
-#pragma unroll
-    for (int m = 0; m < kLoadPer; ++m) {
-        const int li = threadIdx.x + m * THREADS;
-        res[m] = make_double2(0.0, 0.0);
-        if (li < cs) {
-            const double2 me = A[li];
-            double S = sed[m], W = me.y;
-#pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                const int n = lcol[q * cs + li];
-                const double d = lel[q * cs + li];
-                if (KEEP && m < kOwnPer) {
-                    nbr[m < kKeep ? m : 0][q] = n;
-                    len[m < kKeep ? m : 0][q] = d;
-                }
-                const double2 nb = A[n];
-                erode_accumulate(me.x, nb.x, nb.y, d, S, W);
-            }
-            double ho, wo, so;
-            erode_finish(me.x, me.y, sed[m], S, W, rain_mid, ho, wo, so);
-            res[m] = make_double2(ho, wo);
-            sed[m] = so;
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int m = 0; m < kLoadPer; ++m) {
-        const int li = threadIdx.x + m * THREADS;
-        if (li < cs) A[li] = res[m];
-    }
-    __syncthreads();
-
-#pragma unroll
-    for (int m = 0; m < kOwnPer; ++m) {
-        const int li = threadIdx.x + m * THREADS;
-        if (li < T) {
-            const double2 me = A[li];
-            double S = sed[m], W = me.y;
-#pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                const int n = KEEP ? nbr[m < kKeep ? m : 0][q] : lcol[q * cs + li];
-                const double d = KEEP ? len[m < kKeep ? m : 0][q] : lel[q * cs + li];
-                const double2 nb = A[n];
-                erode_accumulate(me.x, nb.x, nb.y, d, S, W);
-            }
-            double ho, wo, so;
-            erode_finish(me.x, me.y, sed[m], S, W, rain_next, ho, wo, so);
-            hw_dst[a + li] = make_double2(ho, wo);
-            s_dst[a + li] = so;
-        }
-    }
+    erode_tile_pair<THREADS, KEEP>(A, hw0, hw1, s0, s1, (meta->buf + j) & 1, P, tile, rain_mid,
+                                   rain_next);
 }
 
 // Kernel variants (NX_OPT_PAIR_KERNEL): block size and register use.
@@ -328,22 +227,6 @@ int erosion_load(nx_ctx *ctx, void *state, int64_t V, const double *h, const dou
                        s, V);
     NX_HIP(hipGetLastError());
     return NX_OK;
-}
-
-static TilePlanDev plan_dev(const nx_tile_plan &q)
-{
-    const char *b = (const char *)q.blob;
-    TilePlanDev P;
-    P.start = (const int32_t *)(b + q.off_start);
-    P.cs = (const int32_t *)(b + q.off_cs);
-    P.load = (const int32_t *)(b + q.off_load);
-    P.rbase = (const int64_t *)(b + q.off_rbase);
-    P.ring = (const int32_t *)(b + q.off_ring);
-    P.cbase = (const int64_t *)(b + q.off_cbase);
-    P.lcol = (const uint16_t *)(b + q.off_lcol);
-    P.lel = (const double *)(b + q.off_lel);
-    P.ntiles = q.ntiles;
-    return P;
 }
 
 // Launch n consecutive steps (the launches read the step counter and the

@@ -76,6 +76,20 @@ NX_EHD void erode_finish(double h, double w, double s, double S, double W, doubl
 }
 
 #if defined(__HIPCC__)
+// Pointer into device global memory.  Pointers read from memory (planet
+// descriptors) are generic to the compiler, which then emits flat accesses
+// that it must also order against LDS; naming the address space gives plain
+// global loads and stores.
+template <class T>
+__device__ __forceinline__ auto gmem(T *p)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return (__attribute__((address_space(1))) T *)p;
+#else
+    return p;  // the host pass only parses device code
+#endif
+}
+
 // Logical block index whose ranges are contiguous per XCD: the hardware
 // hands consecutive workgroups to the 8 XCDs round-robin, so block b runs on
 // XCD b % 8; giving each XCD one contiguous eighth of the vertices keeps the
@@ -93,19 +107,19 @@ __device__ __forceinline__ void erode_vertex_step(int64_t i, int64_t V, const do
                                                   double *s_dst, const int32_t *col,
                                                   const double *elen, double rain)
 {
-    const double2 me = hw_src[i];
-    double S = s_src[i], W = me.y;
+    const double2 me = gmem(hw_src)[i];
+    double S = gmem(s_src)[i], W = me.y;
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
-        const int32_t n = col[(int64_t)k * V + i];
-        const double d = elen[(int64_t)k * V + i];
-        const double2 nb = hw_src[n];
+        const int32_t n = gmem(col)[(int64_t)k * V + i];
+        const double d = gmem(elen)[(int64_t)k * V + i];
+        const double2 nb = gmem(hw_src)[n];
         erode_accumulate(me.x, nb.x, nb.y, d, S, W);
     }
     double ho, wo, so;
-    erode_finish(me.x, me.y, s_src[i], S, W, rain, ho, wo, so);
-    hw_dst[i] = make_double2(ho, wo);
-    s_dst[i] = so;
+    erode_finish(me.x, me.y, gmem(s_src)[i], S, W, rain, ho, wo, so);
+    gmem(hw_dst)[i] = make_double2(ho, wo);
+    gmem(s_dst)[i] = so;
 }
 #endif
 

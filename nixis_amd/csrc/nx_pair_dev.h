@@ -1,0 +1,145 @@
+// nx_pair_dev.h -- two erosion steps over one tile of a tile plan.
+//
+// A tile owns vertices [start, start+T) in device order; ring 1 holds its
+// outside neighbours and ring 2 theirs (nx_tiles.cpp builds the plan).  h and
+// w of all three are staged in LDS, the first step runs on tile + ring 1
+// from LDS, its h and w go back to LDS, and the second step runs on the tile
+// and is written to the other buffer.  Ring-1 vertices are computed by every
+// tile that touches them, with the same inputs and arithmetic
+// (nx_erosion_dev.h), so the values agree and the result equals two
+// one-step launches bit for bit.
+#pragma once
+
+#include <stdint.h>
+
+#include "../../include/nixis.h"
+#include "nx_erosion_dev.h"
+
+namespace nx {
+
+constexpr int kTileMax = 1024;   // owned vertices per tile
+constexpr int kLoadMax = 2048;   // tile + both rings
+constexpr int kTileMin = 32;     // smallest tile the planner cuts to
+
+// Device view of a plan image (pointers into nx_tile_plan::blob).
+struct TilePlanDev {
+    const int32_t *start;    // [ntiles + 1]
+    const int32_t *cs;       // tile + ring 1 per tile
+    const int32_t *load;     // tile + ring 1 + ring 2 per tile
+    const int64_t *rbase;    // [ntiles + 1] into ring
+    const int32_t *ring;     // global indices: ring 1 then ring 2, per tile
+    const int64_t *cbase;    // [ntiles + 1] prefix of cs
+    const uint16_t *lcol;    // local neighbour index, [6][cs] per tile
+    const double *lel;       // edge length, [6][cs] per tile
+    int64_t ntiles;
+};
+
+inline TilePlanDev plan_dev(const nx_tile_plan &q)
+{
+    const char *b = (const char *)q.blob;
+    TilePlanDev P;
+    P.start = (const int32_t *)(b + q.off_start);
+    P.cs = (const int32_t *)(b + q.off_cs);
+    P.load = (const int32_t *)(b + q.off_load);
+    P.rbase = (const int64_t *)(b + q.off_rbase);
+    P.ring = (const int32_t *)(b + q.off_ring);
+    P.cbase = (const int64_t *)(b + q.off_cbase);
+    P.lcol = (const uint16_t *)(b + q.off_lcol);
+    P.lel = (const double *)(b + q.off_lel);
+    P.ntiles = q.ntiles;
+    return P;
+}
+
+#if defined(__HIPCC__)
+// Two steps of tile `tile`, reading buffer p and writing buffer p ^ 1.
+// THREADS is the block size; KEEP holds the tile's neighbour indices and
+// edge lengths in registers between the steps (otherwise the second step
+// reads them again).  A is the block's LDS staging area.
+template <int THREADS, bool KEEP>
+__device__ __forceinline__ void erode_tile_pair(double2 *A, double2 *hw0, double2 *hw1, double *s0,
+                                                double *s1, int p, const TilePlanDev &P,
+                                                int64_t tile, double rain_mid, double rain_next)
+{
+    constexpr int kLoadPer = (kLoadMax + THREADS - 1) / THREADS;
+    constexpr int kOwnPer = (kTileMax + THREADS - 1) / THREADS;
+    constexpr int kKeep = KEEP ? kOwnPer : 1;
+    const auto hw_src = gmem(p ? hw1 : hw0);
+    const auto s_src = gmem(p ? s1 : s0);
+    const auto hw_dst = gmem(p ? hw0 : hw1);
+    const auto s_dst = gmem(p ? s0 : s1);
+    const int64_t a = gmem(P.start)[tile];
+    const int T = (int)(gmem(P.start)[tile + 1] - a);
+    const int cs = gmem(P.cs)[tile], L = gmem(P.load)[tile];
+    const auto ring = gmem(P.ring) + gmem(P.rbase)[tile];
+    const int64_t cb = gmem(P.cbase)[tile];
+    const auto lcol = gmem(P.lcol) + 6 * cb;
+    const auto lel = gmem(P.lel) + 6 * cb;
+
+    for (int li = threadIdx.x; li < L; li += THREADS)
+        A[li] = hw_src[li < T ? a + li : ring[li - T]];
+    double sed[kLoadPer];
+#pragma unroll
+    for (int m = 0; m < kLoadPer; ++m) {
+        const int li = threadIdx.x + m * THREADS;
+        sed[m] = li < cs ? s_src[li < T ? a + li : ring[li - T]] : 0.0;
+    }
+    __syncthreads();
+
+    double2 res[kLoadPer];
+    int nbr[kKeep][6];
+    double len[kKeep][6];
+#pragma unroll
+    for (int m = 0; m < kLoadPer; ++m) {
+        const int li = threadIdx.x + m * THREADS;
+        res[m] = make_double2(0.0, 0.0);
+        if (li < cs) {
+            const double2 me = A[li];
+            double S = sed[m], W = me.y;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                const int n = lcol[q * cs + li];
+                const double d = lel[q * cs + li];
+                if (KEEP && m < kOwnPer) {
+                    nbr[m < kKeep ? m : 0][q] = n;
+                    len[m < kKeep ? m : 0][q] = d;
+                }
+                const double2 nb = A[n];
+                erode_accumulate(me.x, nb.x, nb.y, d, S, W);
+            }
+            double ho, wo, so;
+            erode_finish(me.x, me.y, sed[m], S, W, rain_mid, ho, wo, so);
+            res[m] = make_double2(ho, wo);
+            sed[m] = so;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < kLoadPer; ++m) {
+        const int li = threadIdx.x + m * THREADS;
+        if (li < cs) A[li] = res[m];
+    }
+    __syncthreads();
+
+#pragma unroll
+    for (int m = 0; m < kOwnPer; ++m) {
+        const int li = threadIdx.x + m * THREADS;
+        if (li < T) {
+            const double2 me = A[li];
+            double S = sed[m], W = me.y;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                const int n = KEEP ? nbr[m < kKeep ? m : 0][q] : lcol[q * cs + li];
+                const double d = KEEP ? len[m < kKeep ? m : 0][q] : lel[q * cs + li];
+                const double2 nb = A[n];
+                erode_accumulate(me.x, nb.x, nb.y, d, S, W);
+            }
+            double ho, wo, so;
+            erode_finish(me.x, me.y, sed[m], S, W, rain_next, ho, wo, so);
+            hw_dst[a + li] = make_double2(ho, wo);
+            s_dst[a + li] = so;
+        }
+    }
+}
+#endif
+
+}  // namespace nx

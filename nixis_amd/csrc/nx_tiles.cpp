@@ -25,14 +25,11 @@
 #include <vector>
 
 #include "nx_internal.h"
+#include "nx_pair_dev.h"
 
 namespace nx {
 
 namespace {
-
-constexpr int kTileMax = 1024;
-constexpr int kLoadMax = 2048;
-constexpr int kTileMin = 32;
 
 struct Tile {
     int64_t a;
