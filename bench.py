@@ -292,7 +292,7 @@ def bench_served(args):
     t0 = time.perf_counter()
     log(f"[served] preparing meshes for {len(set(j.d for j in proto))} subdivisions")
     plain = serve.PlainServer()
-    served = serve.PlanetServer(hbm_budget=64 << 30, steps_per_tick=16)
+    served = serve.PlanetServer(hbm_budget=64 << 30)
     serve._prepare_meshes(plain.meshes, proto)
     served.meshes = plain.meshes   # same per-subdivision assets for both
     log(f"[served] {n} jobs, meshes in {time.perf_counter() - t0:.1f}s")
@@ -304,10 +304,21 @@ def bench_served(args):
     rate = args.served_load * n / busy
     jobs = serve.served_jobs(n, rate=rate)
     log(f"[served] plain back-to-back busy {busy:.2f}s -> rate {rate:.1f} jobs/s")
-    res_plain = serve.latency_summary(plain.serve(jobs, prepare_meshes=False))
-    res_served = serve.latency_summary(served.serve(jobs, prepare_meshes=False))
+    out_plain = plain.serve(jobs, prepare_meshes=False)
+    out_served = served.serve(jobs, prepare_meshes=False)
+    res_plain = serve.latency_summary(out_plain)
+    res_served = serve.latency_summary(out_served)
     log(f"[served] served p50 {res_served['p50_ms']:.1f} ms p99 {res_served['p99_ms']:.1f} ms; "
         f"plain p50 {res_plain['p50_ms']:.1f} ms p99 {res_plain['p99_ms']:.1f} ms")
+    for name, outs in (("served", out_served), ("plain", out_plain)):
+        worst = sorted(outs, key=lambda o: -o.latency)[:4]
+        log(f"[served] {name} slowest: " + ", ".join(
+            f"d={o.job.d} it={o.job.erosion_iters} arr={o.job.arrival:.3f}s lat={o.latency * 1e3:.0f}ms"
+            for o in worst))
+    # capacity: every job waiting at t=0, both forms
+    cap_s = serve.latency_summary(served.serve(b2b, prepare_meshes=False))
+    log(f"[served] all jobs at t=0: served makespan {cap_s['makespan_s']:.3f}s, "
+        f"plain {busy:.3f}s")
     return {
         "workload": f"{n} jobs, Poisson arrivals at {rate:.2f} jobs/s ({args.served_load:.0%} of "
                     "the plain form's back-to-back capacity), d log-uniform in [32,2048], "
@@ -317,6 +328,10 @@ def bench_served(args):
         "served": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in res_served.items()},
         "plain": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in res_plain.items()},
         "peak_job_bytes": int(served.peak_bytes),
+        "capacity_all_at_t0": {"served_makespan_s": round(cap_s["makespan_s"], 4),
+                               "plain_makespan_s": round(busy, 4),
+                               "served_jobs_per_s": round(n / cap_s["makespan_s"], 2),
+                               "plain_jobs_per_s": round(n / busy, 2)},
     }
 
 

@@ -207,6 +207,8 @@ typedef struct nx_planet {
     double rain;             /* rain per step (reference: 0.3/320) */
     int64_t steps_done;      /* set to 0 by nx_batch_noise, advanced by
                                 nx_batch_erode */
+    int64_t step_cap;        /* nx_batch_erode: at most this many steps for
+                                this planet in the call (0: max_steps) */
 } nx_planet;
 
 /* Workspace (device, 256-byte aligned) for a batch of n planets; one

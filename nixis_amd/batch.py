@@ -31,7 +31,7 @@ class NxPlanet(ctypes.Structure):
         ("lower", ctypes.c_double), ("upper", ctypes.c_double),
         ("erosion_state", ctypes.c_void_p), ("col", ctypes.c_void_p), ("elen", ctypes.c_void_p),
         ("plan", ctypes.c_void_p), ("erosion_iters", ctypes.c_int64), ("rain", ctypes.c_double),
-        ("steps_done", ctypes.c_int64),
+        ("steps_done", ctypes.c_int64), ("step_cap", ctypes.c_int64),
     ]
 
 
@@ -248,11 +248,13 @@ class BatchRunner:
             r.steps_done = 0
         return arr
 
-    def erode(self, runs, max_steps, use_graph=True):
-        """Up to max_steps erosion steps for every run with steps left."""
+    def erode(self, runs, max_steps, use_graph=True, caps=None):
+        """Up to max_steps erosion steps for every run with steps left
+        (caps: optional per-run limits for this call)."""
         arr = self._structs(runs)
         for i, r in enumerate(runs):
             arr[i].steps_done = getattr(r, "steps_done", 0)
+            arr[i].step_cap = int(caps[i]) if caps is not None else 0
         ws = self._workspace(len(runs))
         _lib.check(_L().nx_batch_erode(self.ctx.handle, arr, len(runs), int(max_steps),
                                        ws.data_ptr(), 1 if use_graph else 0,

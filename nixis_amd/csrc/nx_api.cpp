@@ -50,6 +50,35 @@ int init_tables(int64_t seed, int32_t *perm, int32_t *pgi)
     return NX_OK;
 }
 
+char *stage_begin(nx_ctx *ctx, size_t bytes, int *slot)
+{
+    std::lock_guard<std::mutex> lk(ctx->stage_mu);
+    const int i = (int)(ctx->stage_next++ % kStageSlots);
+    StageSlot &s = ctx->stage[i];
+    if (s.pending && hipEventSynchronize(s.ev) != hipSuccess) return nullptr;
+    s.pending = false;
+    if (!s.ev && hipEventCreateWithFlags(&s.ev, hipEventDisableTiming) != hipSuccess) return nullptr;
+    if (s.cap < bytes) {
+        if (s.host) (void)hipHostFree(s.host);
+        s.host = nullptr;
+        s.cap = 0;
+        const size_t cap = std::max<size_t>(bytes, 64 << 10);
+        if (hipHostMalloc(&s.host, cap, hipHostMallocDefault) != hipSuccess) return nullptr;
+        s.cap = cap;
+    }
+    *slot = i;
+    return (char *)s.host;
+}
+
+int stage_commit(nx_ctx *ctx, int slot, void *dst, size_t bytes, hipStream_t stream)
+{
+    StageSlot &s = ctx->stage[slot];
+    NX_HIP(hipMemcpyAsync(dst, s.host, bytes, hipMemcpyHostToDevice, stream));
+    NX_HIP(hipEventRecord(s.ev, stream));
+    s.pending = true;
+    return NX_OK;
+}
+
 }  // namespace nx
 
 using namespace nx;
@@ -97,6 +126,10 @@ int nx_ctx_destroy(nx_ctx *ctx)
     (void)hipSetDevice(ctx->device);
     (void)hipDeviceSynchronize();
     ctx->clear_graphs();
+    for (auto &s : ctx->stage) {
+        if (s.ev) (void)hipEventDestroy(s.ev);
+        if (s.host) (void)hipHostFree(s.host);
+    }
     if (ctx->d_err) (void)hipFree(ctx->d_err);
     if (ctx->d_keys) (void)hipFree(ctx->d_keys);
     if (ctx->capture_stream) (void)hipStreamDestroy(ctx->capture_stream);

@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <initializer_list>
 #include <numeric>
 #include <vector>
 
@@ -375,6 +376,25 @@ static int fill_desc(const nx_planet &q, BatchPlanetDev &d)
     return NX_OK;
 }
 
+// Host arrays into their places in the workspace region [L.desc, L.total),
+// in one asynchronous copy through a pinned staging buffer.
+struct Part {
+    size_t off;
+    const void *src;
+    size_t bytes;
+};
+
+static int upload(nx_ctx *ctx, char *w, const BatchLayout &L, std::initializer_list<Part> parts,
+                  hipStream_t stream)
+{
+    const size_t bytes = L.total - L.desc;
+    int slot = 0;
+    char *h = stage_begin(ctx, bytes, &slot);
+    if (!h) return set_error(NX_EHIP, "pinned staging buffer unavailable");
+    for (const Part &p : parts) std::memcpy(h + (p.off - L.desc), p.src, p.bytes);
+    return stage_commit(ctx, slot, w + L.desc, bytes, stream);
+}
+
 int batch_noise(nx_ctx *ctx, nx_planet *planets, int n, void *ws, int tile, int blocks_per_cu,
                 hipStream_t stream)
 {
@@ -397,9 +417,12 @@ int batch_noise(nx_ctx *ctx, nx_planet *planets, int n, void *ws, int tile, int 
     }
     char *w = (char *)ws;
     NX_HIP(hipSetDevice(ctx->device));
-    NX_HIP(hipMemcpyAsync(w + L.desc, desc.data(), sizeof(BatchPlanetDev) * n, hipMemcpyHostToDevice, stream));
-    NX_HIP(hipMemcpyAsync(w + L.noise_prefix, tpre.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, stream));
-    NX_HIP(hipMemcpyAsync(w + L.rs_prefix, rpre.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, stream));
+    int rc = upload(ctx, w, L,
+                    {{L.desc, desc.data(), sizeof(BatchPlanetDev) * n},
+                     {L.noise_prefix, tpre.data(), sizeof(int64_t) * (n + 1)},
+                     {L.rs_prefix, rpre.data(), sizeof(int64_t) * (n + 1)}},
+                    stream);
+    if (rc) return rc;
     auto *counter = (unsigned *)(w + L.counter);
     auto *keys = (unsigned long long *)(w + L.keys);
     const auto *D = (const BatchPlanetDev *)(w + L.desc);
@@ -445,7 +468,11 @@ int batch_erode(nx_ctx *ctx, nx_planet *planets, int n, int max_steps, void *ws,
     std::vector<int32_t> launches(n, 0);
     for (int p = 0; p < n; ++p) {
         const nx_planet &q = planets[p];
-        if (q.erosion_state) steps[p] = std::max<int64_t>(0, std::min<int64_t>(max_steps, q.erosion_iters - q.steps_done));
+        if (q.erosion_state) {
+            int64_t cap = max_steps;
+            if (q.step_cap > 0) cap = std::min<int64_t>(cap, q.step_cap);
+            steps[p] = std::max<int64_t>(0, std::min<int64_t>(cap, q.erosion_iters - q.steps_done));
+        }
         pairs[p] = desc[p].has_plan ? steps[p] / 2 : 0;
         ones[p] = steps[p] - 2 * pairs[p];
         launches[p] = (int32_t)(pairs[p] + ones[p]);
@@ -470,14 +497,17 @@ int batch_erode(nx_ctx *ctx, nx_planet *planets, int n, int max_steps, void *ws,
     }
     char *w = (char *)ws;
     NX_HIP(hipSetDevice(ctx->device));
-    NX_HIP(hipMemcpyAsync(w + L.desc, desc.data(), sizeof(BatchPlanetDev) * n, hipMemcpyHostToDevice, stream));
-    NX_HIP(hipMemcpyAsync(w + L.pair_prefix, tpre.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, stream));
-    NX_HIP(hipMemcpyAsync(w + L.pair_order, ordP.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, stream));
-    NX_HIP(hipMemcpyAsync(w + L.one_prefix, bpre.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, stream));
-    NX_HIP(hipMemcpyAsync(w + L.one_order, ordS.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, stream));
-    NX_HIP(hipMemcpyAsync(w + L.one_base, sbase.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, stream));
-    NX_HIP(hipMemcpyAsync(w + L.steps, steps.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, stream));
-    NX_HIP(hipMemcpyAsync(w + L.launches, launches.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, stream));
+    int rc = upload(ctx, w, L,
+                    {{L.desc, desc.data(), sizeof(BatchPlanetDev) * n},
+                     {L.pair_prefix, tpre.data(), sizeof(int64_t) * (n + 1)},
+                     {L.pair_order, ordP.data(), sizeof(int32_t) * n},
+                     {L.one_prefix, bpre.data(), sizeof(int64_t) * (n + 1)},
+                     {L.one_order, ordS.data(), sizeof(int32_t) * n},
+                     {L.one_base, sbase.data(), sizeof(int64_t) * n},
+                     {L.steps, steps.data(), sizeof(int64_t) * n},
+                     {L.launches, launches.data(), sizeof(int32_t) * n}},
+                    stream);
+    if (rc) return rc;
     const auto *D = (const BatchPlanetDev *)(w + L.desc);
     const auto *OP = (const int32_t *)(w + L.pair_order);
     const auto *TP = (const int64_t *)(w + L.pair_prefix);
@@ -556,8 +586,11 @@ int batch_store(nx_ctx *ctx, const nx_planet *planets, int n, void *ws, hipStrea
     }
     char *w = (char *)ws;
     NX_HIP(hipSetDevice(ctx->device));
-    NX_HIP(hipMemcpyAsync(w + L.desc, desc.data(), sizeof(BatchPlanetDev) * n, hipMemcpyHostToDevice, stream));
-    NX_HIP(hipMemcpyAsync(w + L.rs_prefix, rpre.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, stream));
+    int rc = upload(ctx, w, L,
+                    {{L.desc, desc.data(), sizeof(BatchPlanetDev) * n},
+                     {L.rs_prefix, rpre.data(), sizeof(int64_t) * (n + 1)}},
+                    stream);
+    if (rc) return rc;
     hipLaunchKernelGGL(k_batch_store, dim3((unsigned)rpre[n]), dim3(256), 0, stream,
                        (const BatchPlanetDev *)(w + L.desc), (const int64_t *)(w + L.rs_prefix), n);
     NX_HIP(hipGetLastError());

@@ -34,6 +34,16 @@ struct GraphKey {
     }
 };
 
+// Pinned host staging buffer for descriptor uploads; `ev` marks the end of
+// the last copy out of it.
+struct StageSlot {
+    void *host = nullptr;
+    size_t cap = 0;
+    hipEvent_t ev = nullptr;
+    bool pending = false;
+};
+constexpr int kStageSlots = 8;
+
 struct nx_ctx {
     int device = 0;
     int num_cus = 256;
@@ -45,6 +55,9 @@ struct nx_ctx {
     int pair_variant = 0;                  // two-step kernel variant (0 = default)
     std::map<GraphKey, hipGraphExec_t> graphs;
     std::mutex mu;
+    StageSlot stage[kStageSlots];
+    unsigned stage_next = 0;
+    std::mutex stage_mu;
 
     void clear_graphs()
     {
@@ -79,6 +92,13 @@ int erosion_run(nx_ctx *ctx, void *state, int64_t V, const int32_t *col, const d
 int erosion_store(nx_ctx *ctx, const void *state, int64_t V, double *h, double *w, double *s,
                   hipStream_t stream);
 int erosion_mass(nx_ctx *ctx, const void *state, int64_t V, double *out, hipStream_t stream);
+// Host-to-device upload through the context's ring of pinned staging
+// buffers, so that it is asynchronous (a copy from pageable memory is not).
+// stage_begin returns a pinned buffer of at least `bytes` for slot *slot,
+// waiting first for that slot's previous copy; stage_commit copies it to
+// `dst` on `stream` and records the slot's event.
+char *stage_begin(nx_ctx *ctx, size_t bytes, int *slot);
+int stage_commit(nx_ctx *ctx, int slot, void *dst, size_t bytes, hipStream_t stream);
 size_t batch_workspace_bytes(int n);
 int batch_noise(nx_ctx *ctx, nx_planet *planets, int n, void *ws, int tile, int blocks_per_cu,
                 hipStream_t stream);

@@ -10,10 +10,14 @@ served, until enough of the budget is released by finished jobs.
 
 ``PlanetServer.serve`` runs a trace in real time.  Each tick launches one
 batched noise pass (on the noise stream) for the jobs admitted since the
-last tick and one batched chunk of erosion steps (on the erosion stream) for
-every job still eroding, so jobs that arrive while others are on the GPU
-join the next tick (continuous batching) and noise-only jobs do not wait
-behind erosion.  At most two ticks are in flight.  Completion times come
+last tick and one batched chunk of erosion steps (on the erosion stream),
+so jobs that arrive while others are on the GPU join the next tick
+(continuous batching) and noise-only jobs do not wait behind erosion.  An
+erosion tick hands out a budget of vertex-steps to the eroding jobs oldest
+first, each job taking as many of its remaining steps as the budget left
+covers (at least two): a small planet finishes all its steps in one tick
+while a large one advances a few steps per tick in arrival order.  At most
+a few ticks are in flight.  Completion times come
 from GPU events relative to an event recorded at the start of the trace.
 Results stay on the device.  ``PlainServer`` runs the same trace one planet
 per call, in arrival order, for comparison.
@@ -130,7 +134,8 @@ def _prepare_meshes(meshes, jobs):
 
 
 class PlanetServer:
-    def __init__(self, hbm_budget=64 << 30, steps_per_tick=16, tuning=None, device=None):
+    def __init__(self, hbm_budget=64 << 30, steps_per_tick=None, tuning=None, device=None,
+                 tick_vertex_steps=400_000_000):
         import torch
         self.device = device or torch.device("cuda", torch.cuda.current_device())
         self.stream = torch.cuda.current_stream(self.device)
@@ -139,9 +144,26 @@ class PlanetServer:
         self.noise_runner = _batch.BatchRunner(self.device, self.stream)
         self.erode_runner = _batch.BatchRunner(self.device, self.erode_stream)
         self.budget = int(hbm_budget)
-        self.steps_per_tick = int(steps_per_tick)
+        # steps_per_tick: a fixed per-job cap per tick instead of the budget
+        self.steps_per_tick = int(steps_per_tick) if steps_per_tick else 0
+        self.tick_vertex_steps = int(tick_vertex_steps)
         self.tuning = tuning
         self.trace = bool(os.environ.get("NX_SERVE_TRACE"))
+
+    def _tick(self, eroding):
+        """(jobs, step caps) of the next erosion tick: oldest first, each
+        taking the steps the remaining budget covers (at least two)."""
+        tick, caps, left = [], [], self.tick_vertex_steps
+        for r in eroding:
+            if tick and left < 2 * r.V:
+                break
+            remaining = r.job.erosion_iters - r.steps_done
+            n = remaining if self.steps_per_tick == 0 else min(remaining, self.steps_per_tick)
+            n = min(n, max(2, left // r.V))
+            tick.append(r)
+            caps.append(n)
+            left -= n * r.V
+        return tick, caps
 
     def serve(self, jobs, keep_results=False, prepare_meshes=True):
         jobs = sorted(jobs, key=lambda j: j.arrival)
@@ -182,8 +204,9 @@ class PlanetServer:
                         r.state.buf.record_stream(self.erode_stream)
                     eroding += new_erode
             if eroding:
-                self.erode_runner.erode(eroding, self.steps_per_tick, use_graph=False)
-                finished = [r for r in eroding if r.steps_done >= r.job.erosion_iters]
+                tick, caps = self._tick(eroding)
+                self.erode_runner.erode(tick, max(caps), use_graph=False, caps=caps)
+                finished = [r for r in tick if r.steps_done >= r.job.erosion_iters]
                 if finished:
                     self.erode_runner.store(finished)
                     eroding = [r for r in eroding if r.steps_done < r.job.erosion_iters]

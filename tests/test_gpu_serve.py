@@ -49,3 +49,15 @@ def test_plain_server_matches_served():
     b = serve.PlanetServer(steps_per_tick=5).serve(jobs, keep_results=True)
     for x, y in zip(a, b):
         assert (x.heights.cpu().numpy() == y.heights.cpu().numpy()).all()
+
+
+def test_budgeted_ticks_match_plain():
+    """Erosion ticks sized by a vertex-step budget (per-planet step caps, a
+    budget small enough to split every job over many ticks, and the default
+    one) give each job the plain single-planet result."""
+    jobs = _trace()
+    a = serve.PlainServer().serve(jobs, keep_results=True)
+    for budget in (60_000, 400_000_000):
+        b = serve.PlanetServer(tick_vertex_steps=budget).serve(jobs, keep_results=True)
+        for x, y in zip(a, b):
+            assert (x.heights.cpu().numpy() == y.heights.cpu().numpy()).all(), (budget, x.job)
