@@ -62,6 +62,20 @@ def parse():
     return ap.parse_args()
 
 
+def profiled_traffic(kind, V):
+    """HBM bytes per launch of the dominant kernel, from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (profiles/traffic.json, copy-
+    calibrated), when it was measured on this workload size; else None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            e = json.load(f).get(kind)
+    except (OSError, ValueError):
+        return None
+    if not e or e.get("V") != V:
+        return None
+    return int(e["bytes_per_launch"])
+
+
 def calibrate_copy():
     """Three device-to-device copies of 1 GiB: a kernel whose HBM traffic is
     known (1 GiB read + 1 GiB written per launch)."""
@@ -182,20 +196,22 @@ def bench_erosion(args):
     m0, m1 = m0.cpu().numpy(), m1.cpu().numpy()
     gvi = V * iters / (ms * 1e-3) / 1e9
     B = 28 * V + 8 * nnz
-    per_launch_ms = ms / iters
-    gbs = B / (per_launch_ms * 1e-3) / 1e9
-    log(f"[erosion] {iters} iters in {ms:.2f} ms ({per_launch_ms * 1e3:.1f} us/iter), "
+    per_step_ms = ms / iters
+    spl = 2 if plan is not None else 1
+    gbs = B / (per_step_ms * 1e-3) / 1e9
+    log(f"[erosion] {iters} iters in {ms:.2f} ms ({per_step_ms * 1e3:.1f} us/iter), "
         f"{gvi:.2f} Gvi/s, {gbs:.0f} GB/s algorithmic")
     return {
         "workload": f"erosion at scale: d={d} icosphere ({V} vertices, {nnz} neighbour slots), "
                     f"12-octave seed-99 terrain rescaled to [-9000, 9000], rain 1.0, "
                     f"{iters} iterations, float-float FP32 fields",
         "value": round(gvi, 3), "unit": "Gvertex-iters/s", "ms_total": round(ms, 3),
-        "us_per_iter": round(per_launch_ms * 1e3, 2),
+        "us_per_iter": round(per_step_ms * 1e3, 2),
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                     "algorithmic_bytes_per_step": B,
-                     "steps_per_launch": 2 if plan is not None else 1},
+                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                     "traffic": profiled_traffic("erosion", V) if spl == 2 else None,
+                     "algorithmic_bytes_per_launch": spl * B, "steps_per_launch": spl,
+                     "launch_us": round(spl * per_step_ms * 1e3, 2)},
         "checks": {"finite": bool(np.isfinite(hh).all()),
                    "mass_drift_rel": float(abs(m1[0] - m0[0]) / max(m1[1], 1e-300)),
                    "height_range": float(hh.max() - hh.min())},
@@ -372,7 +388,8 @@ def main():
         },
         "roofline": {"bound": "valu", "achieved": round(achieved, 2),
                      "peak": round(NOISE_PEAK_TFLOPS, 1), "unit": "TFLOP/s",
-                     "frac": round(achieved / NOISE_PEAK_TFLOPS, 4), "traffic": None,
+                     "frac": round(achieved / NOISE_PEAK_TFLOPS, 4),
+                     "traffic": profiled_traffic("noise", nz["V"]),
                      "flops_per_vertex_octave": NOISE_FLOPS},
     }
     if not args.no_cpu:

@@ -19,7 +19,9 @@
 extern "C" {
 #endif
 
-/* opensimplex.py:90-107 (+ over, 110-112) */
+/* opensimplex.py:90-107 (+ over, 110-112).  Parity unpinned for the int32
+ * reading of over() (numba's int32(int32) signature): no numba here to
+ * confirm it; the tables' structure and the LCG arithmetic are tested. */
 void ora_init(int64_t seed, int32_t *perm, int32_t *pgi);
 
 /* opensimplex.py:266-759.  sel (nullable) receives the lattice selection:
@@ -72,7 +74,8 @@ void ora_erode_terrain3(const double *verts, const int32_t *nbrs, int64_t V,
                         double *h, int num_iter, double rain, double *water,
                         double *sediment);
 
-/* erosion.py:76-99 (v1) and 301-381 (v4, without its tie print) */
+/* erosion.py:76-99 (v1) and 301-381 (v4, without its tie print).  Not on
+ * the product path and not tested: parity unpinned. */
 void ora_erosion_iteration1(const int32_t *nbrs, int64_t V, const double *r,
                             double *w_out);
 void ora_erosion_iteration4(const double *verts, const int32_t *nbrs,
