@@ -33,6 +33,8 @@
 namespace nx {
 
 int pack_tables(const int32_t *perm, const int32_t *pgi, NoiseTablesPacked &p);
+int octave_weights(const double *rough, const double *strength, double radius, int n_oct,
+                   double *r_out, double *w_out, int *check_oct);
 
 struct BatchPlanetDev {
     const double *verts;
@@ -55,7 +57,8 @@ struct BatchPlanetDev {
     TilePlanDev plan;
     int32_t has_plan;
     double rough[NX_MAX_OCTAVES];
-    double strength[NX_MAX_OCTAVES];
+    double weight[NX_MAX_OCTAVES];    // 0.5 * strength * radius
+    int32_t check_oct;                 // octave with the largest |rough|
     NoiseTablesPacked tables;
 };
 
@@ -173,8 +176,9 @@ __global__ __launch_bounds__(256) void k_noise_batch(const BatchPlanetDev *__res
             double acc = D.accumulate ? elev[i] : 0.0;
             for (int o = 0; o < D.n_oct; ++o) {
                 const double r = D.rough[o];
-                const float n = noise3<false>(x * r, y * r, z * r, T, bad, nullptr);
-                acc = acc + (((double)n + 1) * 0.5 * D.strength[o]) * D.radius;
+                const float n = noise3<false>(x * r, y * r, z * r, T, o == D.check_oct, bad,
+                                              nullptr);
+                acc = fma((double)n + 1.0, D.weight[o], acc);
             }
             elev[i] = acc;
         }
@@ -338,10 +342,10 @@ static int fill_desc(const nx_planet &q, BatchPlanetDev &d)
     d.n_oct = q.n_octaves;
     d.accumulate = q.accumulate ? 1 : 0;
     d.radius = q.radius;
-    for (int o = 0; o < q.n_octaves; ++o) {
-        d.rough[o] = q.rough[o];
-        d.strength[o] = q.strength[o];
-    }
+    int check = 0;
+    rc = octave_weights(q.rough, q.strength, q.radius, q.n_octaves, d.rough, d.weight, &check);
+    if (rc) return rc;
+    d.check_oct = check;
     d.lower = q.lower;
     d.upper = q.upper;
     d.do_rescale = q.lower < q.upper;

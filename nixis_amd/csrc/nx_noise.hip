@@ -19,10 +19,10 @@ namespace nx {
 struct OctaveArgs {
     NoiseTablesPacked tables;
     double rough[NX_MAX_OCTAVES];
-    double strength[NX_MAX_OCTAVES];
-    double radius;
+    double weight[NX_MAX_OCTAVES];   // 0.5 * strength * radius
     int n_oct;
     int accumulate;
+    int check_oct;                   // octave with the largest |rough|
 };
 
 template <bool DBG>
@@ -43,9 +43,10 @@ __global__ __launch_bounds__(256) void k_sample_octaves(OctaveArgs a, const doub
         for (int o = 0; o < a.n_oct; ++o) {
             const double r = a.rough[o];
             NoiseSel sel;
-            const float n = noise3<DBG>(x * r, y * r, z * r, T, bad, &sel);
-            // terrain.py:28 then terrain.py:43
-            acc = acc + (((double)n + 1) * 0.5 * a.strength[o]) * a.radius;
+            const float n = noise3<DBG>(x * r, y * r, z * r, T, o == a.check_oct, bad, &sel);
+            // terrain.py:28 then terrain.py:43: ((n + 1) * 0.5 * strength) * radius,
+            // with the constant factors folded on the host
+            acc = fma((double)n + 1.0, a.weight[o], acc);
             if (DBG) {
                 if (dbg_noise) dbg_noise[(int64_t)o * V + i] = n;
                 if (dbg_sel) {
@@ -70,7 +71,7 @@ __global__ __launch_bounds__(256) void k_noise3d(NoiseTablesPacked tp, const dou
     bool bad = false;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-        out[i] = (double)noise3<false>(x[i], y[i], z[i], T, bad, nullptr);
+        out[i] = (double)noise3<false>(x[i], y[i], z[i], T, true, bad, nullptr);
     if (bad) __hip_atomic_fetch_or(err, NX_ERR_RANGE_BIT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
@@ -101,7 +102,28 @@ int pack_tables(const int32_t *perm, const int32_t *pgi, NoiseTablesPacked &p)
         const int gy = (k == 1 ? 11 : 4) * ((s & 2) ? -1 : 1);
         const int gz = (k == 2 ? 11 : 4) * ((s & 4) ? -1 : 1);
         p.g16[2 * g] = half_of_small(gx) | (half_of_small(gy) << 16);
-        p.g16[2 * g + 1] = half_of_small(gz);
+        const float fz = (float)gz;
+        std::memcpy(&p.g16[2 * g + 1], &fz, 4);
+    }
+    return NX_OK;
+}
+
+// Per-octave frequency, folded weight 0.5 * strength * radius, and the
+// octave with the largest |frequency| (its domain check bounds the others).
+int octave_weights(const double *rough, const double *strength, double radius, int n_oct,
+                   double *r_out, double *w_out, int *check_oct)
+{
+    *check_oct = 0;
+    double best = -1.0;
+    for (int o = 0; o < n_oct; ++o) {
+        if (!std::isfinite(rough[o]))
+            return set_error(NX_EINVAL, "octave %d frequency is not finite", o);
+        r_out[o] = rough[o];
+        w_out[o] = (0.5 * strength[o]) * radius;
+        if (std::fabs(rough[o]) > best) {
+            best = std::fabs(rough[o]);
+            *check_oct = o;
+        }
     }
     return NX_OK;
 }
@@ -127,11 +149,8 @@ int sample_octaves(nx_ctx *ctx, const double *verts, int64_t V, const int32_t *p
     std::memset(&a, 0, sizeof(a));
     int rc = pack_tables(perm, pgi, a.tables);
     if (rc) return rc;
-    for (int o = 0; o < n_oct; ++o) {
-        a.rough[o] = rough[o];
-        a.strength[o] = strength[o];
-    }
-    a.radius = radius;
+    rc = octave_weights(rough, strength, radius, n_oct, a.rough, a.weight, &a.check_oct);
+    if (rc) return rc;
     a.n_oct = n_oct;
     a.accumulate = accumulate ? 1 : 0;
     if (V == 0) return NX_OK;

@@ -24,6 +24,7 @@
 #include <stdint.h>
 
 #include <cmath>
+#include <cstring>
 
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
@@ -139,8 +140,8 @@ constexpr ExtLut make_ext_lut()
 // value are stored as bytes with wrap-around padding: entry i + 1 holds
 // table[i mod 256] for i in -1..(kWrap - 2), so every hash index used below
 // (lattice coordinate mod 256 plus an offset in -1..2, plus a table value)
-// lands in range without masking.  Then the 24 gradient vectors as fp16
-// (gx, gy | gz, 0) and the extra-vertex table.
+// lands in range without masking.  Then the 24 gradient vectors as
+// (fp16 gx | fp16 gy << 16, FP32 bits of gz) and the extra-vertex table.
 constexpr int kWrap = 520;
 struct NoiseTables {
     uint8_t perm[kWrap];
@@ -213,6 +214,17 @@ NX_ND float clamp0(float a, float cap)
 #endif
 }
 
+NX_ND float float_of_bits(uint32_t b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __uint_as_float(b);
+#else
+    float f;
+    std::memcpy(&f, &b, 4);
+    return f;
+#endif
+}
+
 NX_ND float contrib(float v, float dx, float dy, float dz, int gi, const NoiseTables &T,
                     float cap)
 {
@@ -221,7 +233,7 @@ NX_ND float contrib(float v, float dx, float dy, float dz, int gi, const NoiseTa
     const float a2 = attn * attn;
     const uint32_t w0 = T.g16[gi][0], w1 = T.g16[gi][1];
     const float dot = fmaf(half_to_float(w0 & 0xffffu), dx,
-                           fmaf(half_to_float(w0 >> 16), dy, half_to_float(w1) * dz));
+                           fmaf(half_to_float(w0 >> 16), dy, float_of_bits(w1) * dz));
     return fmaf(a2 * a2, dot, v);
 }
 
@@ -243,12 +255,13 @@ struct NoiseSel {
 };
 
 // Evaluate noise3d at (x, y, z) (already multiplied by the octave's
-// frequency).  Returns value / 103 in FP32.  `bad` is set when the point is
-// outside the exact domain (the value is then meaningless).  Every branch of
-// the reference's region logic is evaluated with selects, so a wave never
-// diverges.
+// frequency).  Returns value / 103 in FP32.  With `check`, `bad` is set when
+// the point is outside the exact domain (the value is then meaningless); the
+// caller checks the octave with the largest frequency, which bounds the
+// others.  Every branch of the reference's region logic is evaluated with
+// selects, so a wave never diverges.
 template <bool SEL>
-NX_ND float noise3(double x, double y, double z, const NoiseTables &T, bool &bad,
+NX_ND float noise3(double x, double y, double z, const NoiseTables &T, bool check, bool &bad,
                    NoiseSel *sel)
 {
     const double STRETCH = -1.0 / 6;
@@ -256,14 +269,19 @@ NX_ND float noise3(double x, double y, double z, const NoiseTables &T, bool &bad
     // opensimplex.py:272-299 (FP64, reference operation order)
     const double so = (x + y + z) * STRETCH;
     const double xs = x + so, ys = y + so, zs = z + so;
-    bad = bad | !((fabs(xs) < kNoiseCoordLimit) & (fabs(ys) < kNoiseCoordLimit) &
-                  (fabs(zs) < kNoiseCoordLimit));
+    if (check)
+        bad = bad | !((fabs(xs) < kNoiseCoordLimit) & (fabs(ys) < kNoiseCoordLimit) &
+                      (fabs(zs) < kNoiseCoordLimit));
     const int xsb = (int)floor(xs), ysb = (int)floor(ys), zsb = (int)floor(zs);
     const double fx = (double)xsb, fy = (double)ysb, fz = (double)zsb;
     const double sq = (fx + fy + fz) * SQUISH;
     const double xins = xs - fx, yins = ys - fy, zins = zs - fz;
     const double in_sum = xins + yins + zins;
-    const double dx0d = x - (fx + sq), dy0d = y - (fy + sq), dz0d = z - (fz + sq);
+    // The deltas to the cell origin, x - (fx + sq) (opensimplex.py:297-299),
+    // only feed the FP32 section: xins - (so + sq) is the same value up to
+    // FP64 rounding, with one shared addition.
+    const double t0 = so + sq;
+    const double dx0d = xins - t0, dy0d = yins - t0, dz0d = zins - t0;
 
     // ---- region logic, FP64 comparisons in the reference's form ----------
     // tetrahedron at (0,0,0): opensimplex.py:302-317
