@@ -250,6 +250,26 @@ int nx_build_adjacency(const int32_t *tris, int64_t T, int64_t V,
                        int32_t *adj);
 /* util.py:638-662 (sort_adjacency): rows into cyclic order, in place. */
 int nx_sort_adjacency(int32_t *adj, int64_t V);
+/* ---- mesh and neighbour tables (device) -------------------------------- */
+
+/* util.py:17-50: the nx_icosphere mesh generated on the device, equal to the
+ * host one bit for bit.  verts device f64 [V,3], tris device int32 [T,3];
+ * either may be NULL. */
+int nx_icosphere_device(nx_ctx *ctx, int64_t d, double *verts, int32_t *tris,
+                        void *stream);
+/* util.py:590-613 on the device, for any triangle list: the same table as
+ * nx_build_adjacency.  adj device int32 [V,6]; workspace device, 256-byte
+ * aligned, of nx_adjacency_workspace_bytes(V) bytes.  A triangle index
+ * outside 0..V-1 or a vertex in more than six triangles is reported by
+ * nx_ctx_sync_check (NX_EINVAL). */
+size_t nx_adjacency_workspace_bytes(int64_t V);
+int nx_build_adjacency_device(nx_ctx *ctx, const int32_t *tris, int64_t T,
+                              int64_t V, int32_t *adj, void *workspace,
+                              void *stream);
+/* util.py:638-662 on the device: adj_out = the rows of adj_in in cyclic
+ * order (the same table as nx_sort_adjacency); the two must differ. */
+int nx_sort_adjacency_device(nx_ctx *ctx, const int32_t *adj_in,
+                             int32_t *adj_out, int64_t V, void *stream);
 /* Space-filling-curve (3D Morton) vertex order: order[new] = old. */
 int nx_sfc_order(const double *verts, int64_t V, int64_t *order);
 /* Device neighbour layout from a [V,6] table (host arrays): col int32 [6,V]

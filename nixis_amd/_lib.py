@@ -81,6 +81,11 @@ SIGNATURES = {
                                  _c_void_p, _c_void_p]),
     "nx_erosion_run": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p, _c_void_p, _c_int,
                                 _c_double, _c_void_p]),
+    "nx_icosphere_device": (_c_int, [_c_void_p, _c_i64, _c_void_p, _c_void_p, _c_void_p]),
+    "nx_adjacency_workspace_bytes": (_c_size, [_c_i64]),
+    "nx_build_adjacency_device": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_i64, _c_void_p,
+                                           _c_void_p, _c_void_p]),
+    "nx_sort_adjacency_device": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_i64, _c_void_p]),
     "nx_tile_plan_build": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p, _c_i64,
                                     ctypes.POINTER(NxTilePlan)]),
     "nx_erosion_run_tiled": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p, _c_void_p,

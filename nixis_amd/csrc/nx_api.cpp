@@ -174,6 +174,9 @@ int nx_ctx_sync_check(nx_ctx *ctx, void *stream)
                          "noise coordinate outside the exact domain (|skewed coordinate| >= 2^30 "
                          "or not finite)");
     if (word & NX_ERR_DIV0_BIT) return set_error(NX_EDIV0, "rescale: input range is zero");
+    if (word & NX_ERR_MESH_BIT)
+        return set_error(NX_EINVAL, "mesh: a triangle index outside 0..V-1, a vertex in more than "
+                                    "six triangles, or a neighbour ring that does not close");
     return NX_OK;
 }
 
@@ -219,6 +222,25 @@ int nx_erosion_run_tiled(nx_ctx *ctx, void *state, int64_t V, const int32_t *col
                          void *stream)
 {
     return erosion_run(ctx, state, V, col, elen, plan, iters, rain, (hipStream_t)stream);
+}
+
+int nx_icosphere_device(nx_ctx *ctx, int64_t d, double *verts, int32_t *tris, void *stream)
+{
+    return icosphere_dev(ctx, d, verts, tris, (hipStream_t)stream);
+}
+
+size_t nx_adjacency_workspace_bytes(int64_t V) { return adjacency_workspace_bytes(V); }
+
+int nx_build_adjacency_device(nx_ctx *ctx, const int32_t *tris, int64_t T, int64_t V,
+                              int32_t *adj, void *workspace, void *stream)
+{
+    return build_adjacency_dev(ctx, tris, T, V, adj, workspace, (hipStream_t)stream);
+}
+
+int nx_sort_adjacency_device(nx_ctx *ctx, const int32_t *adj_in, int32_t *adj_out, int64_t V,
+                             void *stream)
+{
+    return sort_adjacency_dev(ctx, adj_in, adj_out, V, (hipStream_t)stream);
 }
 
 int nx_tile_plan_build(const int32_t *col, const double *elen, int64_t V, void *out, int64_t cap,

@@ -13,6 +13,7 @@
 // Bits of the per-context sticky error word written by kernels.
 #define NX_ERR_RANGE_BIT 1
 #define NX_ERR_DIV0_BIT 2
+#define NX_ERR_MESH_BIT 4  // device mesh builders: bad index, degree > 6 or broken ring
 
 constexpr int kKeySlots = 256;
 constexpr size_t kMaxGraphs = 64;
@@ -99,6 +100,12 @@ int erosion_mass(nx_ctx *ctx, const void *state, int64_t V, double *out, hipStre
 // `dst` on `stream` and records the slot's event.
 char *stage_begin(nx_ctx *ctx, size_t bytes, int *slot);
 int stage_commit(nx_ctx *ctx, int slot, void *dst, size_t bytes, hipStream_t stream);
+int icosphere_dev(nx_ctx *ctx, int64_t d, double *verts, int32_t *tris, hipStream_t stream);
+size_t adjacency_workspace_bytes(int64_t V);
+int build_adjacency_dev(nx_ctx *ctx, const int32_t *tris, int64_t T, int64_t V, int32_t *adj,
+                        void *workspace, hipStream_t stream);
+int sort_adjacency_dev(nx_ctx *ctx, const int32_t *adj_in, int32_t *adj_out, int64_t V,
+                       hipStream_t stream);
 size_t batch_workspace_bytes(int n);
 int batch_noise(nx_ctx *ctx, nx_planet *planets, int n, void *ws, int tile, int blocks_per_cu,
                 hipStream_t stream);

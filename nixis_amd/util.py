@@ -103,3 +103,57 @@ def icosphere_with_neighbors(divisions):
     adj = build_adjacency(cells)
     sort_adjacency(adj)
     return pts, adj
+
+
+# ---- device-side mesh (nx_*_device): the same arrays, generated on the GPU ----
+
+def create_mesh_device(divisions, device=None, stream=None):
+    """util.py:17-50 on the device: (points float64 [V,3], cells int32 [T,3])
+    tensors equal to create_mesh(divisions) bit for bit."""
+    import torch
+    V, T = icosphere_counts(divisions)
+    if device is None:
+        device = torch.device("cuda", torch.cuda.current_device())
+    pts = torch.empty((V, 3), dtype=torch.float64, device=device)
+    cells = torch.empty((T, 3), dtype=torch.int32, device=device)
+    ctx = _lib.Context.get(pts.device)
+    _lib.check(_lib.lib().nx_icosphere_device(ctx.handle, int(divisions), _lib.ptr(pts),
+                                              _lib.ptr(cells), _lib.stream_ptr(stream)))
+    return pts, cells
+
+
+def build_adjacency_device(cells, V=None, stream=None):
+    """util.py:590-613 on the device: int32 [V,6] tensor, -1 padded."""
+    import torch
+    if cells.dtype != torch.int32 or cells.ndim != 2 or cells.shape[1] != 3 or not cells.is_contiguous():
+        raise ValueError("cells must be a contiguous int32 [T,3] device tensor")
+    T = cells.shape[0]
+    V = (T + 4) // 2 if V is None else int(V)
+    adj = torch.empty((V, 6), dtype=torch.int32, device=cells.device)
+    ws = torch.empty(int(_lib.lib().nx_adjacency_workspace_bytes(V)), dtype=torch.uint8,
+                     device=cells.device)
+    ctx = _lib.Context.get(cells.device)
+    _lib.check(_lib.lib().nx_build_adjacency_device(ctx.handle, _lib.ptr(cells), T, V,
+                                                    _lib.ptr(adj), _lib.ptr(ws),
+                                                    _lib.stream_ptr(stream)))
+    ctx.sync_check(stream)
+    return adj
+
+
+def sort_adjacency_device(adj, stream=None):
+    """util.py:638-662 on the device: a new int32 [V,6] tensor in cyclic order."""
+    import torch
+    if adj.dtype != torch.int32 or adj.ndim != 2 or adj.shape[1] != 6 or not adj.is_contiguous():
+        raise ValueError("adj must be a contiguous int32 [V,6] device tensor")
+    out = torch.empty_like(adj)
+    ctx = _lib.Context.get(adj.device)
+    _lib.check(_lib.lib().nx_sort_adjacency_device(ctx.handle, _lib.ptr(adj), _lib.ptr(out),
+                                                   adj.shape[0], _lib.stream_ptr(stream)))
+    ctx.sync_check(stream)
+    return out
+
+
+def icosphere_with_neighbors_device(divisions, device=None):
+    """(points, neighbours) of a d-division icosphere, generated on the device."""
+    pts, cells = create_mesh_device(divisions, device)
+    return pts, sort_adjacency_device(build_adjacency_device(cells, pts.shape[0]))
