@@ -270,6 +270,20 @@ int nx_build_adjacency_device(nx_ctx *ctx, const int32_t *tris, int64_t T,
  * order (the same table as nx_sort_adjacency); the two must differ. */
 int nx_sort_adjacency_device(nx_ctx *ctx, const int32_t *adj_in,
                              int32_t *adj_out, int64_t V, void *stream);
+/* nx_sfc_order on the device (same order; verts device f64 [V,3], order
+ * device int64 [V]), workspace of nx_sfc_workspace_bytes(V) bytes. */
+size_t nx_sfc_workspace_bytes(int64_t V);
+int nx_sfc_order_device(nx_ctx *ctx, const double *verts, int64_t V,
+                        int64_t *order, void *workspace, void *stream);
+/* nx_ell_from_adjacency on the device (same col and elen; all arrays on
+ * the device).  With an order, workspace of nx_ell_workspace_bytes(V)
+ * bytes; a bad neighbour index or order is reported by nx_ctx_sync_check. */
+size_t nx_ell_workspace_bytes(int64_t V);
+int nx_ell_from_adjacency_device(nx_ctx *ctx, const int32_t *adj, int64_t V,
+                                 const double *verts, const int64_t *order,
+                                 int32_t *col, double *elen, void *workspace,
+                                 void *stream);
+
 /* Space-filling-curve (3D Morton) vertex order: order[new] = old. */
 int nx_sfc_order(const double *verts, int64_t V, int64_t *order);
 /* Device neighbour layout from a [V,6] table (host arrays): col int32 [6,V]

@@ -86,6 +86,12 @@ SIGNATURES = {
     "nx_build_adjacency_device": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_i64, _c_void_p,
                                            _c_void_p, _c_void_p]),
     "nx_sort_adjacency_device": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_i64, _c_void_p]),
+    "nx_sfc_workspace_bytes": (_c_size, [_c_i64]),
+    "nx_sfc_order_device": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p, _c_void_p,
+                                     _c_void_p]),
+    "nx_ell_workspace_bytes": (_c_size, [_c_i64]),
+    "nx_ell_from_adjacency_device": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p, _c_void_p,
+                                              _c_void_p, _c_void_p, _c_void_p, _c_void_p]),
     "nx_tile_plan_build": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p, _c_i64,
                                     ctypes.POINTER(NxTilePlan)]),
     "nx_erosion_run_tiled": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p, _c_void_p,

@@ -97,19 +97,39 @@ class MeshCache:
     """Device vertices (and neighbour tables with tile plans when needed) per
     subdivision, numbered in space-filling-curve order (util.sfc_order):
     a planet's fields are per vertex in that order, ``order(d)[k]`` being
-    the generator's index of vertex k (``PlanetRun.heights`` maps back)."""
+    the generator's index of vertex k (``PlanetRun.heights`` maps back).
+    With ``on_device`` (default) the icosphere, its neighbour tables, the
+    order and the device neighbour layout are generated on the GPU (the same
+    arrays as the host generator); the tile plan is cut on the host."""
 
-    def __init__(self, device=None):
+    def __init__(self, device=None, on_device=True):
         import torch
         self.device = device or torch.device("cuda", torch.cuda.current_device())
+        self.on_device = bool(on_device)
         self._verts = {}
         self._graphs = {}
         self._orders = {}
+        self._orders_dev = {}
+
+    def _order_dev(self, d):
+        import torch
+        od = self._orders_dev.get(d)
+        if od is None:
+            if self.on_device:
+                pts, _ = _util.create_mesh_device(d, self.device, cells=False)
+                od = _util.sfc_order_device(pts)
+            else:
+                od = torch.from_numpy(self.order(d)).to(self.device)
+            self._orders_dev[d] = od
+        return od
 
     def order(self, d):
         o = self._orders.get(d)
         if o is None:
-            o = _util.sfc_order(_util.create_points(d))
+            if self.on_device:
+                o = self._order_dev(d).cpu().numpy()
+            else:
+                o = _util.sfc_order(_util.create_points(d))
             self._orders[d] = o
         return o
 
@@ -117,19 +137,27 @@ class MeshCache:
         import torch
         v = self._verts.get(d)
         if v is None:
-            pts = _util.create_points(d)
-            o = self._orders.get(d)
-            if o is None:
-                o = self._orders[d] = _util.sfc_order(pts)
-            v = torch.from_numpy(np.ascontiguousarray(pts[o])).to(self.device)
+            if self.on_device:
+                pts, _ = _util.create_mesh_device(d, self.device, cells=False)
+                v = pts.index_select(0, self._order_dev(d)).contiguous()
+            else:
+                pts = _util.create_points(d)
+                v = torch.from_numpy(np.ascontiguousarray(pts[self.order(d)])).to(self.device)
             self._verts[d] = v
         return v
 
     def graph(self, d):
         g = self._graphs.get(d)
         if g is None:
-            pts, nb = _util.icosphere_with_neighbors(d)
-            g = _erosion.ErosionGraph(pts, nb, order=self.order(d), device=self.device)
+            if self.on_device:
+                pts, cells = _util.create_mesh_device(d, self.device)
+                adj = _util.sort_adjacency_device(
+                    _util.build_adjacency_device(cells, pts.shape[0]))
+                del cells
+                g = _erosion.ErosionGraph.from_device(pts, adj, order=self._order_dev(d))
+            else:
+                pts, nb = _util.icosphere_with_neighbors(d)
+                g = _erosion.ErosionGraph(pts, nb, order=self.order(d), device=self.device)
             self._graphs[d] = g
         return g
 
@@ -137,6 +165,7 @@ class MeshCache:
         self._verts.pop(d, None)
         self._graphs.pop(d, None)
         self._orders.pop(d, None)
+        self._orders_dev.pop(d, None)
 
     def nbytes(self):
         n = sum(v.numel() * 8 for v in self._verts.values())
@@ -156,12 +185,17 @@ class PlanetRun:
         self.rough, self.strength = _terrain.octave_scalars(
             job.n_octaves, job.n_init_roughness, job.n_init_strength, job.n_roughness,
             job.n_persistence, job.world_radius)
-        self.order = meshes.order(job.d)
+        self.meshes = meshes
         self.state = None
         self.graph = None
         if job.erosion_iters > 0:
             self.graph = meshes.graph(job.d)
             self.state = _erosion.ErosionState(self.V, meshes.device)
+
+    @property
+    def order(self):
+        """order[k] = the generator's index of device vertex k."""
+        return self.meshes.order(self.job.d)
 
     def heights(self):
         """The planet's heights on the host, in the generator's vertex order."""

@@ -243,6 +243,24 @@ int nx_sort_adjacency_device(nx_ctx *ctx, const int32_t *adj_in, int32_t *adj_ou
     return sort_adjacency_dev(ctx, adj_in, adj_out, V, (hipStream_t)stream);
 }
 
+size_t nx_sfc_workspace_bytes(int64_t V) { return sfc_workspace_bytes(V); }
+
+int nx_sfc_order_device(nx_ctx *ctx, const double *verts, int64_t V, int64_t *order,
+                        void *workspace, void *stream)
+{
+    return sfc_order_dev(ctx, verts, V, order, workspace, (hipStream_t)stream);
+}
+
+size_t nx_ell_workspace_bytes(int64_t V) { return ell_workspace_bytes(V); }
+
+int nx_ell_from_adjacency_device(nx_ctx *ctx, const int32_t *adj, int64_t V, const double *verts,
+                                 const int64_t *order, int32_t *col, double *elen,
+                                 void *workspace, void *stream)
+{
+    return ell_from_adjacency_dev(ctx, adj, V, verts, order, col, elen, workspace,
+                                  (hipStream_t)stream);
+}
+
 int nx_tile_plan_build(const int32_t *col, const double *elen, int64_t V, void *out, int64_t cap,
                        nx_tile_plan *plan)
 {

@@ -106,6 +106,13 @@ int build_adjacency_dev(nx_ctx *ctx, const int32_t *tris, int64_t T, int64_t V, 
                         void *workspace, hipStream_t stream);
 int sort_adjacency_dev(nx_ctx *ctx, const int32_t *adj_in, int32_t *adj_out, int64_t V,
                        hipStream_t stream);
+size_t sfc_workspace_bytes(int64_t V);
+int sfc_order_dev(nx_ctx *ctx, const double *verts, int64_t V, int64_t *order, void *ws,
+                  hipStream_t stream);
+size_t ell_workspace_bytes(int64_t V);
+int ell_from_adjacency_dev(nx_ctx *ctx, const int32_t *adj, int64_t V, const double *verts,
+                           const int64_t *order, int32_t *col, double *elen, void *ws,
+                           hipStream_t stream);
 size_t batch_workspace_bytes(int n);
 int batch_noise(nx_ctx *ctx, nx_planet *planets, int n, void *ws, int tile, int blocks_per_cu,
                 hipStream_t stream);

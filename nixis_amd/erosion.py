@@ -70,6 +70,43 @@ class ErosionGraph:
         if tiled:
             self._build_plan(col, elen)
 
+    @classmethod
+    def from_device(cls, verts, neighbors, order="sfc", tiled=True, stream=None):
+        """The same graph from device tensors (verts float64 [V,3], neighbours
+        int32 [V,6]), built on the device (nx_sfc_order_device,
+        nx_ell_from_adjacency_device); the tile plan is cut on the host from
+        copies of the device tables."""
+        import torch
+        from . import util
+        V = verts.shape[0]
+        if neighbors.shape != (V, 6) or neighbors.dtype != torch.int32:
+            raise ValueError("neighbors must be an int32 [V,6] device tensor")
+        if isinstance(order, str):
+            if order != "sfc":
+                raise ValueError(f"unknown order {order!r}")
+            order = util.sfc_order_device(verts, stream)
+        self = cls.__new__(cls)
+        self.V = V
+        self.device = verts.device
+        col = torch.empty((6, V), dtype=torch.int32, device=verts.device)
+        elen = torch.empty((6, V), dtype=torch.float64, device=verts.device)
+        ws = torch.empty(int(_lib.lib().nx_ell_workspace_bytes(V)), dtype=torch.uint8,
+                         device=verts.device)
+        ctx = _lib.Context.get(verts.device)
+        _lib.check(_lib.lib().nx_ell_from_adjacency_device(
+            ctx.handle, _lib.ptr(neighbors.contiguous()), V, _lib.ptr(verts.contiguous()),
+            _lib.ptr(order), _lib.ptr(col), _lib.ptr(elen), _lib.ptr(ws),
+            _lib.stream_ptr(stream)))
+        ctx.sync_check(stream)
+        self.col, self.elen = col, elen
+        self._odev = order
+        self.order = order.cpu().numpy() if order is not None else None
+        self.plan = None
+        self._plan_blob = None
+        if tiled:
+            self._build_plan(col.cpu().numpy(), elen.cpu().numpy())
+        return self
+
     def _build_plan(self, col, elen):
         import torch
         L = _lib.lib()

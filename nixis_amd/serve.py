@@ -129,6 +129,7 @@ def _prepare_meshes(meshes, jobs):
     """Per-subdivision assets (vertices, neighbour tables) built up front."""
     for j in jobs:
         meshes.verts(j.d)
+        meshes.order(j.d)
         if j.erosion_iters > 0:
             meshes.graph(j.d)
 

@@ -107,19 +107,19 @@ def icosphere_with_neighbors(divisions):
 
 # ---- device-side mesh (nx_*_device): the same arrays, generated on the GPU ----
 
-def create_mesh_device(divisions, device=None, stream=None):
-    """util.py:17-50 on the device: (points float64 [V,3], cells int32 [T,3])
-    tensors equal to create_mesh(divisions) bit for bit."""
+def create_mesh_device(divisions, device=None, stream=None, cells=True):
+    """util.py:17-50 on the device: (points float64 [V,3], cells int32 [T,3]
+    or None) tensors equal to create_mesh(divisions) bit for bit."""
     import torch
     V, T = icosphere_counts(divisions)
     if device is None:
         device = torch.device("cuda", torch.cuda.current_device())
     pts = torch.empty((V, 3), dtype=torch.float64, device=device)
-    cells = torch.empty((T, 3), dtype=torch.int32, device=device)
+    tris = torch.empty((T, 3), dtype=torch.int32, device=device) if cells else None
     ctx = _lib.Context.get(pts.device)
     _lib.check(_lib.lib().nx_icosphere_device(ctx.handle, int(divisions), _lib.ptr(pts),
-                                              _lib.ptr(cells), _lib.stream_ptr(stream)))
-    return pts, cells
+                                              _lib.ptr(tris), _lib.stream_ptr(stream)))
+    return pts, tris
 
 
 def build_adjacency_device(cells, V=None, stream=None):
@@ -157,3 +157,19 @@ def icosphere_with_neighbors_device(divisions, device=None):
     """(points, neighbours) of a d-division icosphere, generated on the device."""
     pts, cells = create_mesh_device(divisions, device)
     return pts, sort_adjacency_device(build_adjacency_device(cells, pts.shape[0]))
+
+
+def sfc_order_device(points, stream=None):
+    """sfc_order on the device: int64 [V] tensor, order[new] = old."""
+    import torch
+    if points.dtype != torch.float64 or points.ndim != 2 or points.shape[1] != 3 \
+            or not points.is_contiguous():
+        raise ValueError("points must be a contiguous float64 [V,3] device tensor")
+    V = points.shape[0]
+    order = torch.empty(V, dtype=torch.int64, device=points.device)
+    ws = torch.empty(int(_lib.lib().nx_sfc_workspace_bytes(V)), dtype=torch.uint8,
+                     device=points.device)
+    ctx = _lib.Context.get(points.device)
+    _lib.check(_lib.lib().nx_sfc_order_device(ctx.handle, _lib.ptr(points), V, _lib.ptr(order),
+                                              _lib.ptr(ws), _lib.stream_ptr(stream)))
+    return order

@@ -55,3 +55,35 @@ def test_device_mesh_at_scale():
     dp, dn = util.icosphere_with_neighbors_device(d)
     assert (dp.cpu().numpy() == pts).all()
     assert (dn.cpu().numpy() == nb).all()
+
+
+@pytest.mark.parametrize("d", [3, 40, 300])
+def test_device_sfc_order_and_layout_equal_host(d):
+    from nixis_amd import erosion
+    pts, nb = util.icosphere_with_neighbors(d)
+    dp, dn = util.icosphere_with_neighbors_device(d)
+    order = util.sfc_order(pts)
+    dorder = util.sfc_order_device(dp)
+    assert (dorder.cpu().numpy() == order).all()
+    g_host = erosion.ErosionGraph(pts, nb)
+    g_dev = erosion.ErosionGraph.from_device(dp, dn)
+    assert (g_dev.order == g_host.order).all()
+    assert (g_dev.col.cpu().numpy() == g_host.col.cpu().numpy()).all()
+    assert (g_dev.elen.cpu().numpy() == g_host.elen.cpu().numpy()).all()
+    assert (g_dev.plan is None) == (g_host.plan is None)
+    if g_host.plan is not None:
+        assert (g_dev._plan_blob.cpu().numpy() == g_host._plan_blob.cpu().numpy()).all()
+
+
+def test_mesh_cache_device_equals_host_path():
+    from nixis_amd import batch
+    job = batch.Job(d=48, seed=5, n_octaves=6, erosion_iters=17, rain=1.0)
+    outs = []
+    for on_device in (True, False):
+        meshes = batch.MeshCache(on_device=on_device)
+        run = batch.PlanetRun(job, meshes)
+        batch.BatchRunner().run([run])
+        outs.append(run.heights())
+        assert (meshes.verts(48).cpu().numpy() ==
+                util.create_points(48)[meshes.order(48)]).all()
+    assert (outs[0] == outs[1]).all()
