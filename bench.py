@@ -204,7 +204,7 @@ def bench_erosion(args):
     return {
         "workload": f"erosion at scale: d={d} icosphere ({V} vertices, {nnz} neighbour slots), "
                     f"12-octave seed-99 terrain rescaled to [-9000, 9000], rain 1.0, "
-                    f"{iters} iterations, float-float FP32 fields",
+                    f"{iters} iterations, FP64 fields in the reference's operation order",
         "value": round(gvi, 3), "unit": "Gvertex-iters/s", "ms_total": round(ms, 3),
         "us_per_iter": round(per_step_ms * 1e3, 2),
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
@@ -217,6 +217,58 @@ def bench_erosion(args):
                    "height_range": float(hh.max() - hh.min())},
         "_graph": graph, "_h": h, "_V": V,
     }
+
+
+CHAIN_BYTES_PER_VERTEX = 75  # five passes: reads 8+8+9+9+8, writes 9+8+8+8
+
+
+def bench_post_chain(args, nz):
+    """nixis.py:330-364 on the noise output (d=2000): rescale to
+    [-4000, 8850], ocean level at 55%, mask, power_rescale 0.5 / 2.0, shift,
+    split rescale.  Each timed call gets a fresh copy of the noise heights
+    (copied outside the timed events)."""
+    import torch
+    from nixis_amd import _lib, util
+    src = torch.from_numpy(nz["out"]).cuda()
+    h = torch.empty_like(src)
+    n = src.numel()
+    L = _lib.lib()
+    ws = torch.empty(L.nx_height_chain_workspace_bytes(n), dtype=torch.uint8, device="cuda")
+    for _ in range(args.warmup):
+        h.copy_(src)
+        util.height_chain_device(h, -4000.0, 8850.0, 55.0, workspace=ws)
+    total = 0.0
+    for _ in range(args.steps):
+        h.copy_(src)
+        total += _events_ms(lambda: util.height_chain_device(h, -4000.0, 8850.0, 55.0,
+                                                             workspace=ws))
+    _lib.Context.get(h.device).sync_check()
+    ms = total / args.steps
+    gbs = CHAIN_BYTES_PER_VERTEX * n / (ms * 1e-3) / 1e9
+    log(f"[chain] {ms:.3f} ms per chain over {n} heights, {gbs:.0f} GB/s")
+    # noise and chain as one call (min/max reduced in the noise kernel)
+    from nixis_amd import terrain
+    rough, strength = terrain.octave_scalars(nz["n_oct"], nz["f0"], 0.4, nz["lac"], nz["pers"],
+                                             1.0)
+    dv = torch.from_numpy(nz["pts"]).cuda()
+
+    def fused():
+        terrain.terrain_heights_device(dv, nz["perm"], nz["pgi"], rough, strength, 1.0, -4000.0,
+                                       8850.0, 55.0, workspace=ws, out=h)
+    for _ in range(args.warmup):
+        fused()
+    fused_ms = _events_ms(lambda: [fused() for _ in range(args.steps)]) / args.steps
+    _lib.Context.get(h.device).sync_check()
+    log(f"[chain] noise + chain in one call: {fused_ms:.3f} ms "
+        f"(noise alone {nz['ms']:.3f} ms + chain {ms:.3f} ms)")
+    return {"workload": f"height chain nixis.py:330-364 on the d={args.noise_d} noise heights "
+                        f"({n} vertices): rescale [-4000, 8850], ocean 55%, power 0.5 / 2.0, "
+                        "shift, split rescale",
+            "ms": round(ms, 4), "Gvertex_per_s": round(n / (ms * 1e-3) / 1e9, 2),
+            "fused_noise_chain_ms": round(fused_ms, 4), "noise_ms": round(nz["ms"], 4),
+            "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                         "bytes_per_vertex": CHAIN_BYTES_PER_VERTEX}}
 
 
 def cpu_erosion_baseline(er, iters=4):
@@ -394,6 +446,7 @@ def main():
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_noise_baseline(nz)
+    line["post_chain"] = bench_post_chain(args, nz)
     if not args.no_erosion:
         er = bench_erosion(args)
         if not args.no_cpu:

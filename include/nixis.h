@@ -111,6 +111,68 @@ int nx_noise3d(nx_ctx *ctx, const double *x, const double *y, const double *z,
 int nx_rescale(nx_ctx *ctx, const double *x, int64_t n, double lower,
                double upper, double *out, double *minmax, void *stream);
 
+/* util.py:110-175 (rescale with every option).  opts == NULL is the
+ * standard form.  has_mid with mode 0 rescales [min, mid] onto
+ * [lower, mid] and (mid, max] onto (mid, upper]; mode 1 ('lower') maps only
+ * the values <= mid, mode 2 ('upper') only the values >= mid, copying the
+ * rest; u_min / u_max widen the input range, never narrow it.  A mode
+ * without mid is NX_EINVAL (the reference prints an error and returns x).
+ * A zero range that a value divides by is NX_EDIV0 at the next sync.
+ * minmax (nullable) receives the input's (min, max) before u_min/u_max. */
+typedef struct nx_rescale_opts {
+    double mid, u_min, u_max;
+    int has_mid;
+    int mode;                /* 0 none, 1 'lower', 2 'upper' */
+    int has_umin, has_umax;
+} nx_rescale_opts;
+
+int nx_rescale_ex(nx_ctx *ctx, const double *x, int64_t n, double lower,
+                  double upper, const nx_rescale_opts *opts, double *out,
+                  double *minmax, void *stream);
+
+/* terrain.py:61-72: mask[i] = h[i] <= level (u8 0/1). */
+int nx_make_bool_elevation_mask(nx_ctx *ctx, const double *h, int64_t n,
+                                double level, uint8_t *mask, void *stream);
+
+/* util.py:178-254 (power_rescale).  mode 1 rescales the values whose mask is
+ * set, mode 0 the others, through [0, 1] ** power and back, over the range
+ * the reference's sequential loop finds: lower = min of the selected values
+ * (starting from max(x)); upper = max of the selected values that were not
+ * a new running minimum when the loop reached them (starting from min(x)),
+ * found from in-order chunk summaries on the device.  x and out may alias.
+ * ws: device workspace of nx_power_rescale_workspace_bytes(n) bytes,
+ * 256-byte aligned. */
+size_t nx_power_rescale_workspace_bytes(int64_t n);
+int nx_power_rescale(nx_ctx *ctx, const double *x, const uint8_t *mask,
+                     int64_t n, int mode, double power, double *out, void *ws,
+                     void *stream);
+
+/* nixis.py:330-364: the height chain after noise, in place on h:
+ *   h = rescale(h, min_alt, max_alt)
+ *   ocean_level = find_percent_val(min(h), max(h), ocean_percent)  (util.py:556-566)
+ *   mask = h <= ocean_level
+ *   h = power_rescale(h, mask, 1, 0.5); h = power_rescale(h, mask, 0, 2.0)
+ *   h -= ocean_level; h = rescale(h, min_alt, max_alt, mid=0)
+ * ocean_level (device f64[1]) and mask (device u8 [n]) are outputs; nothing
+ * returns to the host.  ws: nx_height_chain_workspace_bytes(n), 256-aligned. */
+size_t nx_height_chain_workspace_bytes(int64_t n);
+int nx_height_chain(nx_ctx *ctx, double *h, int64_t n, double min_alt,
+                    double max_alt, double ocean_percent, uint8_t *mask,
+                    double *ocean_level, void *ws, void *stream);
+
+/* terrain.py:32-59 then nixis.py:330-364 in one call: the noise heights at
+ * the vertices (starting from zero, as nixis.py:330 passes no elevations)
+ * with their min/max reduced inside the noise kernel, then the height chain
+ * on them.  Arguments as nx_sample_octaves and nx_height_chain; h (device f64
+ * [V]) receives the final heights.  Results equal nx_sample_octaves followed
+ * by nx_height_chain.  ws: nx_height_chain_workspace_bytes(V). */
+int nx_terrain_heights(nx_ctx *ctx, const double *verts, int64_t V,
+                       const int32_t *perm, const int32_t *pgi, int n_octaves,
+                       const double *rough, const double *strength,
+                       double radius, double min_alt, double max_alt,
+                       double ocean_percent, double *h, uint8_t *mask,
+                       double *ocean_level, void *ws, void *stream);
+
 /* ---- erosion (device) -------------------------------------------------- */
 
 /* Fields live in a caller-allocated device workspace of

@@ -55,6 +55,13 @@ class NxTilePlan(ctypes.Structure):
         for n in ("start", "cs", "load", "rbase", "ring", "cbase", "lcol", "lel")]
 
 
+class NxRescaleOpts(ctypes.Structure):
+    """Mirror of nx_rescale_opts (include/nixis.h)."""
+    _fields_ = [("mid", ctypes.c_double), ("u_min", ctypes.c_double), ("u_max", ctypes.c_double),
+                ("has_mid", ctypes.c_int), ("mode", ctypes.c_int), ("has_umin", ctypes.c_int),
+                ("has_umax", ctypes.c_int)]
+
+
 _c_i64 = ctypes.c_int64
 _c_int = ctypes.c_int
 _c_double = ctypes.c_double
@@ -76,6 +83,20 @@ SIGNATURES = {
                             _c_void_p, _c_void_p, _c_void_p]),
     "nx_rescale": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_double, _c_double, _c_void_p,
                             _c_void_p, _c_void_p]),
+    "nx_rescale_ex": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_double, _c_double,
+                               ctypes.POINTER(NxRescaleOpts), _c_void_p, _c_void_p, _c_void_p]),
+    "nx_make_bool_elevation_mask": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_double,
+                                             _c_void_p, _c_void_p]),
+    "nx_power_rescale_workspace_bytes": (_c_size, [_c_i64]),
+    "nx_power_rescale": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_i64, _c_int, _c_double,
+                                  _c_void_p, _c_void_p, _c_void_p]),
+    "nx_height_chain_workspace_bytes": (_c_size, [_c_i64]),
+    "nx_terrain_heights": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p, _c_void_p, _c_int,
+                                    _c_void_p, _c_void_p, _c_double, _c_double, _c_double,
+                                    _c_double, _c_void_p, _c_void_p, _c_void_p, _c_void_p,
+                                    _c_void_p]),
+    "nx_height_chain": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_double, _c_double, _c_double,
+                                 _c_void_p, _c_void_p, _c_void_p, _c_void_p]),
     "nx_erosion_state_bytes": (_c_size, [_c_i64]),
     "nx_erosion_load": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p, _c_void_p,
                                  _c_void_p, _c_void_p]),
@@ -109,7 +130,8 @@ SIGNATURES = {
 }
 
 _lib = None
-_lock = threading.Lock()
+_lock = threading.Lock()       # library loading
+_ctx_lock = threading.Lock()   # the per-device context table
 
 
 def lib():
@@ -172,7 +194,8 @@ class Context:
             device = torch.cuda.current_device()
         if isinstance(device, torch.device):
             device = device.index if device.index is not None else torch.cuda.current_device()
-        with _lock:
+        lib()  # load first: lib() takes _lock, which is not reentrant
+        with _ctx_lock:
             c = cls._ctxs.get(device)
             if c is None:
                 c = cls(device)
@@ -213,7 +236,7 @@ def side_stream(device=None):
     if isinstance(device, torch.device):
         device = device.index if device.index is not None else torch.cuda.current_device()
     lib()
-    with _lock:
+    with _ctx_lock:
         s = _side.get(device)
         if s is None:
             hip = ctypes.CDLL("libamdhip64.so.7")  # the runtime already loaded by torch

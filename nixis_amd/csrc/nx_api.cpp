@@ -203,6 +203,46 @@ int nx_rescale(nx_ctx *ctx, const double *x, int64_t n, double lower, double upp
     return rescale(ctx, x, n, lower, upper, out, minmax, (hipStream_t)stream);
 }
 
+int nx_rescale_ex(nx_ctx *ctx, const double *x, int64_t n, double lower, double upper,
+                  const nx_rescale_opts *opts, double *out, double *minmax, void *stream)
+{
+    return rescale_ex(ctx, x, n, lower, upper, opts, out, minmax, (hipStream_t)stream);
+}
+
+int nx_make_bool_elevation_mask(nx_ctx *ctx, const double *h, int64_t n, double level,
+                                uint8_t *mask, void *stream)
+{
+    return make_mask(ctx, h, n, level, mask, (hipStream_t)stream);
+}
+
+size_t nx_power_rescale_workspace_bytes(int64_t n) { return power_rescale_workspace_bytes(n); }
+
+int nx_power_rescale(nx_ctx *ctx, const double *x, const uint8_t *mask, int64_t n, int mode,
+                     double power, double *out, void *ws, void *stream)
+{
+    return power_rescale(ctx, x, mask, n, mode, power, out, ws, (hipStream_t)stream);
+}
+
+size_t nx_height_chain_workspace_bytes(int64_t n) { return height_chain_workspace_bytes(n); }
+
+int nx_terrain_heights(nx_ctx *ctx, const double *verts, int64_t V, const int32_t *perm,
+                       const int32_t *pgi, int n_octaves, const double *rough,
+                       const double *strength, double radius, double min_alt, double max_alt,
+                       double ocean_percent, double *h, uint8_t *mask, double *ocean_level,
+                       void *ws, void *stream)
+{
+    return terrain_heights(ctx, verts, V, perm, pgi, n_octaves, rough, strength, radius, min_alt,
+                           max_alt, ocean_percent, h, mask, ocean_level, ws, (hipStream_t)stream);
+}
+
+int nx_height_chain(nx_ctx *ctx, double *h, int64_t n, double min_alt, double max_alt,
+                    double ocean_percent, uint8_t *mask, double *ocean_level, void *ws,
+                    void *stream)
+{
+    return height_chain(ctx, h, n, min_alt, max_alt, ocean_percent, mask, ocean_level, ws,
+                        (hipStream_t)stream);
+}
+
 size_t nx_erosion_state_bytes(int64_t V) { return erosion_state_bytes(V); }
 
 int nx_erosion_load(nx_ctx *ctx, void *state, int64_t V, const double *h, const double *w,

@@ -85,6 +85,25 @@ int noise3d(nx_ctx *ctx, const double *x, const double *y, const double *z, int6
             const int32_t *perm, const int32_t *pgi, double *out, hipStream_t stream);
 int rescale(nx_ctx *ctx, const double *x, int64_t n, double lower, double upper, double *out,
             double *minmax, hipStream_t stream);
+int rescale_ex(nx_ctx *ctx, const double *x, int64_t n, double lower, double upper,
+               const nx_rescale_opts *o, double *out, double *minmax, hipStream_t stream);
+int make_mask(nx_ctx *ctx, const double *h, int64_t n, double level, uint8_t *mask,
+              hipStream_t stream);
+size_t power_rescale_workspace_bytes(int64_t n);
+int power_rescale(nx_ctx *ctx, const double *x, const uint8_t *mask, int64_t n, int mode,
+                  double power, double *out, void *ws, hipStream_t stream);
+int sample_octaves_keys(nx_ctx *ctx, const double *verts, int64_t V, const int32_t *perm,
+                        const int32_t *pgi, int n_oct, const double *rough,
+                        const double *strength, double radius, double *elev, int accumulate,
+                        float *dbg_noise, int32_t *dbg_sel, unsigned long long *keys,
+                        hipStream_t stream);
+int terrain_heights(nx_ctx *ctx, const double *verts, int64_t V, const int32_t *perm,
+                    const int32_t *pgi, int n_oct, const double *rough, const double *strength,
+                    double radius, double min_alt, double max_alt, double ocean_percent,
+                    double *h, uint8_t *mask, double *level, void *ws, hipStream_t stream);
+size_t height_chain_workspace_bytes(int64_t n);
+int height_chain(nx_ctx *ctx, double *h, int64_t n, double min_alt, double max_alt,
+                 double ocean_percent, uint8_t *mask, double *level, void *ws, hipStream_t stream);
 size_t erosion_state_bytes(int64_t V);
 int erosion_load(nx_ctx *ctx, void *state, int64_t V, const double *h, const double *w,
                  const double *s, hipStream_t stream);

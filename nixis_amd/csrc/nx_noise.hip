@@ -13,6 +13,7 @@
 
 #include "nx_ctx.h"
 #include "nx_noise_dev.h"
+#include "nx_reduce_dev.h"
 
 namespace nx {
 
@@ -25,13 +26,17 @@ struct OctaveArgs {
     int check_oct;                   // octave with the largest |rough|
 };
 
+// keys (nullable): order-preserving min/max keys of the elevations written,
+// reduced here so a rescale after noise needs no pass of its own.
 template <bool DBG>
 __global__ __launch_bounds__(256) void k_sample_octaves(OctaveArgs a, const double *__restrict__ verts,
                                                         int64_t V, double *__restrict__ elev,
                                                         float *__restrict__ dbg_noise,
                                                         int32_t *__restrict__ dbg_sel,
-                                                        int *__restrict__ err)
+                                                        int *__restrict__ err,
+                                                        unsigned long long *__restrict__ keys)
 {
+    unsigned long long lo = ~0ull, hi = 0ull;
     __shared__ NoiseTables T;
     load_tables(T, a.tables);
     __syncthreads();
@@ -56,8 +61,20 @@ __global__ __launch_bounds__(256) void k_sample_octaves(OctaveArgs a, const doub
             }
         }
         elev[i] = acc;
+        if (keys) {
+            const unsigned long long k = dkey(acc);
+            lo = k < lo ? k : lo;
+            hi = k > hi ? k : hi;
+        }
     }
     if (bad) __hip_atomic_fetch_or(err, NX_ERR_RANGE_BIT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (keys) {
+        block_minmax(lo, hi);
+        if (threadIdx.x == 0) {
+            __hip_atomic_fetch_min(&keys[0], lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_max(&keys[1], hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
 }
 
 __global__ __launch_bounds__(256) void k_noise3d(NoiseTablesPacked tp, const double *__restrict__ x,
@@ -140,6 +157,16 @@ int sample_octaves(nx_ctx *ctx, const double *verts, int64_t V, const int32_t *p
                    double radius, double *elev, int accumulate, float *dbg_noise,
                    int32_t *dbg_sel, hipStream_t stream)
 {
+    return sample_octaves_keys(ctx, verts, V, perm, pgi, n_oct, rough, strength, radius, elev,
+                               accumulate, dbg_noise, dbg_sel, nullptr, stream);
+}
+
+int sample_octaves_keys(nx_ctx *ctx, const double *verts, int64_t V, const int32_t *perm,
+                        const int32_t *pgi, int n_oct, const double *rough,
+                        const double *strength, double radius, double *elev, int accumulate,
+                        float *dbg_noise, int32_t *dbg_sel, unsigned long long *keys,
+                        hipStream_t stream)
+{
     if (!ctx) return set_error(NX_EINVAL, "null context");
     if (V < 0 || (V > 0 && (!verts || !elev))) return set_error(NX_EINVAL, "bad verts/elev");
     if (n_oct < 0 || n_oct > NX_MAX_OCTAVES)
@@ -158,10 +185,10 @@ int sample_octaves(nx_ctx *ctx, const double *verts, int64_t V, const int32_t *p
     const int grid = grid_for(ctx, V, 8);
     if (dbg_noise || dbg_sel)
         hipLaunchKernelGGL(k_sample_octaves<true>, dim3(grid), dim3(256), 0, stream, a, verts, V,
-                           elev, dbg_noise, dbg_sel, ctx->d_err);
+                           elev, dbg_noise, dbg_sel, ctx->d_err, keys);
     else
         hipLaunchKernelGGL(k_sample_octaves<false>, dim3(grid), dim3(256), 0, stream, a, verts, V,
-                           elev, nullptr, nullptr, ctx->d_err);
+                           elev, nullptr, nullptr, ctx->d_err, keys);
     NX_HIP(hipGetLastError());
     return NX_OK;
 }

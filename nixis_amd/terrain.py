@@ -103,3 +103,65 @@ def sample_octaves(verts, elevations, perm, pgi, n_octaves=1, n_init_roughness=1
     _lib.Context.get(dv.device).sync_check()
     elevations[...] = de.cpu().numpy()
     return elevations
+
+
+def make_bool_elevation_mask_device(height, mask_elevation, out=None, stream=None):
+    """terrain.py:61-72 on a float64 device tensor: uint8 [n], 1 where
+    height <= mask_elevation."""
+    import torch
+    if height.dtype != torch.float64 or not height.is_contiguous():
+        raise ValueError("height must be a contiguous float64 tensor")
+    if out is None:
+        out = torch.empty(height.numel(), dtype=torch.uint8, device=height.device)
+    ctx = _lib.Context.get(height.device)
+    _lib.check(_lib.lib().nx_make_bool_elevation_mask(
+        ctx.handle, _lib.ptr(height), height.numel(), float(mask_elevation), _lib.ptr(out),
+        _lib.stream_ptr(stream)))
+    return out
+
+
+def make_bool_elevation_mask(height, mask_elevation):
+    """terrain.py:61-72: a bool array, True where height <= mask_elevation."""
+    import torch
+    a = np.ascontiguousarray(height, dtype=np.float64).ravel()
+    if a.size == 0:
+        return np.zeros(0, dtype=np.bool_)
+    dev = torch.device("cuda", torch.cuda.current_device())
+    m = make_bool_elevation_mask_device(torch.from_numpy(a).to(dev), mask_elevation)
+    _lib.Context.get(dev).sync_check()
+    return m.cpu().numpy().astype(np.bool_)
+
+
+def terrain_heights_device(verts, perm, pgi, rough, strength, radius=1.0, min_alt=-4000.0,
+                           max_alt=8850.0, ocean_percent=55.0, stream=None, workspace=None,
+                           out=None):
+    """terrain.py:32-59 then nixis.py:330-364 in one call on device tensors:
+    the octave sum at the vertices (from zero), its min/max reduced inside the
+    noise kernel, then the height chain.  Returns (heights float64 [V], ocean
+    mask uint8 [V], ocean level float64 [1]), all on the device; equal to
+    sample_octaves_device followed by util.height_chain_device."""
+    import torch
+    from .opensimplex import _tables
+    perm, pgi = _tables(perm, pgi)
+    rough = np.ascontiguousarray(rough, dtype=np.float64)
+    strength = np.ascontiguousarray(strength, dtype=np.float64)
+    if rough.shape != strength.shape or rough.ndim != 1:
+        raise ValueError("rough and strength must be 1-D of equal length")
+    if verts.dtype != torch.float64 or verts.dim() != 2 or verts.shape[1] != 3:
+        raise ValueError("verts must be a float64 [V,3] tensor")
+    verts = verts.contiguous()
+    V = verts.shape[0]
+    L = _lib.lib()
+    need = L.nx_height_chain_workspace_bytes(V)
+    if workspace is None or workspace.numel() < need:
+        workspace = torch.empty(need, dtype=torch.uint8, device=verts.device)
+    h = out if out is not None else torch.empty(V, dtype=torch.float64, device=verts.device)
+    mask = torch.empty(V, dtype=torch.uint8, device=verts.device)
+    level = torch.empty(1, dtype=torch.float64, device=verts.device)
+    ctx = _lib.Context.get(verts.device)
+    _lib.check(L.nx_terrain_heights(
+        ctx.handle, _lib.ptr(verts), V, _lib.ptr(perm), _lib.ptr(pgi), rough.size,
+        _lib.ptr(rough), _lib.ptr(strength), float(radius), float(min_alt), float(max_alt),
+        float(ocean_percent), _lib.ptr(h), _lib.ptr(mask), _lib.ptr(level),
+        _lib.ptr(workspace), _lib.stream_ptr(stream)))
+    return h, mask, level

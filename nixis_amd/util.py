@@ -1,7 +1,10 @@
 """Drop-ins for the reference's util.py functions on the terrain path.
 
-* ``rescale`` (util.py:110-140, standard mode) runs on the GPU (device
-  min/max reduction + map, bit-identical FP64 arithmetic).
+* ``rescale`` (util.py:110-175, every mode) and ``power_rescale``
+  (util.py:178-254) run on the GPU (device min/max reductions, a min-scan
+  for power_rescale's sequential bounds, FP64 maps in the reference's
+  operation order); ``height_chain_device`` runs nixis.py:330-364 after
+  noise without leaving the device.
 * ``create_mesh`` (util.py:17-50) returns this package's own icosphere
   (meshzoo is not available): same counts V = 10d^2+2, T = 20d^2, the 12
   degree-5 vertices first, unit radius.
@@ -15,38 +18,166 @@ import numpy as np
 from . import _lib
 
 
-def rescale_device(x, lower, upper, out=None, stream=None):
-    """Standard-mode rescale of a float64 device tensor; returns (out, minmax)."""
+_MODES = {None: 0, "lower": 1, "upper": 2}
+
+
+def _rescale_opts(mid, mode, u_min, u_max):
+    if mid is None and mode is None and u_min is None and u_max is None:
+        return None
+    o = _lib.NxRescaleOpts()
+    o.has_mid = mid is not None
+    o.mid = float(mid) if mid is not None else 0.0
+    o.mode = _MODES[mode]
+    o.has_umin = u_min is not None
+    o.u_min = float(u_min) if u_min is not None else 0.0
+    o.has_umax = u_max is not None
+    o.u_max = float(u_max) if u_max is not None else 0.0
+    return o
+
+
+def _f64_device(x, name="x"):
     import torch
-    if x.dtype != torch.float64 or not x.is_contiguous():
-        raise ValueError("x must be a contiguous float64 tensor")
+    if x.dtype != torch.float64 or not x.is_contiguous() or x.device.type != "cuda":
+        raise ValueError(f"{name} must be a contiguous float64 device tensor")
+
+
+def rescale_device(x, lower, upper, out=None, stream=None, mid=None, mode=None, u_min=None,
+                   u_max=None):
+    """util.py:110-175 on a float64 device tensor; returns (out, minmax).
+
+    minmax holds the input's (min, max) before u_min / u_max.  out may be x.
+    mode is None, 'lower' or 'upper' and needs mid."""
+    import torch
+    _f64_device(x)
+    if mode not in _MODES:
+        raise ValueError(f"rescale mode must be None, 'lower' or 'upper', not {mode!r}")
+    if mode is not None and mid is None:
+        raise ValueError("rescale modes need a mid value")
     if out is None:
         out = torch.empty_like(x)
     minmax = torch.empty(2, dtype=torch.float64, device=x.device)
     ctx = _lib.Context.get(x.device)
-    _lib.check(_lib.lib().nx_rescale(ctx.handle, _lib.ptr(x), x.numel(), float(lower),
-                                     float(upper), _lib.ptr(out), _lib.ptr(minmax),
-                                     _lib.stream_ptr(stream)))
+    opts = _rescale_opts(mid, mode, u_min, u_max)
+    if opts is None:
+        rc = _lib.lib().nx_rescale(ctx.handle, _lib.ptr(x), x.numel(), float(lower),
+                                   float(upper), _lib.ptr(out), _lib.ptr(minmax),
+                                   _lib.stream_ptr(stream))
+    else:
+        rc = _lib.lib().nx_rescale_ex(ctx.handle, _lib.ptr(x), x.numel(), float(lower),
+                                      float(upper), ctypes.byref(opts), _lib.ptr(out),
+                                      _lib.ptr(minmax), _lib.stream_ptr(stream))
+    _lib.check(rc)
     return out, minmax
 
 
-def rescale(x, lower, upper, mid=None, mode=None, u_min=None, u_max=None):
-    """util.py:110-140 (standard mode: mid, mode, u_min, u_max all None).
-
-    Raises ZeroDivisionError when x is constant, as numba's error model does.
-    The split-range and one-sided modes are not on the terrain path.
-    """
-    if mid is not None or mode is not None or u_min is not None or u_max is not None:
-        raise NotImplementedError("only the standard rescale mode is provided")
+def _host_to_device(x):
     import torch
     a = np.ascontiguousarray(x, dtype=np.float64)
     if a.size == 0:
         raise ValueError("zero-size array to reduction operation minimum")
     dev = torch.device("cuda", torch.cuda.current_device())
-    t = torch.from_numpy(a.ravel()).to(dev)
-    out, _ = rescale_device(t, lower, upper, out=t)
+    return a, dev, torch.from_numpy(a.ravel()).to(dev)
+
+
+def rescale(x, lower, upper, mid=None, mode=None, u_min=None, u_max=None):
+    """util.py:110-175: every form of the reference's rescale, on the GPU.
+
+    As the reference: a mode without mid prints an error and returns x
+    itself; a mode other than 'lower' / 'upper' (with mid) returns None; a
+    zero range that a value divides by raises ZeroDivisionError (numba's
+    error model)."""
+    if mode is not None and mid is None:
+        print("ERROR: Must supply a middle value to use rescale modes.")
+        print("Continuing with unmodified data.")
+        return x
+    if mode is not None and mode not in ("lower", "upper"):
+        return None
+    a, dev, t = _host_to_device(x)
+    out, _ = rescale_device(t, lower, upper, out=t, mid=mid, mode=mode, u_min=u_min, u_max=u_max)
     _lib.Context.get(dev).sync_check()
     return out.cpu().numpy().reshape(a.shape)
+
+
+def power_rescale_device(x, mask, mode, power, out=None, stream=None, workspace=None):
+    """util.py:178-254 on device tensors: x float64 [n], mask bool/uint8 [n].
+
+    mode 1 rescales the masked values, mode 0 the others.  out may be x."""
+    import torch
+    _f64_device(x)
+    if mask.dtype not in (torch.bool, torch.uint8) or not mask.is_contiguous() \
+            or mask.numel() != x.numel() or mask.device != x.device:
+        raise ValueError("mask must be a contiguous bool/uint8 tensor of x's length on x's device")
+    if out is None:
+        out = torch.empty_like(x)
+    L = _lib.lib()
+    n = x.numel()
+    need = L.nx_power_rescale_workspace_bytes(n)
+    if workspace is None or workspace.numel() < need:
+        workspace = torch.empty(need, dtype=torch.uint8, device=x.device)
+    ctx = _lib.Context.get(x.device)
+    _lib.check(L.nx_power_rescale(ctx.handle, _lib.ptr(x), _lib.ptr(mask.view(torch.uint8)), n,
+                                  int(mode), float(power), _lib.ptr(out), _lib.ptr(workspace),
+                                  _lib.stream_ptr(stream)))
+    return out
+
+
+def power_rescale(x, mask=None, mode=None, power=1.0):
+    """util.py:178-254 on the GPU; returns a new array.  Mode None (or any
+    value other than 0 and 1) selects nothing and returns a copy, as the
+    reference's loops do."""
+    import torch
+    if mode not in (0, 1):
+        return np.array(x, dtype=np.float64, copy=True)
+    a, dev, t = _host_to_device(x)
+    m = np.ascontiguousarray(mask).astype(np.uint8, copy=False).ravel()
+    if m.size != a.size:
+        raise ValueError("mask and x differ in length")
+    md = torch.from_numpy(m).to(dev)
+    out = power_rescale_device(t, md, mode, power, out=t)
+    _lib.Context.get(dev).sync_check()
+    return out.cpu().numpy().reshape(a.shape)
+
+
+def find_percent_val(minval, maxval, percent):
+    """util.py:556-566: the value `percent` percent of the way from minval to
+    maxval; a percent outside (0, 100) is taken as 50 after an error line."""
+    if not 0.0 < percent < 100.0:
+        print("\n" + "    ERROR: Percent must be between 0 and 100.")
+        print("    Defaulting to 50 percent." + "\n")
+        percent = 50.0
+    return minval + ((maxval - minval) * percent / 100.0)
+
+
+def find_val_percent(minval, maxval, x):
+    """util.py:568-577: where x lies between minval and maxval, in percent
+    (an x outside the range prints an error line and is still computed)."""
+    if not minval < x < maxval:
+        print("\n" + "    ERROR: X must be between minval and maxval.")
+    return (x - minval) / (maxval - minval) * 100
+
+
+def height_chain_device(h, min_alt, max_alt, ocean_percent, stream=None, workspace=None):
+    """nixis.py:330-364 in place on the float64 device tensor h (the noise
+    heights): rescale to [min_alt, max_alt], ocean level at ocean_percent,
+    the ocean mask, power_rescale of the ocean floor (0.5) and of the land
+    (2.0), the shift to sea level 0 and the split rescale around 0.
+
+    Returns (h, mask uint8 [n], ocean_level float64 [1]), all on the device;
+    nothing is copied to the host."""
+    import torch
+    _f64_device(h, "h")
+    L = _lib.lib()
+    n = h.numel()
+    need = L.nx_height_chain_workspace_bytes(n)
+    if workspace is None or workspace.numel() < need:
+        workspace = torch.empty(need, dtype=torch.uint8, device=h.device)
+    mask = torch.empty(n, dtype=torch.uint8, device=h.device)
+    level = torch.empty(1, dtype=torch.float64, device=h.device)
+    ctx = _lib.Context.get(h.device)
+    _lib.check(L.nx_height_chain(ctx.handle, _lib.ptr(h), n, float(min_alt), float(max_alt),
+                                 float(ocean_percent), _lib.ptr(mask), _lib.ptr(level),
+                                 _lib.ptr(workspace), _lib.stream_ptr(stream)))
+    return h, mask, level
 
 
 def icosphere_counts(divisions):

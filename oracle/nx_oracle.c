@@ -898,3 +898,134 @@ void ora_sort_adjacency(int32_t *adj, int64_t V)
     }
     free(orig);
 }
+
+/* util.py:110-175 (rescale, every form).  mode: 0 none, 1 'lower',
+ * 2 'upper'.  Returns 0; -1 when an executed division has a zero divisor
+ * (numba raises ZeroDivisionError); 1 when a mode is given without mid (the
+ * reference returns x itself; out is then a copy of x). */
+int ora_rescale_ex(const double *x, int64_t n, double lower, double upper, int has_mid,
+                   double mid, int mode, int has_umin, double u_min, int has_umax,
+                   double u_max, double *out)
+{
+    if (n <= 0) return 0;
+    memcpy(out, x, sizeof(double) * (size_t)n);          /* new_array = np.copy(x) */
+    double x_min = x[0], x_max = x[0];
+    for (int64_t i = 1; i < n; ++i) {
+        if (x[i] < x_min) x_min = x[i];
+        if (x[i] > x_max) x_max = x[i];
+    }
+    if (has_umin && u_min < x_min) x_min = u_min;       /* 131-134 */
+    if (has_umax && u_max > x_max) x_max = u_max;
+    int zero = 0;
+    if (mode == 0) {
+        if (!has_mid) {                                  /* 137-141 */
+            const double x_range = x_max - x_min, new_range = upper - lower;
+            for (int64_t a = 0; a < n; ++a) {
+                if (x_range == 0) zero = 1;
+                out[a] = ((x[a] - x_min) / x_range) * new_range + lower;
+            }
+        } else {                                         /* 143-153 */
+            const double x_lower_range = mid - x_min, new_lower_range = mid - lower;
+            const double x_upper_range = x_max - mid, new_upper_range = upper - mid;
+            for (int64_t a = 0; a < n; ++a) {
+                if (x[a] <= mid) {
+                    if (x_lower_range == 0) zero = 1;
+                    out[a] = ((x[a] - x_min) / x_lower_range) * new_lower_range + lower;
+                } else {
+                    if (x_upper_range == 0) zero = 1;
+                    out[a] = ((x[a] - mid) / x_upper_range) * new_upper_range + mid;
+                }
+            }
+        }
+        return zero ? -1 : 0;
+    }
+    if (!has_mid) return 1;                              /* 156-161 */
+    if (mode == 1) {                                     /* 162-167 */
+        const double x_range = mid - x_min, new_range = mid - lower;
+        for (int64_t a = 0; a < n; ++a)
+            if (x[a] <= mid) {
+                if (x_range == 0) zero = 1;
+                out[a] = ((x[a] - x_min) / x_range) * new_range + lower;
+            }
+    } else if (mode == 2) {                              /* 169-174 */
+        const double x_range = x_max - mid, new_range = upper - mid;
+        for (int64_t a = 0; a < n; ++a)
+            if (x[a] >= mid) {
+                if (x_range == 0) zero = 1;
+                out[a] = ((x[a] - mid) / x_range) * new_range + mid;
+            }
+    }
+    return zero ? -1 : 0;
+}
+
+/* terrain.py:61-72 */
+void ora_make_bool_elevation_mask(const double *h, int64_t n, double level, uint8_t *mask)
+{
+    for (int64_t i = 0; i < n; ++i) mask[i] = h[i] <= level ? 1 : 0;
+}
+
+/* util.py:178-254 (power_rescale), sequentially as the reference's range
+ * loops; mode 1 rescales the masked values, mode 0 the unmasked ones.
+ * Returns -1 when a rescaled value divides by a zero mask range. */
+int ora_power_rescale(const double *x, const uint8_t *mask, int64_t n, int mode, double power,
+                      double *out)
+{
+    if (n <= 0) return 0;
+    memcpy(out, x, sizeof(double) * (size_t)n);
+    const double temp_upper = 1.0, temp_lower = 0.0;
+    double x_min = x[0], x_max = x[0];
+    for (int64_t i = 1; i < n; ++i) {
+        if (x[i] < x_min) x_min = x[i];
+        if (x[i] > x_max) x_max = x[i];
+    }
+    double mask_lower = x_max, mask_upper = x_min;
+    const double temp_range = temp_upper - temp_lower;
+    for (int64_t a = 0; a < n; ++a) {
+        const int sel = (mode == 1 && mask[a]) || (mode == 0 && !mask[a]);
+        if (sel && x[a] < mask_lower)
+            mask_lower = x[a];
+        else if (sel && x[a] > mask_upper)
+            mask_upper = x[a];
+    }
+    const double mask_range = mask_upper - mask_lower;
+    int zero = 0;
+    for (int64_t a = 0; a < n; ++a) {
+        const int sel = (mode == 1 && mask[a]) || (mode == 0 && !mask[a]);
+        if (!sel) continue;
+        if (mask_range == 0) zero = 1;
+        double t = ((x[a] - mask_lower) / mask_range) * temp_range + temp_lower;
+        t = pow(t, power);
+        out[a] = ((t - temp_lower) / temp_range) * mask_range + mask_lower;
+    }
+    return zero ? -1 : 0;
+}
+
+/* util.py:556-566 */
+double ora_find_percent_val(double minval, double maxval, double percent)
+{
+    if (!(0.0 < percent && percent < 100.0)) percent = 50.0;
+    return minval + ((maxval - minval) * percent / 100.0);
+}
+
+/* nixis.py:330-364: the height chain after noise.  h is replaced by the
+ * final heights, mask receives the ocean mask; returns the ocean level. */
+double ora_height_chain(double *h, int64_t n, double min_alt, double max_alt,
+                        double ocean_percent, uint8_t *mask, int *rc)
+{
+    double *t = (double *)malloc(sizeof(double) * (size_t)n);
+    int r = ora_rescale_ex(h, n, min_alt, max_alt, 0, 0.0, 0, 0, 0.0, 0, 0.0, t);
+    double minval = t[0], maxval = t[0];
+    for (int64_t i = 1; i < n; ++i) {
+        if (t[i] < minval) minval = t[i];
+        if (t[i] > maxval) maxval = t[i];
+    }
+    const double ocean_level = ora_find_percent_val(minval, maxval, ocean_percent);
+    ora_make_bool_elevation_mask(t, n, ocean_level, mask);
+    int r2 = ora_power_rescale(t, mask, n, 1, 0.5, h);
+    int r3 = ora_power_rescale(h, mask, n, 0, 2.0, t);
+    for (int64_t i = 0; i < n; ++i) t[i] -= ocean_level;
+    int r4 = ora_rescale_ex(t, n, min_alt, max_alt, 1, 0.0, 0, 0, 0.0, 0, 0.0, h);
+    free(t);
+    *rc = (r || r2 || r3 || r4) ? -1 : 0;
+    return ocean_level;
+}

@@ -91,6 +91,22 @@ void ora_sort_adjacency(int32_t *adj, int64_t V);
 /* Number of OpenMP threads the oracle will use. */
 int ora_num_threads(void);
 
+/* util.py:110-175 (rescale, every form; mode 0 none, 1 'lower', 2 'upper').
+ * Returns 0, -1 for a zero divisor, 1 for a mode without mid. */
+int ora_rescale_ex(const double *x, int64_t n, double lower, double upper, int has_mid,
+                   double mid, int mode, int has_umin, double u_min, int has_umax,
+                   double u_max, double *out);
+/* terrain.py:61-72 */
+void ora_make_bool_elevation_mask(const double *h, int64_t n, double level, uint8_t *mask);
+/* util.py:178-254 */
+int ora_power_rescale(const double *x, const uint8_t *mask, int64_t n, int mode, double power,
+                      double *out);
+/* util.py:556-566 */
+double ora_find_percent_val(double minval, double maxval, double percent);
+/* nixis.py:330-364 */
+double ora_height_chain(double *h, int64_t n, double min_alt, double max_alt,
+                        double ocean_percent, uint8_t *mask, int *rc);
+
 #ifdef __cplusplus
 }
 #endif

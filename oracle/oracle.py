@@ -54,6 +54,15 @@ def lib():
         L.ora_build_adjacency.argtypes = [_p, _i64, _i64, _p]
         L.ora_sort_adjacency.argtypes = [_p, _i64]
         L.ora_num_threads.restype = _i32
+        L.ora_rescale_ex.argtypes = [_p, _i64, _d, _d, _i32, _d, _i32, _i32, _d, _i32, _d, _p]
+        L.ora_rescale_ex.restype = _i32
+        L.ora_make_bool_elevation_mask.argtypes = [_p, _i64, _d, _p]
+        L.ora_power_rescale.argtypes = [_p, _p, _i64, _i32, _d, _p]
+        L.ora_power_rescale.restype = _i32
+        L.ora_find_percent_val.argtypes = [_d, _d, _d]
+        L.ora_find_percent_val.restype = _d
+        L.ora_height_chain.argtypes = [_p, _i64, _d, _d, _d, _p, ctypes.POINTER(_i32)]
+        L.ora_height_chain.restype = _d
         _lib = L
     return _lib
 
@@ -214,3 +223,60 @@ def erosion_iteration4(verts, neighbors, r_buff, wat, sed):
     verts, nb = _f64(verts), _i32a(neighbors)
     lib().ora_erosion_iteration4(_ptr(verts), _ptr(nb), nb.shape[0],
                                  _ptr(r_buff), _ptr(wat), _ptr(sed))
+
+
+# ---- post-processing after noise (util.py, terrain.py, nixis.py) -----------
+
+_MODES = {None: 0, "lower": 1, "upper": 2}
+
+
+def rescale_ex(x, lower, upper, mid=None, mode=None, u_min=None, u_max=None):
+    """util.py:110-175 in every form.  Returns the new array, or x itself
+    when a mode is given without mid; ZeroDivisionError on a zero divisor."""
+    x = _f64(x)
+    out = np.empty_like(x)
+    rc = lib().ora_rescale_ex(_ptr(x), x.size, float(lower), float(upper), mid is not None,
+                              float(mid or 0.0), _MODES[mode], u_min is not None,
+                              float(u_min or 0.0), u_max is not None, float(u_max or 0.0),
+                              _ptr(out))
+    if rc == -1:
+        raise ZeroDivisionError("rescale: zero range")
+    if rc == 1:
+        return x
+    return out
+
+
+def make_bool_elevation_mask(height, level):
+    """terrain.py:61-72"""
+    h = _f64(height)
+    m = np.zeros(h.size, np.uint8)
+    lib().ora_make_bool_elevation_mask(_ptr(h), h.size, float(level), _ptr(m))
+    return m.astype(bool)
+
+
+def power_rescale(x, mask, mode, power):
+    """util.py:178-254"""
+    x = _f64(x)
+    m = np.ascontiguousarray(mask, dtype=np.uint8)
+    out = np.empty_like(x)
+    rc = lib().ora_power_rescale(_ptr(x), _ptr(m), x.size, int(mode), float(power), _ptr(out))
+    if rc == -1:
+        raise ZeroDivisionError("power_rescale: zero mask range")
+    return out
+
+
+def find_percent_val(minval, maxval, percent):
+    """util.py:556-566"""
+    return lib().ora_find_percent_val(float(minval), float(maxval), float(percent))
+
+
+def height_chain(height, min_alt=-4000.0, max_alt=8850.0, ocean_percent=55.0):
+    """nixis.py:330-364 -> (heights, ocean mask, ocean level)."""
+    h = _f64(height).copy()
+    m = np.zeros(h.size, np.uint8)
+    rc = _i32(0)
+    level = lib().ora_height_chain(_ptr(h), h.size, float(min_alt), float(max_alt),
+                                   float(ocean_percent), _ptr(m), ctypes.byref(rc))
+    if rc.value:
+        raise ZeroDivisionError("height chain: zero range")
+    return h, m.astype(bool), level

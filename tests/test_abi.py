@@ -99,3 +99,23 @@ def test_bad_arguments_are_rejected_without_a_gpu():
         _lib.check(L.nx_icosphere_counts(0, ctypes.byref(V), ctypes.byref(T)))
     with pytest.raises(ValueError):
         util.sort_adjacency(np.zeros((4, 5), np.int32))
+
+
+def test_first_context_does_not_deadlock_the_loader():
+    """Creating the first context before anything else has loaded the
+    library must return (here: with the no-device error) rather than wait on
+    the loader's own lock (with a device it returns the context).  Run in a
+    fresh process so the library is not yet loaded."""
+    import subprocess
+    import sys
+    code = ("from nixis_amd import _lib\n"
+            "assert _lib._lib is None\n"
+            "try:\n"
+            "    _lib.Context.get(0)\n"
+            "except _lib.NixisError:\n"
+            "    pass\n"
+            "print('returned')\n")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0 and "returned" in r.stdout, r.stderr[-2000:]
