@@ -268,7 +268,8 @@ def bench_post_chain(args, nz):
             "fused_noise_chain_ms": round(fused_ms, 4), "noise_ms": round(nz["ms"], 4),
             "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
-                         "bytes_per_vertex": CHAIN_BYTES_PER_VERTEX}}
+                         "bytes_per_vertex": CHAIN_BYTES_PER_VERTEX,
+                         "traffic": profiled_traffic("chain", n)}}
 
 
 def cpu_erosion_baseline(er, iters=4):
