@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--erosion-d", type=int, default=1000)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-export", action="store_true")
+    ap.add_argument("--export-d", type=int, default=2500)
     ap.add_argument("--served-jobs", type=int, default=120)
     ap.add_argument("--served-load", type=float, default=0.8,
                     help="arrival rate as a fraction of the plain form's capacity")
@@ -272,6 +274,43 @@ def bench_post_chain(args, nz):
                          "traffic": profiled_traffic("chain", n)}}
 
 
+def bench_export(args):
+    """nixis.py:270-302 and 516-540 on the device: the pixel query of a
+    d-division planet (grid build = the reference's KD-tree build, then the
+    3 nearest vertices of every pixel of make_ll_arr) and the gray sampling
+    of a height layer, for the reference's default 2048 x 1024 image and a
+    16384 x 8192 one.  The mesh is generated on the device."""
+    import torch
+    from nixis_amd import _lib, util
+    d = args.export_d
+    pts, _ = util.create_mesh_device(d, cells=False)
+    V = pts.shape[0]
+    colors = torch.rand(V, dtype=torch.float64, device="cuda") * 255.0
+    util.KnnGrid(pts)                                       # warm-up build
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    grid = util.KnnGrid(pts)
+    torch.cuda.synchronize()
+    build_ms = (time.perf_counter() - t) * 1e3
+    out = {"workload": f"d={d} icosphere ({V} vertices) made on the device; grid build, then "
+                       "make_ll_arr + 3-nearest query and gray sampling per image",
+           "grid_build_ms": round(build_ms, 2), "images": {}}
+    for (w, h) in ((2048, 1024), (16384, 8192)):
+        grid.query_pixels_device(w, h, 1.0, 3)              # warm-up
+        dd, nn = grid.query_pixels_device(w, h, 1.0, 3)
+        q_ms = _events_ms(lambda: grid.query_pixels_device(w, h, 1.0, 3))
+        util.make_gray_array_device(dd, nn, colors)
+        g_ms = _events_ms(lambda: util.make_gray_array_device(dd, nn, colors))
+        _lib.Context.get(pts.device).sync_check()
+        npix = w * h
+        out["images"][f"{w}x{h}"] = {"query_ms": round(q_ms, 3), "gray_ms": round(g_ms, 3),
+                                      "Mpixel_per_s_query": round(npix / (q_ms * 1e-3) / 1e6, 1)}
+        log(f"[export] d={d} V={V}: {w}x{h} query {q_ms:.2f} ms, gray {g_ms:.2f} ms")
+        del dd, nn
+    log(f"[export] grid build {build_ms:.1f} ms")
+    return out
+
+
 def cpu_erosion_baseline(er, iters=4):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -459,6 +498,8 @@ def main():
         line["batched"] = bench_batched(args)
     if not args.no_served:
         line["served"] = bench_served(args)
+    if not args.no_export:
+        line["export"] = bench_export(args)
     print(json.dumps(line), flush=True)
 
 

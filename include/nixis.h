@@ -173,6 +173,48 @@ int nx_terrain_heights(nx_ctx *ctx, const double *verts, int64_t V,
                        double ocean_percent, double *h, uint8_t *mask,
                        double *ocean_level, void *ws, void *stream);
 
+/* ---- equirectangular export (device) ----------------------------------- */
+
+/* util.py:275-287 (build_KDTree) and nixis.py:282 (KDTree.query(ll, k=3)):
+ * the k nearest points of each query point, nearest first, as scipy's
+ * KDTree returns them: dists = sqrt of the squared differences summed in
+ * x, y, z order (f64), nbrs = point indices (int64); equal distances are
+ * ordered by index.  Where fewer than k points exist the rest is +inf / V.
+ * The structure is a uniform grid of cells in a device workspace of
+ * nx_knn_grid_bytes(V) bytes (256-byte aligned): points sorted by cell and
+ * a hash table of occupied cells.  1 <= k <= 8, V < 2^31 - 1. */
+size_t nx_knn_grid_bytes(int64_t V);
+int nx_knn_grid_build(nx_ctx *ctx, const double *points, int64_t V,
+                      void *grid, void *stream);
+int nx_knn_query(nx_ctx *ctx, const void *grid, int64_t V, const double *q,
+                 int64_t nq, int k, double *dists, int64_t *nbrs, void *stream);
+
+/* util.py:291-310 (make_ll_arr) and the query of its pixels in one pass:
+ * pixel (h, w) of a width x height map sits at latitude
+ * max(90 - h * 180 / (height - 1), -90) and longitude
+ * max(-180 + w * 360 / width, -180), at (r cos lat cos lon,
+ * r cos lat sin lon, r sin lat) (util.py:80-90); the cosines and sines of
+ * each row and column are computed on the host with the C library's cos /
+ * sin, so the coordinates equal the reference's.  dists / nbrs are
+ * [height][width][k].  ws: nx_pixel_workspace_bytes(width, height). */
+size_t nx_pixel_workspace_bytes(int64_t width, int64_t height);
+int nx_knn_query_pixels(nx_ctx *ctx, const void *grid, int64_t V,
+                        int64_t width, int64_t height, double radius, int k,
+                        double *dists, int64_t *nbrs, void *ws, void *stream);
+/* util.py:291-310: out is device f64 [height][width][3]. */
+int nx_make_ll_arr(nx_ctx *ctx, int64_t width, int64_t height, double radius,
+                   double *out, void *ws, void *stream);
+
+/* util.py:343-366 (make_gray_array, and make_rgb_array's value,
+ * util.py:313-340): for each of npix pixels with k neighbours,
+ * sd = sum(d), w_i = 1 / ((d_i + 0.00001) / sd), t = sum(w),
+ * out = (int32)(sum colors[n_i] * (w_i / t)), sums in neighbour order,
+ * truncated toward zero.  colors: device f64 [ncolors]; a neighbour index
+ * outside it is NX_EINVAL at the next sync. */
+int nx_make_gray_array(nx_ctx *ctx, int64_t npix, int k, const double *dists,
+                       const int64_t *nbrs, const double *colors,
+                       int64_t ncolors, int32_t *out, void *stream);
+
 /* ---- erosion (device) -------------------------------------------------- */
 
 /* Fields live in a caller-allocated device workspace of

@@ -97,6 +97,17 @@ SIGNATURES = {
                                     _c_void_p]),
     "nx_height_chain": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_double, _c_double, _c_double,
                                  _c_void_p, _c_void_p, _c_void_p, _c_void_p]),
+    "nx_knn_grid_bytes": (_c_size, [_c_i64]),
+    "nx_knn_grid_build": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p, _c_void_p]),
+    "nx_knn_query": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p, _c_i64, _c_int,
+                              _c_void_p, _c_void_p, _c_void_p]),
+    "nx_pixel_workspace_bytes": (_c_size, [_c_i64, _c_i64]),
+    "nx_knn_query_pixels": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_i64, _c_i64, _c_double,
+                                     _c_int, _c_void_p, _c_void_p, _c_void_p, _c_void_p]),
+    "nx_make_ll_arr": (_c_int, [_c_void_p, _c_i64, _c_i64, _c_double, _c_void_p, _c_void_p,
+                                _c_void_p]),
+    "nx_make_gray_array": (_c_int, [_c_void_p, _c_i64, _c_int, _c_void_p, _c_void_p, _c_void_p,
+                                    _c_i64, _c_void_p, _c_void_p]),
     "nx_erosion_state_bytes": (_c_size, [_c_i64]),
     "nx_erosion_load": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p, _c_void_p,
                                  _c_void_p, _c_void_p]),

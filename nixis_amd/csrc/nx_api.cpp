@@ -174,6 +174,8 @@ int nx_ctx_sync_check(nx_ctx *ctx, void *stream)
                          "noise coordinate outside the exact domain (|skewed coordinate| >= 2^30 "
                          "or not finite)");
     if (word & NX_ERR_DIV0_BIT) return set_error(NX_EDIV0, "rescale: input range is zero");
+    if (word & NX_ERR_INDEX_BIT)
+        return set_error(NX_EINVAL, "gray array: a neighbour index outside the colors array");
     if (word & NX_ERR_MESH_BIT)
         return set_error(NX_EINVAL, "mesh: a triangle index outside 0..V-1, a vertex in more than "
                                     "six triangles, or a neighbour ring that does not close");
@@ -224,6 +226,45 @@ int nx_power_rescale(nx_ctx *ctx, const double *x, const uint8_t *mask, int64_t 
 }
 
 size_t nx_height_chain_workspace_bytes(int64_t n) { return height_chain_workspace_bytes(n); }
+
+size_t nx_knn_grid_bytes(int64_t V) { return knn_grid_bytes(V); }
+
+int nx_knn_grid_build(nx_ctx *ctx, const double *points, int64_t V, void *grid, void *stream)
+{
+    return knn_grid_build(ctx, points, V, grid, (hipStream_t)stream);
+}
+
+int nx_knn_query(nx_ctx *ctx, const void *grid, int64_t V, const double *q, int64_t nq, int k,
+                 double *dists, int64_t *nbrs, void *stream)
+{
+    return knn_query(ctx, grid, V, q, nq, k, dists, nbrs, (hipStream_t)stream);
+}
+
+size_t nx_pixel_workspace_bytes(int64_t width, int64_t height)
+{
+    return pixel_workspace_bytes(width, height);
+}
+
+int nx_knn_query_pixels(nx_ctx *ctx, const void *grid, int64_t V, int64_t width, int64_t height,
+                        double radius, int k, double *dists, int64_t *nbrs, void *ws,
+                        void *stream)
+{
+    return knn_query_pixels(ctx, grid, V, width, height, radius, k, dists, nbrs, ws,
+                            (hipStream_t)stream);
+}
+
+int nx_make_ll_arr(nx_ctx *ctx, int64_t width, int64_t height, double radius, double *out,
+                   void *ws, void *stream)
+{
+    return make_ll_arr(ctx, width, height, radius, out, ws, (hipStream_t)stream);
+}
+
+int nx_make_gray_array(nx_ctx *ctx, int64_t npix, int k, const double *dists,
+                       const int64_t *nbrs, const double *colors, int64_t ncolors, int32_t *out,
+                       void *stream)
+{
+    return make_gray_array(ctx, npix, k, dists, nbrs, colors, ncolors, out, (hipStream_t)stream);
+}
 
 int nx_terrain_heights(nx_ctx *ctx, const double *verts, int64_t V, const int32_t *perm,
                        const int32_t *pgi, int n_octaves, const double *rough,

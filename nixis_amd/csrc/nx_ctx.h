@@ -14,6 +14,7 @@
 #define NX_ERR_RANGE_BIT 1
 #define NX_ERR_DIV0_BIT 2
 #define NX_ERR_MESH_BIT 4  // device mesh builders: bad index, degree > 6 or broken ring
+#define NX_ERR_INDEX_BIT 8  // gray sampling: a neighbour index outside the colors
 
 constexpr int kKeySlots = 256;
 constexpr size_t kMaxGraphs = 64;
@@ -102,6 +103,18 @@ int terrain_heights(nx_ctx *ctx, const double *verts, int64_t V, const int32_t *
                     double radius, double min_alt, double max_alt, double ocean_percent,
                     double *h, uint8_t *mask, double *level, void *ws, hipStream_t stream);
 size_t height_chain_workspace_bytes(int64_t n);
+size_t knn_grid_bytes(int64_t V);
+int knn_grid_build(nx_ctx *ctx, const double *pts, int64_t V, void *grid, hipStream_t stream);
+int knn_query(nx_ctx *ctx, const void *grid, int64_t V, const double *q, int64_t nq, int k,
+              double *dists, int64_t *nbrs, hipStream_t stream);
+size_t pixel_workspace_bytes(int64_t width, int64_t height);
+int knn_query_pixels(nx_ctx *ctx, const void *grid, int64_t V, int64_t width, int64_t height,
+                     double radius, int k, double *dists, int64_t *nbrs, void *ws,
+                     hipStream_t stream);
+int make_ll_arr(nx_ctx *ctx, int64_t width, int64_t height, double radius, double *out, void *ws,
+                hipStream_t stream);
+int make_gray_array(nx_ctx *ctx, int64_t npix, int k, const double *dists, const int64_t *nbrs,
+                    const double *colors, int64_t ncolors, int32_t *out, hipStream_t stream);
 int height_chain(nx_ctx *ctx, double *h, int64_t n, double min_alt, double max_alt,
                  double ocean_percent, uint8_t *mask, double *level, void *ws, hipStream_t stream);
 size_t erosion_state_bytes(int64_t V);

@@ -304,3 +304,177 @@ def sfc_order_device(points, stream=None):
     _lib.check(_lib.lib().nx_sfc_order_device(ctx.handle, _lib.ptr(points), V, _lib.ptr(order),
                                               _lib.ptr(ws), _lib.stream_ptr(stream)))
     return order
+
+
+# ---- equirectangular export (util.py:275-455, nixis.py:270-302) ----------------
+
+class KnnGrid:
+    """The k-nearest-neighbour structure that replaces the reference's scipy
+    KDTree (util.py:275-287): a uniform grid of cells on the device.
+    `query` returns what KDTree.query returns (nearest first, float64
+    distances, int64 indices; with an int k of 1 the last axis is dropped);
+    equal distances are ordered by index."""
+
+    def __init__(self, points, device=None):
+        import torch
+        if isinstance(points, torch.Tensor):
+            dev = points.device if device is None else torch.device(device)
+            p = points.to(dev, torch.float64).reshape(-1, 3).contiguous()
+        else:
+            dev = torch.device("cuda", torch.cuda.current_device()) if device is None \
+                else torch.device(device)
+            p = torch.from_numpy(np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)).to(dev)
+        self.device = dev
+        self.points = p
+        self.V = p.shape[0]
+        L = _lib.lib()
+        self.ws = torch.empty(L.nx_knn_grid_bytes(self.V), dtype=torch.uint8, device=dev)
+        self.ctx = _lib.Context.get(dev)
+        _lib.check(L.nx_knn_grid_build(self.ctx.handle, _lib.ptr(p), self.V, _lib.ptr(self.ws),
+                                       _lib.stream_ptr()))
+
+    def query_device(self, q, k=3):
+        """q: float64 device tensor [..., 3] -> (dists [..., k], nbrs [..., k])."""
+        import torch
+        q = q.to(self.device, torch.float64).contiguous()
+        shape = tuple(q.shape[:-1])
+        nq = q.numel() // 3
+        d = torch.empty(shape + (k,), dtype=torch.float64, device=self.device)
+        n = torch.empty(shape + (k,), dtype=torch.int64, device=self.device)
+        _lib.check(_lib.lib().nx_knn_query(self.ctx.handle, _lib.ptr(self.ws), self.V,
+                                           _lib.ptr(q), nq, int(k), _lib.ptr(d), _lib.ptr(n),
+                                           _lib.stream_ptr()))
+        return d, n
+
+    def query(self, x, k=3, workers=-1):
+        """KDTree.query(x, k, workers) on host arrays; workers is accepted
+        and ignored."""
+        import torch
+        kk = int(k)
+        q = torch.from_numpy(np.ascontiguousarray(x, dtype=np.float64)).to(self.device)
+        d, n = self.query_device(q, kk)
+        self.ctx.sync_check()
+        d, n = d.cpu().numpy(), n.cpu().numpy()
+        if isinstance(k, (int, np.integer)) and kk == 1:
+            d, n = d[..., 0], n[..., 0]
+        return d, n
+
+    def query_pixels_device(self, width, height, radius, k=3):
+        """make_ll_arr(width, height, radius) and its query in one pass ->
+        (dists [height, width, k], nbrs [height, width, k]) on the device."""
+        import torch
+        L = _lib.lib()
+        ws = torch.empty(L.nx_pixel_workspace_bytes(int(width), int(height)), dtype=torch.uint8,
+                         device=self.device)
+        d = torch.empty((int(height), int(width), int(k)), dtype=torch.float64, device=self.device)
+        n = torch.empty((int(height), int(width), int(k)), dtype=torch.int64, device=self.device)
+        _lib.check(L.nx_knn_query_pixels(self.ctx.handle, _lib.ptr(self.ws), self.V, int(width),
+                                         int(height), float(radius), int(k), _lib.ptr(d),
+                                         _lib.ptr(n), _lib.ptr(ws), _lib.stream_ptr()))
+        return d, n
+
+
+def build_KDTree(points, lf=10):
+    """util.py:275-287.  The reference stores the tree in cfg.KDT; this
+    returns it (leafsize has no meaning for the grid and is ignored)."""
+    return KnnGrid(points)
+
+
+def make_ll_arr_device(width, height, radius, device=None):
+    """util.py:291-310 on the device -> float64 [height, width, 3]."""
+    import torch
+    dev = torch.device("cuda", torch.cuda.current_device()) if device is None else device
+    L = _lib.lib()
+    ws = torch.empty(L.nx_pixel_workspace_bytes(int(width), int(height)), dtype=torch.uint8,
+                     device=dev)
+    out = torch.empty((int(height), int(width), 3), dtype=torch.float64, device=dev)
+    ctx = _lib.Context.get(dev)
+    _lib.check(L.nx_make_ll_arr(ctx.handle, int(width), int(height), float(radius),
+                                _lib.ptr(out), _lib.ptr(ws), _lib.stream_ptr()))
+    return out
+
+
+def make_ll_arr(width, height, radius):
+    """util.py:291-310 -> float64 [height, width, 3] (host array)."""
+    out = make_ll_arr_device(width, height, radius)
+    _lib.Context.get(out.device).sync_check()
+    return out.cpu().numpy()
+
+
+def make_gray_array_device(dists, nbrs, colors):
+    """util.py:343-366 on device tensors: dists float64 [..., k], nbrs int64
+    [..., k], colors float64 [n] -> int32 [...] (before any astype)."""
+    import torch
+    k = dists.shape[-1]
+    if nbrs.shape != dists.shape:
+        raise ValueError("dists and nbrs differ in shape")
+    d = dists.contiguous()
+    n = nbrs.to(torch.int64).contiguous()
+    c = colors.to(d.device, torch.float64).contiguous()
+    out = torch.empty(tuple(d.shape[:-1]), dtype=torch.int32, device=d.device)
+    ctx = _lib.Context.get(d.device)
+    _lib.check(_lib.lib().nx_make_gray_array(ctx.handle, out.numel(), int(k), _lib.ptr(d),
+                                             _lib.ptr(n), _lib.ptr(c), c.numel(), _lib.ptr(out),
+                                             _lib.stream_ptr()))
+    return out
+
+
+def _gray_host(width, height, dists, nbrs, colors):
+    import torch
+    dev = torch.device("cuda", torch.cuda.current_device())
+    d = torch.from_numpy(np.ascontiguousarray(dists, dtype=np.float64)).to(dev)
+    n = torch.from_numpy(np.ascontiguousarray(nbrs, dtype=np.int64)).to(dev)
+    c = torch.from_numpy(np.ascontiguousarray(colors, dtype=np.float64).ravel()).to(dev)
+    out = make_gray_array_device(d, n, c)
+    _lib.Context.get(dev).sync_check()
+    return out.cpu().numpy().reshape(int(height), int(width))
+
+
+def make_gray_array(width, height, dists, nbrs, colors):
+    """util.py:343-366: the grayscale map, in the colors' dtype."""
+    return _gray_host(width, height, dists, nbrs, colors).astype(np.asarray(colors).dtype)
+
+
+def make_rgb_array(width, height, dists, nbrs, colors):
+    """util.py:313-340: the same value in three channels, in the colors' dtype."""
+    g = _gray_host(width, height, dists, nbrs, colors)
+    return np.repeat(g[:, :, None], 3, axis=2).astype(np.asarray(colors).dtype)
+
+
+def build_image_data(colors, width, height, query_data):
+    """util.py:369-438.  The reference reads the image size from options.json
+    and the query results from cfg.IMG_QUERY_DATA; here they are arguments
+    (query_data = (dists, nbrs) as KnnGrid.query returns them)."""
+    dists, nbrs = query_data
+    result = {}
+    if not isinstance(colors, dict):
+        print("ERROR: Must pass a dict when saving out texture maps.")
+    for key, container in colors.items():
+        array, mode = container[0], container[1]
+        if array.dtype in ('int8', 'uint8', 'bool_'):
+            colors[key] = [rescale(array.astype(np.float64), 0, 255), mode]
+        elif array.dtype in ('uint16', 'uint32'):
+            pass
+        else:
+            colors[key] = [rescale(array, 0, 255), mode]
+    for key, container in colors.items():
+        array, mode = container[0], container[1]
+        if mode in ('rgb', 'RGB'):
+            pixels = make_rgb_array(width, height, dists, nbrs, array)
+        elif mode in ('gray', 'GRAY', 'grey', 'GREY'):
+            pixels = make_gray_array(width, height, dists, nbrs, array)
+        colors[key] = None
+        if array.dtype in ('float16', 'int32', 'uint32', 'float32', 'int64', 'uint64', 'float64'):
+            result[key] = pixels.astype('uint8')
+        else:
+            result[key] = pixels.astype(array.dtype)
+    return result
+
+
+def save_image(data, path, name, fmt="png"):
+    """util.py:441-455 with Pillow, as the reference (the format comes from
+    options.json there and is an argument here)."""
+    import os
+    from PIL import Image
+    for key, array in data.items():
+        Image.fromarray(array).save(os.path.join(path, f"{name}_{key}.{fmt}"))

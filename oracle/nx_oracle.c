@@ -1029,3 +1029,88 @@ double ora_height_chain(double *h, int64_t n, double min_alt, double max_alt,
     *rc = (r || r2 || r3 || r4) ? -1 : 0;
     return ocean_level;
 }
+
+/* util.py:291-310 with util.py:80-90.  lat and lon follow the reference's
+ * expressions (90 - h * latpercent, -180 + w * lonpercent, each clamped with
+ * max); x = (r * cos(lat * (pi/180))) * cos(lon * (pi/180)) and so on.  GCC
+ * pairs each sin and cos of one angle into a glibc sincos call, as LLVM does
+ * for numba's sin/cos intrinsics; whether numba's build does so is not
+ * checked (parity unpinned to the last bit of a few coordinates). */
+void ora_make_ll_arr(int64_t width, int64_t height, double radius, double *out)
+{
+    const double latpercent = 180.0 / (double)(height - 1);
+    const double lonpercent = 360.0 / (double)width;
+    const double d2r = 3.141592653589793 / 180.0;  /* np.pi / 180 */
+#pragma omp parallel for schedule(static)
+    for (int64_t h = 0; h < height; ++h) {
+        double lat = 90.0 - ((double)h * latpercent);
+        if (lat < -90.0) lat = -90.0;
+        for (int64_t w = 0; w < width; ++w) {
+            double lon = -180.0 + ((double)w * lonpercent);
+            if (lon < -180.0) lon = -180.0;
+            double *o = out + (h * width + w) * 3;
+            o[0] = radius * cos(lat * d2r) * cos(lon * d2r);
+            o[1] = radius * cos(lat * d2r) * sin(lon * d2r);
+            o[2] = radius * sin(lat * d2r);
+        }
+    }
+}
+
+/* scipy.spatial.KDTree.query, brute force (see the header). */
+void ora_knn(const double *pts, int64_t V, const double *q, int64_t nq, int k, double *dists,
+             int64_t *nbrs)
+{
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t a = 0; a < nq; ++a) {
+        double bd[16];
+        int64_t bi[16];
+        int have = 0;
+        const double *p = q + 3 * a;
+        for (int64_t v = 0; v < V; ++v) {
+            const double *c = pts + 3 * v;
+            double s = 0.0, d;
+            d = p[0] - c[0];
+            s += d * d;
+            d = p[1] - c[1];
+            s += d * d;
+            d = p[2] - c[2];
+            s += d * d;
+            if (have == k && !(s < bd[k - 1])) continue;
+            int j = have < k ? have++ : k - 1;
+            while (j > 0 && s < bd[j - 1]) {
+                bd[j] = bd[j - 1];
+                bi[j] = bi[j - 1];
+                --j;
+            }
+            bd[j] = s;
+            bi[j] = v;
+        }
+        for (int j = 0; j < k; ++j) {
+            dists[a * k + j] = j < have ? sqrt(bd[j]) : INFINITY;
+            nbrs[a * k + j] = j < have ? bi[j] : V;
+        }
+    }
+}
+
+/* util.py:343-366: sd = sum(d); w_i = 1 / ((d_i + 0.00001) / sd);
+ * t = sum(w); iw_i = w_i / t; value = int(sum(colors[n_i] * iw_i)), each sum
+ * accumulated from 0 in neighbour order. */
+void ora_make_gray_array(int64_t npix, int k, const double *dists, const int64_t *nbrs,
+                         const double *colors, int32_t *out)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < npix; ++p) {
+        const double *d = dists + p * k;
+        const int64_t *n = nbrs + p * k;
+        double sd = 0.0;
+        for (int i = 0; i < k; ++i) sd += d[i];
+        double ws[16], t = 0.0;
+        for (int i = 0; i < k; ++i) {
+            ws[i] = 1.0 / ((d[i] + 0.00001) / sd);
+            t += ws[i];
+        }
+        double v = 0.0;
+        for (int i = 0; i < k; ++i) v += colors[n[i]] * (ws[i] / t);
+        out[p] = (int32_t)(int64_t)v;
+    }
+}

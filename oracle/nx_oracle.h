@@ -107,6 +107,20 @@ double ora_find_percent_val(double minval, double maxval, double percent);
 double ora_height_chain(double *h, int64_t n, double min_alt, double max_alt,
                         double ocean_percent, uint8_t *mask, int *rc);
 
+/* util.py:291-310 (make_ll_arr) with util.py:80-90 (latlon2xyz): out is
+ * [height][width][3], row 0 at latitude 90, column 0 at longitude -180. */
+void ora_make_ll_arr(int64_t width, int64_t height, double radius, double *out);
+/* scipy.spatial.KDTree.query(q, k) as util.py:275-287 and nixis.py:282 use
+ * it: the k nearest points, nearest first, distance sqrt of the squared
+ * differences summed in x, y, z order.  Brute force; equal distances keep the
+ * lower index. */
+void ora_knn(const double *pts, int64_t V, const double *q, int64_t nq, int k, double *dists,
+             int64_t *nbrs);
+/* util.py:343-366 (make_gray_array) over k neighbours per pixel:
+ * out[p] = (int)(sum_i colors[nbrs[p][i]] * iw_i). */
+void ora_make_gray_array(int64_t npix, int k, const double *dists, const int64_t *nbrs,
+                         const double *colors, int32_t *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -63,6 +63,9 @@ def lib():
         L.ora_find_percent_val.restype = _d
         L.ora_height_chain.argtypes = [_p, _i64, _d, _d, _d, _p, ctypes.POINTER(_i32)]
         L.ora_height_chain.restype = _d
+        L.ora_make_ll_arr.argtypes = [_i64, _i64, _d, _p]
+        L.ora_knn.argtypes = [_p, _i64, _p, _i64, ctypes.c_int, _p, _p]
+        L.ora_make_gray_array.argtypes = [_i64, ctypes.c_int, _p, _p, _p, _p]
         _lib = L
     return _lib
 
@@ -280,3 +283,35 @@ def height_chain(height, min_alt=-4000.0, max_alt=8850.0, ocean_percent=55.0):
     if rc.value:
         raise ZeroDivisionError("height chain: zero range")
     return h, m.astype(bool), level
+
+
+def make_ll_arr(width, height, radius):
+    """util.py:291-310 -> float64 [height, width, 3]."""
+    out = np.empty((int(height), int(width), 3), np.float64)
+    lib().ora_make_ll_arr(int(width), int(height), float(radius), _ptr(out))
+    return out
+
+
+def knn(points, queries, k=3):
+    """scipy KDTree.query(queries, k) by brute force -> (dists, nbrs) shaped
+    queries.shape[:-1] + (k,)."""
+    p = _f64(points).reshape(-1, 3)
+    q = _f64(queries)
+    shape = q.shape[:-1]
+    q = q.reshape(-1, 3)
+    d = np.empty((q.shape[0], k), np.float64)
+    n = np.empty((q.shape[0], k), np.int64)
+    lib().ora_knn(_ptr(p), p.shape[0], _ptr(q), q.shape[0], int(k), _ptr(d), _ptr(n))
+    return d.reshape(shape + (k,)), n.reshape(shape + (k,))
+
+
+def make_gray_array(width, height, dists, nbrs, colors):
+    """util.py:343-366 -> int32 [height, width] (before the reference's
+    astype to the colors' dtype)."""
+    d = _f64(dists)
+    k = d.shape[-1]
+    n = np.ascontiguousarray(nbrs, dtype=np.int64)
+    c = _f64(colors)
+    out = np.empty(int(height) * int(width), np.int32)
+    lib().ora_make_gray_array(out.size, int(k), _ptr(d), _ptr(n), _ptr(c), _ptr(out))
+    return out.reshape(int(height), int(width))
