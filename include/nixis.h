@@ -274,6 +274,24 @@ int nx_erosion_run_tiled(nx_ctx *ctx, void *state, int64_t V,
                          const nx_tile_plan *plan, int iters, double rain,
                          void *stream);
 
+/* erosion.py:301-381 (erosion_iteration4): rule 4 runs the same step as
+ * rule 3 (erosion_iteration3) with every neighbour contribution and every
+ * update clamped by max(x, 0) as numba evaluates it (0 when 0 > x, else x).
+ * Otherwise as nx_erosion_run_tiled (rule 3 is exactly that call); results
+ * equal the reference's FP64 step bit for bit. */
+int nx_erosion_run_rule(nx_ctx *ctx, void *state, int64_t V,
+                        const int32_t *col, const double *elen,
+                        const nx_tile_plan *plan, int iters, double rain,
+                        int rule, void *stream);
+
+/* erosion.py:42-70 (erode_terrain1) over erosion.py:76-99
+ * (erosion_iteration1): iters Jacobi steps in place on h (device f64 [V]),
+ * each adding 0.0005 per higher neighbour and subtracting it per lower one
+ * in slot order; tmp is a device scratch buffer of V doubles.  col as for
+ * nx_erosion_run. */
+int nx_erosion1_run(nx_ctx *ctx, double *h, double *tmp, int64_t V,
+                    const int32_t *col, int iters, void *stream);
+
 /* Store the current fields back as FP64 (any output may be NULL). */
 int nx_erosion_store(nx_ctx *ctx, const void *state, int64_t V, double *h,
                      double *w, double *s, void *stream);

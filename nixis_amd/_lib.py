@@ -128,6 +128,11 @@ SIGNATURES = {
                                     ctypes.POINTER(NxTilePlan)]),
     "nx_erosion_run_tiled": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p, _c_void_p,
                                       ctypes.POINTER(NxTilePlan), _c_int, _c_double, _c_void_p]),
+    "nx_erosion_run_rule": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p, _c_void_p,
+                                     ctypes.POINTER(NxTilePlan), _c_int, _c_double, _c_int,
+                                     _c_void_p]),
+    "nx_erosion1_run": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_i64, _c_void_p, _c_int,
+                                 _c_void_p]),
     "nx_erosion_store": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p, _c_void_p,
                                   _c_void_p, _c_void_p]),
     "nx_erosion_mass": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p, _c_void_p]),

@@ -305,6 +305,20 @@ int nx_erosion_run_tiled(nx_ctx *ctx, void *state, int64_t V, const int32_t *col
     return erosion_run(ctx, state, V, col, elen, plan, iters, rain, (hipStream_t)stream);
 }
 
+int nx_erosion_run_rule(nx_ctx *ctx, void *state, int64_t V, const int32_t *col,
+                        const double *elen, const nx_tile_plan *plan, int iters, double rain,
+                        int rule, void *stream)
+{
+    return erosion_run_rule(ctx, state, V, col, elen, plan, iters, rain, rule,
+                            (hipStream_t)stream);
+}
+
+int nx_erosion1_run(nx_ctx *ctx, double *h, double *tmp, int64_t V, const int32_t *col, int iters,
+                    void *stream)
+{
+    return erosion1_run(ctx, h, tmp, V, col, iters, (hipStream_t)stream);
+}
+
 int nx_icosphere_device(nx_ctx *ctx, int64_t d, double *verts, int32_t *tris, void *stream)
 {
     return icosphere_dev(ctx, d, verts, tris, (hipStream_t)stream);

@@ -122,6 +122,11 @@ int erosion_load(nx_ctx *ctx, void *state, int64_t V, const double *h, const dou
                  const double *s, hipStream_t stream);
 int erosion_run(nx_ctx *ctx, void *state, int64_t V, const int32_t *col, const double *elen,
                 const nx_tile_plan *plan, int iters, double rain, hipStream_t stream);
+int erosion_run_rule(nx_ctx *ctx, void *state, int64_t V, const int32_t *col, const double *elen,
+                     const nx_tile_plan *plan, int iters, double rain, int rule,
+                     hipStream_t stream);
+int erosion1_run(nx_ctx *ctx, double *h, double *tmp, int64_t V, const int32_t *col, int iters,
+                 hipStream_t stream);
 int erosion_store(nx_ctx *ctx, const void *state, int64_t V, double *h, double *w, double *s,
                   hipStream_t stream);
 int erosion_mass(nx_ctx *ctx, const void *state, int64_t V, double *out, hipStream_t stream);

@@ -55,7 +55,8 @@ static ErosionBufs bufs(void *state, int64_t V)
 
 // Launch j of a sequence reads the buffer that j earlier launches flipped
 // to; the step counter and buffer come from the workspace so that replays
-// of a captured sequence stay correct.
+// of a captured sequence stay correct.  RULE 3 or 4 (nx_erosion_dev.h).
+template <int RULE>
 __global__ __launch_bounds__(256) void k_erode_step(const ErosionMeta *__restrict__ meta,
                                                     double2 *__restrict__ hw0,
                                                     double2 *__restrict__ hw1,
@@ -69,12 +70,12 @@ __global__ __launch_bounds__(256) void k_erode_step(const ErosionMeta *__restric
     const int64_t tile = xcd_tile(blockIdx.x, nblocks);
     const int64_t i = tile * blockDim.x + threadIdx.x;
     if (tile < nblocks && i < V)
-        erode_vertex_step(i, V, p ? hw1 : hw0, p ? s1 : s0, p ? hw0 : hw1, p ? s0 : s1, col, elen,
-                          rain_next);
+        erode_vertex_step<RULE>(i, V, p ? hw1 : hw0, p ? s1 : s0, p ? hw0 : hw1, p ? s0 : s1, col,
+                                elen, rain_next);
 }
 
 // Two steps per launch, one block per tile of the plan (nx_pair_dev.h).
-template <int THREADS, bool KEEP>
+template <int THREADS, bool KEEP, int RULE = 3>
 __global__ __launch_bounds__(THREADS) void k_erode_pair(const ErosionMeta *__restrict__ meta,
                                                        double2 *__restrict__ hw0,
                                                        double2 *__restrict__ hw1,
@@ -85,8 +86,8 @@ __global__ __launch_bounds__(THREADS) void k_erode_pair(const ErosionMeta *__res
     __shared__ double2 A[kLoadMax];
     const int64_t tile = xcd_tile(blockIdx.x, P.ntiles);
     if (tile >= P.ntiles) return;
-    erode_tile_pair<THREADS, KEEP>(A, hw0, hw1, s0, s1, (meta->buf + j) & 1, P, tile, rain_mid,
-                                   rain_next);
+    erode_tile_pair<THREADS, KEEP, RULE>(A, hw0, hw1, s0, s1, (meta->buf + j) & 1, P, tile,
+                                         rain_mid, rain_next);
 }
 
 // Kernel variants (NX_OPT_PAIR_KERNEL): block size and register use.
@@ -101,6 +102,34 @@ static const PairVariant kPairVariants[] = {
     {k_erode_pair<1024, false>, 1024}, {k_erode_pair<256, true>, 256},
     {k_erode_pair<256, false>, 256},
 };
+// erosion_iteration4 uses the default variant only.
+static const PairVariant kPairRule4 = {k_erode_pair<1024, true, 4>, 1024};
+
+// erosion.py:76-99 (erosion_iteration1): w[i] = r[i] + amt with amt moved
+// by 0.0005 towards each higher neighbour and away from each lower one, in
+// slot order (a pad slot points at the vertex itself and changes nothing,
+// as the reference skipping -1).
+__global__ __launch_bounds__(256) void k_erode_v1(const double *__restrict__ r,
+                                                  double *__restrict__ w, int64_t V,
+                                                  const int32_t *__restrict__ col,
+                                                  int64_t nblocks)
+{
+    const int64_t tile = xcd_tile(blockIdx.x, nblocks);
+    const int64_t i = tile * blockDim.x + threadIdx.x;
+    if (tile >= nblocks || i >= V) return;
+    const double simple_constant = 0.0005;
+    const double thisvert = r[i];
+    double amt = 0.0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const double rn = r[col[(int64_t)k * V + i]];
+        if (rn > thisvert)
+            amt += simple_constant;
+        else if (rn < thisvert)
+            amt -= simple_constant;
+    }
+    w[i] = thisvert + amt;
+}
 
 
 // erosion.py:184-186: water += rain on the current fields (before a step).
@@ -234,8 +263,8 @@ int erosion_load(nx_ctx *ctx, void *state, int64_t V, const double *h, const dou
 // With a tile plan, steps go two per launch and an odd last step runs alone.
 // The last step's rain for a following step is `rain` unless `last`.
 static void launch_steps(const ErosionBufs &b, int64_t V, const int32_t *col, const double *elen,
-                         const nx_tile_plan *plan, int variant, int n, double rain, bool last,
-                         hipStream_t stream)
+                         const nx_tile_plan *plan, int variant, int rule, int n, double rain,
+                         bool last, hipStream_t stream)
 {
     const int64_t nblocks = (V + 255) / 256;
     const int64_t grid = ((nblocks + 7) / 8) * 8;
@@ -245,15 +274,16 @@ static void launch_steps(const ErosionBufs &b, int64_t V, const int32_t *col, co
         const int64_t tgrid = ((P.ntiles + 7) / 8) * 8;
         for (; k + 2 <= n; k += 2, ++j) {
             const double rain_next = (last && k + 2 == n) ? 0.0 : rain;
-            const PairVariant &pv = kPairVariants[variant];
+            const PairVariant &pv = rule == 4 ? kPairRule4 : kPairVariants[variant];
             hipLaunchKernelGGL(pv.fn, dim3((unsigned)tgrid), dim3(pv.threads), 0, stream, b.meta,
                                b.hw[0], b.hw[1], b.s[0], b.s[1], P, j, rain, rain_next);
         }
     }
     for (; k < n; ++k, ++j) {
         const double rain_next = (last && k + 1 == n) ? 0.0 : rain;
-        hipLaunchKernelGGL(k_erode_step, dim3((unsigned)grid), dim3(256), 0, stream, b.meta,
-                           b.hw[0], b.hw[1], b.s[0], b.s[1], V, col, elen, nblocks, j, rain_next);
+        hipLaunchKernelGGL(rule == 4 ? k_erode_step<4> : k_erode_step<3>, dim3((unsigned)grid),
+                           dim3(256), 0, stream, b.meta, b.hw[0], b.hw[1], b.s[0], b.s[1], V, col,
+                           elen, nblocks, j, rain_next);
     }
     hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, stream, b.meta, (int64_t)n, j);
 }
@@ -263,11 +293,11 @@ static void launch_steps(const ErosionBufs &b, int64_t V, const int32_t *col, co
 // buffer and exponent state from the workspace rather than from its
 // arguments.  No memset is recorded into the graph.
 static int chunk_graph(nx_ctx *ctx, const ErosionBufs &b, void *state, int64_t V,
-                       const int32_t *col, const double *elen, const nx_tile_plan *plan, int n,
-                       double rain, bool last, hipGraphExec_t *out)
+                       const int32_t *col, const double *elen, const nx_tile_plan *plan, int rule,
+                       int n, double rain, bool last, hipGraphExec_t *out)
 {
-    GraphKey key{state, V, col, elen, n, rain, last,
-                 plan ? (uint64_t)(uintptr_t)plan->blob * 16 + (uint64_t)ctx->pair_variant : 0};
+    const uint64_t ptag = plan ? (uint64_t)(uintptr_t)plan->blob * 16 + (uint64_t)ctx->pair_variant : 0;
+    GraphKey key{state, V, col, elen, n, rain, last, ptag * 8 + (uint64_t)rule};
     std::lock_guard<std::mutex> lk(ctx->mu);
     auto it = ctx->graphs.find(key);
     if (it != ctx->graphs.end()) {
@@ -277,7 +307,7 @@ static int chunk_graph(nx_ctx *ctx, const ErosionBufs &b, void *state, int64_t V
     if (ctx->graphs.size() >= kMaxGraphs) ctx->clear_graphs();
     hipGraph_t g = nullptr;
     NX_HIP(hipStreamBeginCapture(ctx->capture_stream, hipStreamCaptureModeThreadLocal));
-    launch_steps(b, V, col, elen, plan, ctx->pair_variant, n, rain, last, ctx->capture_stream);
+    launch_steps(b, V, col, elen, plan, ctx->pair_variant, rule, n, rain, last, ctx->capture_stream);
     const hipError_t le = hipGetLastError();
     const hipError_t ce = hipStreamEndCapture(ctx->capture_stream, &g);
     if (le != hipSuccess || ce != hipSuccess) {
@@ -311,8 +341,16 @@ static int check_plan(const nx_tile_plan *plan, int64_t V)
 int erosion_run(nx_ctx *ctx, void *state, int64_t V, const int32_t *col, const double *elen,
                 const nx_tile_plan *plan, int iters, double rain, hipStream_t stream)
 {
+    return erosion_run_rule(ctx, state, V, col, elen, plan, iters, rain, 3, stream);
+}
+
+int erosion_run_rule(nx_ctx *ctx, void *state, int64_t V, const int32_t *col, const double *elen,
+                     const nx_tile_plan *plan, int iters, double rain, int rule,
+                     hipStream_t stream)
+{
     int rc = check_state(ctx, state, V);
     if (rc) return rc;
+    if (rule != 3 && rule != 4) return set_error(NX_EINVAL, "erosion rule %d is not 3 or 4", rule);
     if (!col || !elen) return set_error(NX_EINVAL, "null neighbour table");
     if ((rc = check_plan(plan, V))) return rc;
     if (iters < 0) return set_error(NX_EINVAL, "iters=%d < 0", iters);
@@ -326,7 +364,7 @@ int erosion_run(nx_ctx *ctx, void *state, int64_t V, const int32_t *col, const d
                            b.hw[0], b.hw[1], V, rain);
     const int chunk = ctx->graph_chunk;
     if (chunk <= 0 || iters < 2) {
-        launch_steps(b, V, col, elen, plan, ctx->pair_variant, iters, rain, true, stream);
+        launch_steps(b, V, col, elen, plan, ctx->pair_variant, rule, iters, rain, true, stream);
         NX_HIP(hipGetLastError());
         return NX_OK;
     }
@@ -336,7 +374,7 @@ int erosion_run(nx_ctx *ctx, void *state, int64_t V, const int32_t *col, const d
         const int n = std::min(chunk, iters - done);
         const bool last = done + n == iters;
         hipGraphExec_t ex = nullptr;
-        rc = chunk_graph(ctx, b, state, V, col, elen, plan, n, rain, last, &ex);
+        rc = chunk_graph(ctx, b, state, V, col, elen, plan, rule, n, rain, last, &ex);
         if (rc) return rc;
         NX_HIP(hipGraphLaunch(ex, stream));
         done += n;
@@ -347,6 +385,29 @@ int erosion_run(nx_ctx *ctx, void *state, int64_t V, const int32_t *col, const d
 }  // namespace nx
 
 namespace nx {
+
+// erosion.py:42-70 (erode_terrain1): iters steps of erosion_iteration1,
+// Jacobi, in place on h, with tmp (V doubles) as the other buffer.
+int erosion1_run(nx_ctx *ctx, double *h, double *tmp, int64_t V, const int32_t *col, int iters,
+                 hipStream_t stream)
+{
+    if (!ctx || !h || !tmp || !col) return set_error(NX_EINVAL, "null argument");
+    if (V <= 0 || V > INT32_MAX) return set_error(NX_EINVAL, "V=%lld outside 1..2^31-1", (long long)V);
+    if (iters < 0) return set_error(NX_EINVAL, "iters=%d < 0", iters);
+    if (iters == 0) return NX_OK;
+    NX_HIP(hipSetDevice(ctx->device));
+    const int64_t nblocks = (V + 255) / 256;
+    const int64_t grid = ((nblocks + 7) / 8) * 8;
+    double *src = h, *dst = tmp;
+    for (int k = 0; k < iters; ++k) {
+        hipLaunchKernelGGL(k_erode_v1, dim3((unsigned)grid), dim3(256), 0, stream, src, dst, V, col,
+                           nblocks);
+        std::swap(src, dst);
+    }
+    if (src != h) NX_HIP(hipMemcpyAsync(h, src, sizeof(double) * (size_t)V, hipMemcpyDeviceToDevice, stream));
+    NX_HIP(hipGetLastError());
+    return NX_OK;
+}
 
 int erosion_store(nx_ctx *ctx, const void *state, int64_t V, double *h, double *w, double *s,
                   hipStream_t stream)

@@ -44,31 +44,60 @@ constexpr double kEvap = 0.1 / 320;
 constexpr double kSol = 0.01 / 320;
 constexpr double kCap = 0.2 / 320;
 
+// RULE 3 is erosion.py:197-279 (erosion_iteration3); RULE 4 is
+// erosion.py:301-381 (erosion_iteration4), the same step with every
+// contribution clamped by max(x, 0).  numba evaluates max(x, 0) as Python
+// does: 0 when 0 > x, else x (so -0.0 and NaN pass through).
+NX_EHD double max0(double x) { return 0.0 > x ? 0.0 : x; }
+
 // One neighbour: hn, wn its height and water, d the edge length.
+template <int RULE = 3>
 NX_EHD void erode_accumulate(double h, double hn, double wn, double d, double &S, double &W)
 {
     double slope = hn - h;
     if (__builtin_expect(fabs(slope) < 0x1p-1000, 0)) slope = slope / (d + 0.00001);
     if (slope > 0) {
-        S += kSol * wn;
-        W += wn * d;
+        if (RULE == 4) {
+            S += max0(kSol * wn);
+            W += max0(wn * d);
+        } else {
+            S += kSol * wn;
+            W += wn * d;
+        }
     } else if (slope < 0) {
-        S -= kSol * wn;
-        W -= wn * d;
+        if (RULE == 4) {
+            S -= max0(kSol * wn);
+            W -= max0(wn * d);
+        } else {
+            S -= kSol * wn;
+            W -= wn * d;
+        }
     }
 }
 
 // Finish the vertex; `rain` (0 for none) is the next iteration's rainfall,
 // added to the new water as the reference's water += rain would.
+template <int RULE = 3>
 NX_EHD void erode_finish(double h, double w, double s, double S, double W, double rain,
                          double &h_out, double &w_out, double &s_out)
 {
-    double hn = h - S;
-    double sn = s + S;
-    const double wn = w + (W - W * kEvap);
-    if (sn > (kCap * wn)) {
-        hn += (sn - (kCap * wn));
-        sn -= (sn - (kCap * wn));
+    double hn, sn, wn;
+    if (RULE == 4) {  // erosion.py:366-371
+        hn = h - max0(S);
+        sn = s + max0(S);
+        wn = w + max0(W - W * kEvap);
+        if (sn > (kCap * wn)) {
+            hn += max0(sn - (kCap * wn));
+            sn -= max0(sn - (kCap * wn));
+        }
+    } else {  // erosion.py:265-270
+        hn = h - S;
+        sn = s + S;
+        wn = w + (W - W * kEvap);
+        if (sn > (kCap * wn)) {
+            hn += (sn - (kCap * wn));
+            sn -= (sn - (kCap * wn));
+        }
     }
     h_out = hn;
     s_out = sn;
@@ -102,6 +131,7 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb)
 
 // One vertex of one step: gather the six neighbours from the source
 // buffers, update, write the destination buffers.
+template <int RULE = 3>
 __device__ __forceinline__ void erode_vertex_step(int64_t i, int64_t V, const double2 *hw_src,
                                                   const double *s_src, double2 *hw_dst,
                                                   double *s_dst, const int32_t *col,
@@ -114,10 +144,10 @@ __device__ __forceinline__ void erode_vertex_step(int64_t i, int64_t V, const do
         const int32_t n = gmem(col)[(int64_t)k * V + i];
         const double d = gmem(elen)[(int64_t)k * V + i];
         const double2 nb = gmem(hw_src)[n];
-        erode_accumulate(me.x, nb.x, nb.y, d, S, W);
+        erode_accumulate<RULE>(me.x, nb.x, nb.y, d, S, W);
     }
     double ho, wo, so;
-    erode_finish(me.x, me.y, gmem(s_src)[i], S, W, rain, ho, wo, so);
+    erode_finish<RULE>(me.x, me.y, gmem(s_src)[i], S, W, rain, ho, wo, so);
     gmem(hw_dst)[i] = make_double2(ho, wo);
     gmem(s_dst)[i] = so;
 }

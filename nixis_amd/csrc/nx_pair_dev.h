@@ -55,7 +55,7 @@ inline TilePlanDev plan_dev(const nx_tile_plan &q)
 // THREADS is the block size; KEEP holds the tile's neighbour indices and
 // edge lengths in registers between the steps (otherwise the second step
 // reads them again).  A is the block's LDS staging area.
-template <int THREADS, bool KEEP>
+template <int THREADS, bool KEEP, int RULE = 3>
 __device__ __forceinline__ void erode_tile_pair(double2 *A, double2 *hw0, double2 *hw1, double *s0,
                                                 double *s1, int p, const TilePlanDev &P,
                                                 int64_t tile, double rain_mid, double rain_next)
@@ -104,10 +104,10 @@ __device__ __forceinline__ void erode_tile_pair(double2 *A, double2 *hw0, double
                     len[m < kKeep ? m : 0][q] = d;
                 }
                 const double2 nb = A[n];
-                erode_accumulate(me.x, nb.x, nb.y, d, S, W);
+                erode_accumulate<RULE>(me.x, nb.x, nb.y, d, S, W);
             }
             double ho, wo, so;
-            erode_finish(me.x, me.y, sed[m], S, W, rain_mid, ho, wo, so);
+            erode_finish<RULE>(me.x, me.y, sed[m], S, W, rain_mid, ho, wo, so);
             res[m] = make_double2(ho, wo);
             sed[m] = so;
         }
@@ -131,10 +131,10 @@ __device__ __forceinline__ void erode_tile_pair(double2 *A, double2 *hw0, double
                 const int n = KEEP ? nbr[m < kKeep ? m : 0][q] : lcol[q * cs + li];
                 const double d = KEEP ? len[m < kKeep ? m : 0][q] : lel[q * cs + li];
                 const double2 nb = A[n];
-                erode_accumulate(me.x, nb.x, nb.y, d, S, W);
+                erode_accumulate<RULE>(me.x, nb.x, nb.y, d, S, W);
             }
             double ho, wo, so;
-            erode_finish(me.x, me.y, sed[m], S, W, rain_next, ho, wo, so);
+            erode_finish<RULE>(me.x, me.y, sed[m], S, W, rain_next, ho, wo, so);
             hw_dst[a + li] = make_double2(ho, wo);
             s_dst[a + li] = so;
         }

@@ -164,7 +164,7 @@ class ErosionGraph:
 
 
 class ErosionState:
-    """Device-resident h/w/s (float-float, double-buffered) of one planet."""
+    """Device-resident h/w/s (FP64, double-buffered) of one planet."""
 
     def __init__(self, V, device=None):
         import torch
@@ -194,14 +194,16 @@ class ErosionState:
                                               _lib.ptr(h), _lib.ptr(w), _lib.ptr(s),
                                               _lib.stream_ptr(stream)))
 
-    def run(self, graph, iters, rain=RAIN_DEFAULT, stream=None):
+    def run(self, graph, iters, rain=RAIN_DEFAULT, stream=None, rule=3):
+        """iters steps, rain added before each; rule 3 is erosion_iteration3
+        (erosion.py:197-279), rule 4 erosion_iteration4 (erosion.py:301-381)."""
         if graph.V != self.V:
             raise ValueError("graph and state sizes differ")
         plan = ctypes.byref(graph.plan) if graph.plan is not None else None
-        _lib.check(_lib.lib().nx_erosion_run_tiled(self.ctx.handle, _lib.ptr(self.buf), self.V,
-                                                   _lib.ptr(graph.col), _lib.ptr(graph.elen),
-                                                   plan, int(iters), float(rain),
-                                                   _lib.stream_ptr(stream)))
+        _lib.check(_lib.lib().nx_erosion_run_rule(self.ctx.handle, _lib.ptr(self.buf), self.V,
+                                                  _lib.ptr(graph.col), _lib.ptr(graph.elen),
+                                                  plan, int(iters), float(rain), int(rule),
+                                                  _lib.stream_ptr(stream)))
 
     def store(self, h=None, w=None, s=None, stream=None):
         _lib.check(_lib.lib().nx_erosion_store(self.ctx.handle, _lib.ptr(self.buf), self.V,
@@ -260,3 +262,65 @@ def erosion_iteration3(verts, neighbors, r_buff, wat, sed):
     st.ctx.sync_check()
     for a, t in zip((r_buff, wat, sed), outs):
         a[...] = graph.unpermute(t.cpu().numpy())
+
+
+def erosion_iteration4(verts, neighbors, r_buff, wat, sed):
+    """erosion.py:301-381: one clamped step; r_buff, wat, sed (float64[V])
+    updated in place."""
+    import torch
+    for a in (r_buff, wat, sed):
+        if a.dtype != np.float64:
+            raise TypeError("fields must be float64")
+    graph = ErosionGraph(verts, neighbors)
+    st = ErosionState(graph.V, graph.device)
+    st.load(_dev(r_buff, graph), _dev(wat, graph), _dev(sed, graph))
+    st.run(graph, 1, 0.0, rule=4)
+    outs = [torch.empty(graph.V, dtype=torch.float64, device=graph.device) for _ in range(3)]
+    st.store(*outs)
+    st.ctx.sync_check()
+    for a, t in zip((r_buff, wat, sed), outs):
+        a[...] = graph.unpermute(t.cpu().numpy())
+
+
+def erode_terrain4(nodes, neighbors, heights, num_iter=1, snapshot=None):
+    """erosion.py:282-296.  As in the reference, this driver adds 0.3/320 of
+    rain and runs erosion_iteration3 (not erosion_iteration4) each pass, so
+    it is erode_terrain3 without the snapshot option."""
+    erode_terrain3(nodes, neighbors, heights, num_iter, False, 0.3 / 320)
+
+
+def _v1_run(graph, h_dev, iters):
+    import torch
+    tmp = torch.empty_like(h_dev)
+    ctx = _lib.Context.get(h_dev.device)
+    _lib.check(_lib.lib().nx_erosion1_run(ctx.handle, _lib.ptr(h_dev), _lib.ptr(tmp), graph.V,
+                                          _lib.ptr(graph.col), int(iters), _lib.stream_ptr()))
+    ctx.sync_check()
+
+
+def erosion_iteration1(neighbors, r_buff, w_buff):
+    """erosion.py:76-99: w_buff (float64[V]) receives one step from r_buff
+    and is returned.  The step needs only the neighbour table."""
+    if w_buff.dtype != np.float64:
+        raise TypeError("w_buff must be float64")
+    nb = np.ascontiguousarray(neighbors, dtype=np.int32)
+    graph = ErosionGraph(np.zeros((nb.shape[0], 3)), nb, order=None, tiled=False)
+    h = _dev(r_buff, graph)
+    _v1_run(graph, h, 1)
+    w_buff[...] = graph.unpermute(h.cpu().numpy())
+    return w_buff
+
+
+def erode_terrain1(nodes, neighbors, heights, num_iter=1, snapshot=None):
+    """erosion.py:42-70: num_iter steps of erosion_iteration1 (at least
+    one), heights (float64[V]) updated in place and returned, as the
+    reference returns its read buffer."""
+    if heights.dtype != np.float64:
+        raise TypeError("heights must be float64")
+    if num_iter <= 0:
+        num_iter = 1
+    graph = ErosionGraph(nodes, neighbors, tiled=False)
+    h = _dev(heights, graph)
+    _v1_run(graph, h, num_iter)
+    heights[...] = graph.unpermute(h.cpu().numpy())
+    return heights

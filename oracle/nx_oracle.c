@@ -777,9 +777,11 @@ void ora_erosion_iteration1(const int32_t *nbrs, int64_t V, const double *r,
     }
 }
 
-static inline double dmax0(double a) { return a > 0 ? a : 0; }
+/* numba's max(a, 0) follows Python's algorithm: the first argument unless
+ * the second is greater, so -0.0 and NaN come back unchanged. */
+static inline double dmax0(double a) { return 0.0 > a ? 0.0 : a; }
 
-/* erosion.py:301-381 (numba's max(a, 0) returns a when a > 0, else 0) */
+/* erosion.py:301-381 */
 void ora_erosion_iteration4(const double *verts, const int32_t *nbrs,
                             int64_t V, double *h, double *w, double *s)
 {
