@@ -215,6 +215,43 @@ int nx_make_gray_array(nx_ctx *ctx, int64_t npix, int k, const double *dists,
                        const int64_t *nbrs, const double *colors,
                        int64_t ncolors, int32_t *out, void *stream);
 
+/* ---- insolation (device) ------------------------------------------------ */
+
+/* Insolation at the top of the atmosphere (climate.py:416-590).  Vertex
+ * coordinates are device f64 [V][3]; results are device float32, summed as
+ * numba adds a double to a float32 element (double sum, rounded to float).
+ * The rotation, tilt and seasonal sines and cosines come from the host C
+ * library; per-vertex asin / atan2 / cos run on the device, so values agree
+ * with the reference's to a few float32 ulp per summed step.
+ * ws: nx_climate_workspace_bytes() of 256-byte aligned device memory. */
+size_t nx_climate_workspace_bytes(void);
+/* climate.py:416-446 (sample_insolation) for nsteps (<= 360) rotations from
+ * `rotation`, adding rot_step after each (climate.py:468-469), into arr;
+ * nsteps = 1 is calc_instant_insolation (climate.py:489-498), 360 steps of
+ * 1 degree from -180 brute_daily_insolation (climate.py:449-486). */
+int nx_sample_insolation(nx_ctx *ctx, const double *verts, int64_t V,
+                         double radius, double rotation, double rot_step,
+                         int nsteps, double tilt, float *arr, void *ws,
+                         void *stream);
+/* climate.py:503-533 (calc_insolation_slice): out is device float32 [181],
+ * index = latitude, negative latitudes at 181 + lat. */
+int nx_insolation_slice(nx_ctx *ctx, double radius, double tilt, float *out,
+                        void *ws, void *stream);
+/* climate.py:547-573 (interpolate_insolation) with a device table [181]. */
+int nx_interpolate_insolation(nx_ctx *ctx, const double *verts, int64_t V,
+                              const float *table, double radius, float *out,
+                              void *stream);
+/* climate.py:536-544 (calc_daily_insolation): slice + interpolation. */
+int nx_daily_insolation(nx_ctx *ctx, const double *verts, int64_t V,
+                        double radius, double tilt, float *out, void *ws,
+                        void *stream);
+/* climate.py:576-590 (calc_yearly_insolation): 360 days, seasonal tilt
+ * sin(day * pi / 180) * axial_tilt (climate.py:193-201), daily values
+ * summed in float32 in day order. */
+int nx_yearly_insolation(nx_ctx *ctx, const double *verts, int64_t V,
+                         double radius, double axial_tilt, float *out,
+                         void *ws, void *stream);
+
 /* ---- erosion (device) -------------------------------------------------- */
 
 /* Fields live in a caller-allocated device workspace of

@@ -97,6 +97,18 @@ SIGNATURES = {
                                     _c_void_p]),
     "nx_height_chain": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_double, _c_double, _c_double,
                                  _c_void_p, _c_void_p, _c_void_p, _c_void_p]),
+    "nx_climate_workspace_bytes": (_c_size, []),
+    "nx_sample_insolation": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_double, _c_double,
+                                      _c_double, _c_int, _c_double, _c_void_p, _c_void_p,
+                                      _c_void_p]),
+    "nx_insolation_slice": (_c_int, [_c_void_p, _c_double, _c_double, _c_void_p, _c_void_p,
+                                     _c_void_p]),
+    "nx_interpolate_insolation": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p, _c_double,
+                                           _c_void_p, _c_void_p]),
+    "nx_daily_insolation": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_double, _c_double,
+                                     _c_void_p, _c_void_p, _c_void_p]),
+    "nx_yearly_insolation": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_double, _c_double,
+                                      _c_void_p, _c_void_p, _c_void_p]),
     "nx_knn_grid_bytes": (_c_size, [_c_i64]),
     "nx_knn_grid_build": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p, _c_void_p]),
     "nx_knn_query": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p, _c_i64, _c_int,

@@ -227,6 +227,40 @@ int nx_power_rescale(nx_ctx *ctx, const double *x, const uint8_t *mask, int64_t 
 
 size_t nx_height_chain_workspace_bytes(int64_t n) { return height_chain_workspace_bytes(n); }
 
+size_t nx_climate_workspace_bytes(void) { return climate_workspace_bytes(); }
+
+int nx_sample_insolation(nx_ctx *ctx, const double *verts, int64_t V, double radius,
+                         double rotation, double rot_step, int nsteps, double tilt, float *arr,
+                         void *ws, void *stream)
+{
+    return sample_insolation(ctx, verts, V, radius, rotation, rot_step, nsteps, tilt, arr, ws,
+                             (hipStream_t)stream);
+}
+
+int nx_insolation_slice(nx_ctx *ctx, double radius, double tilt, float *out, void *ws,
+                        void *stream)
+{
+    return insolation_slice(ctx, radius, tilt, out, ws, (hipStream_t)stream);
+}
+
+int nx_daily_insolation(nx_ctx *ctx, const double *verts, int64_t V, double radius, double tilt,
+                        float *out, void *ws, void *stream)
+{
+    return daily_insolation(ctx, verts, V, radius, tilt, out, ws, (hipStream_t)stream);
+}
+
+int nx_interpolate_insolation(nx_ctx *ctx, const double *verts, int64_t V, const float *table,
+                              double radius, float *out, void *stream)
+{
+    return interpolate_insolation(ctx, verts, V, table, radius, out, (hipStream_t)stream);
+}
+
+int nx_yearly_insolation(nx_ctx *ctx, const double *verts, int64_t V, double radius,
+                         double axial_tilt, float *out, void *ws, void *stream)
+{
+    return yearly_insolation(ctx, verts, V, radius, axial_tilt, out, ws, (hipStream_t)stream);
+}
+
 size_t nx_knn_grid_bytes(int64_t V) { return knn_grid_bytes(V); }
 
 int nx_knn_grid_build(nx_ctx *ctx, const double *points, int64_t V, void *grid, void *stream)

@@ -103,6 +103,18 @@ int terrain_heights(nx_ctx *ctx, const double *verts, int64_t V, const int32_t *
                     double radius, double min_alt, double max_alt, double ocean_percent,
                     double *h, uint8_t *mask, double *level, void *ws, hipStream_t stream);
 size_t height_chain_workspace_bytes(int64_t n);
+size_t climate_workspace_bytes();
+int sample_insolation(nx_ctx *ctx, const double *verts, int64_t V, double radius, double rot0,
+                      double rot_step, int nsteps, double tilt, float *arr, void *ws,
+                      hipStream_t stream);
+int insolation_slice(nx_ctx *ctx, double radius, double tilt, float *out, void *ws,
+                     hipStream_t stream);
+int daily_insolation(nx_ctx *ctx, const double *verts, int64_t V, double radius, double tilt,
+                     float *out, void *ws, hipStream_t stream);
+int interpolate_insolation(nx_ctx *ctx, const double *verts, int64_t V, const float *table,
+                           double radius, float *out, hipStream_t stream);
+int yearly_insolation(nx_ctx *ctx, const double *verts, int64_t V, double radius,
+                      double axial_tilt, float *out, void *ws, hipStream_t stream);
 size_t knn_grid_bytes(int64_t V);
 int knn_grid_build(nx_ctx *ctx, const double *pts, int64_t V, void *grid, hipStream_t stream);
 int knn_query(nx_ctx *ctx, const void *grid, int64_t V, const double *q, int64_t nq, int k,

@@ -23,6 +23,7 @@
  *                          force on tiny meshes (tests/test_oracle_erosion.py).
  *   ora_rescale         -- closed form on affine inputs.
  */
+#define _GNU_SOURCE /* sincos */
 #include "nx_oracle.h"
 
 #include <math.h>
@@ -1115,4 +1116,106 @@ void ora_make_gray_array(int64_t npix, int k, const double *dists, const int64_t
         for (int i = 0; i < k; ++i) v += colors[n[i]] * (ws[i] / t);
         out[p] = (int32_t)(int64_t)v;
     }
+}
+
+/* ------------------------------------------------------------ insolation -- */
+
+/* Python's max(a, b) / min(a, b): the first argument unless the second is
+ * strictly greater / smaller. */
+static inline double py_max(double a, double b) { return b > a ? b : a; }
+static inline double py_min(double a, double b) { return b < a ? b : a; }
+
+/* util.py:60-76 (xyz2latlon): degrees(asin(min(max(z / r, -1), 1))) and
+ * degrees(atan2(y, x)), degrees(v) = v * (180 / pi). */
+static inline void ora_xyz2latlon(double x, double y, double z, double r, double *lat,
+                                  double *lon)
+{
+    const double r2d = 180.0 / 3.141592653589793;
+    *lat = asin(py_min(py_max(z / r, -1.0), 1.0)) * r2d;
+    *lon = atan2(y, x) * r2d;
+}
+
+/* climate.py:416-446 for one rotation (one call of sample_insolation). */
+static void ora_insolation_step(float *arr, const double *verts, int64_t V, double radius,
+                                double rotation, double tilt)
+{
+    const double pi = 3.141592653589793;
+    double sr, cr, st, ct;
+    sincos(rotation * pi / 180, &sr, &cr);
+    sincos(tilt * pi / 180, &st, &ct);
+#pragma omp parallel for schedule(static)
+    for (int64_t v = 0; v < V; ++v) {
+        const double x = verts[3 * v], y = verts[3 * v + 1], z = verts[3 * v + 2];
+        const double rx = x * cr - y * sr;
+        const double ry = x * sr + y * cr;
+        const double rz = z;
+        const double tx = rx * ct + rz * st;
+        const double ty = ry;
+        const double tz = rz * ct - rx * st;
+        double lat, lon;
+        ora_xyz2latlon(tx, ty, tz, radius, &lat, &lon);
+        const double val = py_max(cos(fabs(lat) * pi / 180), 0.0) * py_max(cos(lon * pi / 180), 0.0);
+        arr[v] = (float)((double)arr[v] + val);
+    }
+}
+
+void ora_sample_insolation(float *arr, const double *verts, int64_t V, double radius,
+                           double rot0, double rot_step, int nsteps, double tilt)
+{
+    double rotation = rot0;
+    for (int i = 0; i < nsteps; ++i) {
+        ora_insolation_step(arr, verts, V, radius, rotation, tilt);
+        rotation += rot_step;
+    }
+}
+
+void ora_insolation_slice(double radius, double tilt, float *out)
+{
+    double verts[181 * 3];
+    const double d2r = 3.141592653589793 / 180.0;
+    for (int i = -90; i <= 90; ++i) {  /* latlon2xyz(i, 0, radius) at index i mod 181 */
+        const int k = i < 0 ? 181 + i : i;
+        double s, c;
+        sincos((double)i * d2r, &s, &c);
+        verts[3 * k] = radius * c * 1.0;   /* cos(0 * (pi/180)) == 1 */
+        verts[3 * k + 1] = radius * c * 0.0;
+        verts[3 * k + 2] = radius * s;
+    }
+    for (int k = 0; k < 181; ++k) out[k] = 0.0f;
+    ora_sample_insolation(out, verts, 181, radius, -180.0, 360.0 / 360, 360, tilt);
+}
+
+void ora_interpolate_insolation(const double *verts, int64_t V, const float *table, float *out,
+                                double radius)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t v = 0; v < V; ++v) {
+        double lat, lon;
+        ora_xyz2latlon(verts[3 * v], verts[3 * v + 1], verts[3 * v + 2], radius, &lat, &lon);
+        const int64_t lower = (int64_t)floor(lat), upper = (int64_t)ceil(lat);
+        const float tl = table[lower < 0 ? 181 + lower : lower];
+        const float tu = table[upper < 0 ? 181 + upper : upper];
+        if (lower == upper) {
+            out[v] = tl;
+        } else {
+            const float diff = tu - tl;  /* float32 subtraction */
+            out[v] = (float)((double)tl + (lat - (double)lower) * ((double)diff / (double)(upper - lower)));
+        }
+    }
+}
+
+void ora_yearly_insolation(const double *verts, int64_t V, double radius, double axial_tilt,
+                           float *out)
+{
+    const double pi = 3.141592653589793;
+    float table[181];
+    float *daily = (float *)malloc(sizeof(float) * (size_t)V);
+    for (int64_t v = 0; v < V; ++v) out[v] = 0.0f;
+    for (int x = 0; x < 360; ++x) {
+        const double tilt = sin((double)x * pi / 180) * axial_tilt;  /* climate.py:201 */
+        ora_insolation_slice(radius, tilt, table);
+        ora_interpolate_insolation(verts, V, table, daily, radius);
+        for (int64_t v = 0; v < V; ++v) out[v] = out[v] + daily[v];
+    }
+    free(daily);
 }

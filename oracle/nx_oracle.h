@@ -121,6 +121,22 @@ void ora_knn(const double *pts, int64_t V, const double *q, int64_t nq, int k, d
 void ora_make_gray_array(int64_t npix, int k, const double *dists, const int64_t *nbrs,
                          const double *colors, int32_t *out);
 
+/* climate.py:416-446 (sample_insolation) over nsteps rotations: rotation
+ * starts at rot0 and grows by rot_step after each step (climate.py:468-469);
+ * each step adds max(cos|lat|, 0) * max(cos lon, 0) of the rotated, tilted
+ * vertex to the float32 arr as numba does (double sum, rounded to float). */
+void ora_sample_insolation(float *arr, const double *verts, int64_t V, double radius,
+                           double rot0, double rot_step, int nsteps, double tilt);
+/* climate.py:503-533 (calc_insolation_slice): out[181], index = latitude
+ * with negative latitudes at 181 + lat. */
+void ora_insolation_slice(double radius, double tilt, float *out);
+/* climate.py:547-573 (interpolate_insolation) */
+void ora_interpolate_insolation(const double *verts, int64_t V, const float *table, float *out,
+                                double radius);
+/* climate.py:577-590 (calc_yearly_insolation without the snapshot) */
+void ora_yearly_insolation(const double *verts, int64_t V, double radius, double axial_tilt,
+                           float *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -66,6 +66,10 @@ def lib():
         L.ora_make_ll_arr.argtypes = [_i64, _i64, _d, _p]
         L.ora_knn.argtypes = [_p, _i64, _p, _i64, ctypes.c_int, _p, _p]
         L.ora_make_gray_array.argtypes = [_i64, ctypes.c_int, _p, _p, _p, _p]
+        L.ora_sample_insolation.argtypes = [_p, _p, _i64, _d, _d, _d, ctypes.c_int, _d]
+        L.ora_insolation_slice.argtypes = [_d, _d, _p]
+        L.ora_interpolate_insolation.argtypes = [_p, _i64, _p, _p, _d]
+        L.ora_yearly_insolation.argtypes = [_p, _i64, _d, _d, _p]
         _lib = L
     return _lib
 
@@ -315,3 +319,42 @@ def make_gray_array(width, height, dists, nbrs, colors):
     out = np.empty(int(height) * int(width), np.int32)
     lib().ora_make_gray_array(out.size, int(k), _ptr(d), _ptr(n), _ptr(c), _ptr(out))
     return out.reshape(int(height), int(width))
+
+
+def sample_insolation(arr, verts, radius, rotation, tilt, nsteps=1, rot_step=1.0):
+    """climate.py:416-446 (nsteps rotations from `rotation`), float32 arr in place."""
+    assert arr.dtype == np.float32 and arr.flags.c_contiguous
+    v = _f64(verts)
+    lib().ora_sample_insolation(_ptr(arr), _ptr(v), v.shape[0], float(radius), float(rotation),
+                                float(rot_step), int(nsteps), float(tilt))
+    return arr
+
+
+def brute_daily_insolation(verts, radius, tilt):
+    """climate.py:449-486 without the unused altitude rescale."""
+    out = np.zeros(len(verts), np.float32)
+    return sample_insolation(out, verts, radius, -180.0, tilt, 360, 360.0 / 360)
+
+
+def insolation_slice(radius, tilt):
+    """climate.py:503-533 -> float32 [181]."""
+    out = np.empty(181, np.float32)
+    lib().ora_insolation_slice(float(radius), float(tilt), _ptr(out))
+    return out
+
+
+def interpolate_insolation(verts, table, radius):
+    """climate.py:547-573 -> float32 [V]."""
+    v = _f64(verts)
+    t = np.ascontiguousarray(table, dtype=np.float32)
+    out = np.empty(v.shape[0], np.float32)
+    lib().ora_interpolate_insolation(_ptr(v), v.shape[0], _ptr(t), _ptr(out), float(radius))
+    return out
+
+
+def yearly_insolation(verts, radius, axial_tilt):
+    """climate.py:577-590 -> float32 [V]."""
+    v = _f64(verts)
+    out = np.empty(v.shape[0], np.float32)
+    lib().ora_yearly_insolation(_ptr(v), v.shape[0], float(radius), float(axial_tilt), _ptr(out))
+    return out
