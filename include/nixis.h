@@ -58,6 +58,8 @@ int nx_ctx_destroy(nx_ctx *ctx);
                                 launches (default 32)                        */
 #define NX_OPT_PAIR_KERNEL 2 /* two-step erosion kernel variant, 0..6 (0 =
                                 default); a tuning knob, results identical   */
+#define NX_OPT_MULTI_KERNEL 3 /* four-step erosion kernel variant, 0..3 (0 =
+                               * 1024 threads, neighbour data read per step) */
 int nx_ctx_set_option(nx_ctx *ctx, int option, int64_t value);
 /* Synchronise `stream`, then return and clear the context's sticky kernel
  * error (NX_ERANGE: a noise coordinate outside |x| < 2^30 after the octave
@@ -293,6 +295,9 @@ typedef struct nx_tile_plan {
     int32_t max_load;    /* largest tile + ring 1 + ring 2, <= 2048 */
     int64_t off_start, off_cs, off_load, off_rbase, off_ring, off_cbase,
             off_lcol, off_lel;   /* byte offsets of the arrays in the image */
+    int32_t steps;       /* steps per launch the plan serves: 2 or 4 */
+    int32_t pad;
+    int64_t off_pref;    /* int32 [ntiles][steps + 1]: tile, tile + ring 1, ... */
 } nx_tile_plan;
 
 /* Build the plan from the host copies of col [6,V] and elen [6,V] (as
@@ -303,6 +308,16 @@ typedef struct nx_tile_plan {
  * use nx_erosion_run. */
 int nx_tile_plan_build(const int32_t *col, const double *elen, int64_t V,
                        void *out, int64_t cap, nx_tile_plan *plan);
+
+/* nx_tile_plan_build for `steps` (2 or 4) steps per launch: tiles carry
+ * `steps` rings (tile + rings at most 2048 vertices for 2 steps, 3072 for
+ * 4); cs counts the tile and rings 1..steps-1, load all of them.  A 4-step
+ * plan runs four steps per launch in nx_erosion_run_tiled / _rule (the
+ * neighbour data read once per launch serves four steps); the batched entry
+ * points take 2-step plans only. */
+int nx_tile_plan_build_steps(const int32_t *col, const double *elen, int64_t V,
+                             int steps, void *out, int64_t cap,
+                             nx_tile_plan *plan);
 
 /* nx_erosion_run with a tile plan (NULL plan: exactly nx_erosion_run).
  * Steps go two per launch; an odd last step runs alone. */

@@ -21,6 +21,7 @@ NX_EBUSY = -6
 NX_MAX_OCTAVES = 32
 NX_OPT_GRAPH_CHUNK = 1
 NX_OPT_PAIR_KERNEL = 2
+NX_OPT_MULTI_KERNEL = 3
 
 
 class NixisError(RuntimeError):
@@ -52,7 +53,8 @@ class NxTilePlan(ctypes.Structure):
                 ("ntiles", ctypes.c_int64), ("max_tile", ctypes.c_int32),
                 ("max_load", ctypes.c_int32)] + [
         (f"off_{n}", ctypes.c_int64)
-        for n in ("start", "cs", "load", "rbase", "ring", "cbase", "lcol", "lel")]
+        for n in ("start", "cs", "load", "rbase", "ring", "cbase", "lcol", "lel")] + [
+        ("steps", ctypes.c_int32), ("pad", ctypes.c_int32), ("off_pref", ctypes.c_int64)]
 
 
 class NxRescaleOpts(ctypes.Structure):
@@ -138,6 +140,8 @@ SIGNATURES = {
                                               _c_void_p, _c_void_p, _c_void_p, _c_void_p]),
     "nx_tile_plan_build": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p, _c_i64,
                                     ctypes.POINTER(NxTilePlan)]),
+    "nx_tile_plan_build_steps": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_int, _c_void_p, _c_i64,
+                                          ctypes.POINTER(NxTilePlan)]),
     "nx_erosion_run_tiled": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p, _c_void_p,
                                       ctypes.POINTER(NxTilePlan), _c_int, _c_double, _c_void_p]),
     "nx_erosion_run_rule": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p, _c_void_p,
@@ -236,6 +240,10 @@ class Context:
     def set_graph_chunk(self, steps):
         """Erosion steps per captured hipGraph (0: eager launches)."""
         check(lib().nx_ctx_set_option(self.handle, NX_OPT_GRAPH_CHUNK, int(steps)))
+
+    def set_multi_kernel(self, variant):
+        """Four-step erosion kernel variant 0..3 (0: default); same results."""
+        check(lib().nx_ctx_set_option(self.handle, NX_OPT_MULTI_KERNEL, int(variant)))
 
     def set_pair_kernel(self, variant):
         """Two-step erosion kernel variant 0..6 (0: default); same results."""

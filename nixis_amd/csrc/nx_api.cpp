@@ -147,6 +147,11 @@ int nx_ctx_set_option(nx_ctx *ctx, int option, int64_t value)
         ctx->clear_graphs();
         ctx->graph_chunk = (int)value;
         return NX_OK;
+    case NX_OPT_MULTI_KERNEL:
+        if (value < 0 || value > 3) return set_error(NX_EINVAL, "multi-step kernel variant %lld outside 0..3", (long long)value);
+        ctx->clear_graphs();
+        ctx->multi_variant = (int)value;
+        return NX_OK;
     case NX_OPT_PAIR_KERNEL:
         if (value < 0 || value > 6) return set_error(NX_EINVAL, "pair kernel variant %lld outside 0..6", (long long)value);
         ctx->clear_graphs();
@@ -393,7 +398,13 @@ int nx_ell_from_adjacency_device(nx_ctx *ctx, const int32_t *adj, int64_t V, con
 int nx_tile_plan_build(const int32_t *col, const double *elen, int64_t V, void *out, int64_t cap,
                        nx_tile_plan *plan)
 {
-    return tile_plan_build(col, elen, V, out, cap, plan);
+    return tile_plan_build(col, elen, V, 2, out, cap, plan);
+}
+
+int nx_tile_plan_build_steps(const int32_t *col, const double *elen, int64_t V, int steps,
+                             void *out, int64_t cap, nx_tile_plan *plan)
+{
+    return tile_plan_build(col, elen, V, steps, out, cap, plan);
 }
 
 int nx_erosion_store(nx_ctx *ctx, const void *state, int64_t V, double *h, double *w, double *s,

@@ -370,8 +370,9 @@ static int fill_desc(const nx_planet &q, BatchPlanetDev &d)
         if (q.plan) {
             const nx_tile_plan &pl = *q.plan;
             if (!pl.blob || ((uintptr_t)pl.blob & 255) || pl.V != q.V || pl.ntiles <= 0 ||
-                pl.ntiles > INT32_MAX || pl.max_tile > kTileMax || pl.max_load > kLoadMax)
-                return set_error(NX_EINVAL, "planet tile plan does not match V=%lld or the kernel limits",
+                pl.ntiles > INT32_MAX || pl.steps != 2 || pl.max_tile > kTileMax ||
+                pl.max_load > kLoadMax)
+                return set_error(NX_EINVAL, "planet tile plan is not a 2-step plan for V=%lld within the kernel limits",
                                  (long long)q.V);
             d.plan = plan_dev(pl);
             d.has_plan = 1;

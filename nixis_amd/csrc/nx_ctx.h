@@ -55,6 +55,7 @@ struct nx_ctx {
     hipStream_t capture_stream = nullptr;  // default priority, graph capture only
     int graph_chunk = 32;                  // erosion steps per captured graph; 0 = eager
     int pair_variant = 0;                  // two-step kernel variant (0 = default)
+    int multi_variant = 0;                 // four-step kernel variant (0 = default)
     std::map<GraphKey, hipGraphExec_t> graphs;
     std::mutex mu;
     StageSlot stage[kStageSlots];

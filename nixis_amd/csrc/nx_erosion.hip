@@ -105,6 +105,29 @@ static const PairVariant kPairVariants[] = {
 // erosion_iteration4 uses the default variant only.
 static const PairVariant kPairRule4 = {k_erode_pair<1024, true, 4>, 1024};
 
+// kMultiSteps steps per launch over a plan built for them (nx_pair_dev.h).
+template <int THREADS, bool KEEP, int RULE = 3>
+__global__ __launch_bounds__(THREADS) void k_erode_multi(const ErosionMeta *__restrict__ meta,
+                                                        double2 *__restrict__ hw0,
+                                                        double2 *__restrict__ hw1,
+                                                        double *__restrict__ s0,
+                                                        double *__restrict__ s1, TilePlanDev P,
+                                                        int j, double rain_mid, double rain_next)
+{
+    __shared__ double2 A[kLoadMaxMulti];
+    const int64_t tile = xcd_tile(blockIdx.x, P.ntiles);
+    if (tile >= P.ntiles) return;
+    erode_tile_multi<THREADS, kMultiSteps, KEEP, RULE>(A, hw0, hw1, s0, s1, (meta->buf + j) & 1, P,
+                                                      tile, rain_mid, rain_next);
+}
+
+// NX_OPT_MULTI_KERNEL variants; erosion_iteration4 uses the first.
+static const PairVariant kMultiVariants[] = {
+    {k_erode_multi<1024, false>, 1024}, {k_erode_multi<1024, true>, 1024},
+    {k_erode_multi<512, false>, 512},   {k_erode_multi<512, true>, 512},
+};
+static const PairVariant kMultiRule4 = {k_erode_multi<1024, false, 4>, 1024};
+
 // erosion.py:76-99 (erosion_iteration1): w[i] = r[i] + amt with amt moved
 // by 0.0005 towards each higher neighbour and away from each lower one, in
 // slot order (a pad slot points at the vertex itself and changes nothing,
@@ -263,13 +286,22 @@ int erosion_load(nx_ctx *ctx, void *state, int64_t V, const double *h, const dou
 // With a tile plan, steps go two per launch and an odd last step runs alone.
 // The last step's rain for a following step is `rain` unless `last`.
 static void launch_steps(const ErosionBufs &b, int64_t V, const int32_t *col, const double *elen,
-                         const nx_tile_plan *plan, int variant, int rule, int n, double rain,
-                         bool last, hipStream_t stream)
+                         const nx_tile_plan *plan, int variant, int mvariant, int rule, int n,
+                         double rain, bool last, hipStream_t stream)
 {
     const int64_t nblocks = (V + 255) / 256;
     const int64_t grid = ((nblocks + 7) / 8) * 8;
     int k = 0, j = 0;
-    if (plan) {
+    if (plan && plan->steps == kMultiSteps) {
+        const TilePlanDev P = plan_dev(*plan);
+        const int64_t tgrid = ((P.ntiles + 7) / 8) * 8;
+        const PairVariant &mv = rule == 4 ? kMultiRule4 : kMultiVariants[mvariant];
+        for (; k + kMultiSteps <= n; k += kMultiSteps, ++j) {
+            const double rain_next = (last && k + kMultiSteps == n) ? 0.0 : rain;
+            hipLaunchKernelGGL(mv.fn, dim3((unsigned)tgrid), dim3(mv.threads), 0, stream, b.meta,
+                               b.hw[0], b.hw[1], b.s[0], b.s[1], P, j, rain, rain_next);
+        }
+    } else if (plan) {
         const TilePlanDev P = plan_dev(*plan);
         const int64_t tgrid = ((P.ntiles + 7) / 8) * 8;
         for (; k + 2 <= n; k += 2, ++j) {
@@ -296,7 +328,10 @@ static int chunk_graph(nx_ctx *ctx, const ErosionBufs &b, void *state, int64_t V
                        const int32_t *col, const double *elen, const nx_tile_plan *plan, int rule,
                        int n, double rain, bool last, hipGraphExec_t *out)
 {
-    const uint64_t ptag = plan ? (uint64_t)(uintptr_t)plan->blob * 16 + (uint64_t)ctx->pair_variant : 0;
+    const uint64_t ptag =
+        plan ? (uint64_t)(uintptr_t)plan->blob * 16 +
+                   (uint64_t)(plan->steps == kMultiSteps ? 8 + ctx->multi_variant : ctx->pair_variant)
+             : 0;
     GraphKey key{state, V, col, elen, n, rain, last, ptag * 8 + (uint64_t)rule};
     std::lock_guard<std::mutex> lk(ctx->mu);
     auto it = ctx->graphs.find(key);
@@ -307,7 +342,8 @@ static int chunk_graph(nx_ctx *ctx, const ErosionBufs &b, void *state, int64_t V
     if (ctx->graphs.size() >= kMaxGraphs) ctx->clear_graphs();
     hipGraph_t g = nullptr;
     NX_HIP(hipStreamBeginCapture(ctx->capture_stream, hipStreamCaptureModeThreadLocal));
-    launch_steps(b, V, col, elen, plan, ctx->pair_variant, rule, n, rain, last, ctx->capture_stream);
+    launch_steps(b, V, col, elen, plan, ctx->pair_variant, ctx->multi_variant, rule, n, rain, last,
+                 ctx->capture_stream);
     const hipError_t le = hipGetLastError();
     const hipError_t ce = hipStreamEndCapture(ctx->capture_stream, &g);
     if (le != hipSuccess || ce != hipSuccess) {
@@ -332,9 +368,13 @@ static int check_plan(const nx_tile_plan *plan, int64_t V)
     if (plan->V != V || plan->ntiles <= 0 || plan->ntiles > INT32_MAX)
         return set_error(NX_EINVAL, "tile plan is for V=%lld, not %lld", (long long)plan->V,
                          (long long)V);
-    if (plan->max_tile > kTileMax || plan->max_load > kLoadMax)
+    if (plan->steps != 2 && plan->steps != kMultiSteps)
+        return set_error(NX_EINVAL, "tile plan for %d steps per launch (2 or %d)", plan->steps,
+                         kMultiSteps);
+    const int lmax = plan->steps == 2 ? kLoadMax : kLoadMaxMulti;
+    if (plan->max_tile > kTileMax || plan->max_load > lmax)
         return set_error(NX_EINVAL, "tile plan tiles exceed %d owned / %d loaded vertices",
-                         kTileMax, kLoadMax);
+                         kTileMax, lmax);
     return NX_OK;
 }
 
@@ -364,7 +404,8 @@ int erosion_run_rule(nx_ctx *ctx, void *state, int64_t V, const int32_t *col, co
                            b.hw[0], b.hw[1], V, rain);
     const int chunk = ctx->graph_chunk;
     if (chunk <= 0 || iters < 2) {
-        launch_steps(b, V, col, elen, plan, ctx->pair_variant, rule, iters, rain, true, stream);
+        launch_steps(b, V, col, elen, plan, ctx->pair_variant, ctx->multi_variant, rule, iters, rain,
+                     true, stream);
         NX_HIP(hipGetLastError());
         return NX_OK;
     }

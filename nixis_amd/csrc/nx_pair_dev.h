@@ -17,9 +17,11 @@
 
 namespace nx {
 
-constexpr int kTileMax = 1024;   // owned vertices per tile
-constexpr int kLoadMax = 2048;   // tile + both rings
-constexpr int kTileMin = 32;     // smallest tile the planner cuts to
+constexpr int kTileMax = 1024;      // owned vertices per tile
+constexpr int kLoadMax = 2048;      // tile + both rings (two steps per launch)
+constexpr int kTileMin = 32;        // smallest tile the planner cuts to
+constexpr int kMultiSteps = 4;      // steps per launch of the multi-step kernel
+constexpr int kLoadMaxMulti = 3072; // tile + its kMultiSteps rings
 
 // Device view of a plan image (pointers into nx_tile_plan::blob).
 struct TilePlanDev {
@@ -31,7 +33,9 @@ struct TilePlanDev {
     const int64_t *cbase;    // [ntiles + 1] prefix of cs
     const uint16_t *lcol;    // local neighbour index, [6][cs] per tile
     const double *lel;       // edge length, [6][cs] per tile
+    const int32_t *pref;     // [ntiles][steps + 1]: tile, tile + ring 1, ...
     int64_t ntiles;
+    int32_t steps;
 };
 
 inline TilePlanDev plan_dev(const nx_tile_plan &q)
@@ -46,7 +50,9 @@ inline TilePlanDev plan_dev(const nx_tile_plan &q)
     P.cbase = (const int64_t *)(b + q.off_cbase);
     P.lcol = (const uint16_t *)(b + q.off_lcol);
     P.lel = (const double *)(b + q.off_lel);
+    P.pref = (const int32_t *)(b + q.off_pref);
     P.ntiles = q.ntiles;
+    P.steps = q.steps;
     return P;
 }
 
@@ -137,6 +143,100 @@ __device__ __forceinline__ void erode_tile_pair(double2 *A, double2 *hw0, double
             erode_finish<RULE>(me.x, me.y, sed[m], S, W, rain_next, ho, wo, so);
             hw_dst[a + li] = make_double2(ho, wo);
             s_dst[a + li] = so;
+        }
+    }
+}
+
+// K steps of tile `tile` (a plan built for K steps), reading buffer p and
+// writing buffer p ^ 1.  h and w of the tile and its K rings are staged in
+// LDS; step j (1..K) runs on the first pref[K - j] local vertices (the tile
+// and rings 1..K-j) and leaves its h and w in LDS for step j + 1, whose
+// vertices need only those; step K writes the tile.  A vertex of ring m is
+// computed by every tile that holds it, with the same inputs and arithmetic,
+// so the result equals K one-step launches bit for bit.  KEEP holds the
+// neighbour indices and edge lengths read in step 1 in registers for the
+// later steps (otherwise they are read again, from cache).
+template <int THREADS, int K, bool KEEP, int RULE = 3>
+__device__ __forceinline__ void erode_tile_multi(double2 *A, double2 *hw0, double2 *hw1,
+                                                 double *s0, double *s1, int p,
+                                                 const TilePlanDev &P, int64_t tile,
+                                                 double rain_mid, double rain_next)
+{
+    constexpr int kPer = (kLoadMaxMulti + THREADS - 1) / THREADS;
+    constexpr int kKeep = KEEP ? kPer : 1;
+    const auto hw_src = gmem(p ? hw1 : hw0);
+    const auto s_src = gmem(p ? s1 : s0);
+    const auto hw_dst = gmem(p ? hw0 : hw1);
+    const auto s_dst = gmem(p ? s0 : s1);
+    const int64_t a = gmem(P.start)[tile];
+    const int T = (int)(gmem(P.start)[tile + 1] - a);
+    const auto pref = gmem(P.pref) + tile * (K + 1);
+    const int cs = pref[K - 1], L = pref[K];
+    const auto ring = gmem(P.ring) + gmem(P.rbase)[tile];
+    const int64_t cb = gmem(P.cbase)[tile];
+    const auto lcol = gmem(P.lcol) + 6 * cb;
+    const auto lel = gmem(P.lel) + 6 * cb;
+
+    for (int li = threadIdx.x; li < L; li += THREADS)
+        A[li] = hw_src[li < T ? a + li : ring[li - T]];
+    double sed[kPer];
+#pragma unroll
+    for (int m = 0; m < kPer; ++m) {
+        const int li = threadIdx.x + m * THREADS;
+        sed[m] = li < cs ? s_src[li < T ? a + li : ring[li - T]] : 0.0;
+    }
+    __syncthreads();
+
+    int nbr[kKeep][6];
+    double len[kKeep][6];
+    double2 res[kPer];
+#pragma unroll
+    for (int j = 1; j <= K; ++j) {
+        const int n = pref[K - j];           // vertices of step j
+        const double rain = j < K ? rain_mid : rain_next;
+#pragma unroll
+        for (int m = 0; m < kPer; ++m) {
+            const int li = threadIdx.x + m * THREADS;
+            if (li < n) {
+                const double2 me = A[li];
+                double S = sed[m], W = me.y;
+#pragma unroll
+                for (int q = 0; q < 6; ++q) {
+                    int nb_i;
+                    double d;
+                    if (KEEP && j > 1) {
+                        nb_i = nbr[KEEP ? m : 0][q];
+                        d = len[KEEP ? m : 0][q];
+                    } else {
+                        nb_i = lcol[q * cs + li];
+                        d = lel[q * cs + li];
+                        if (KEEP) {
+                            nbr[KEEP ? m : 0][q] = nb_i;
+                            len[KEEP ? m : 0][q] = d;
+                        }
+                    }
+                    const double2 nb = A[nb_i];
+                    erode_accumulate<RULE>(me.x, nb.x, nb.y, d, S, W);
+                }
+                double ho, wo, so;
+                erode_finish<RULE>(me.x, me.y, sed[m], S, W, rain, ho, wo, so);
+                sed[m] = so;
+                if (j < K) {
+                    res[m] = make_double2(ho, wo);
+                } else {
+                    hw_dst[a + li] = make_double2(ho, wo);
+                    s_dst[a + li] = so;
+                }
+            }
+        }
+        if (j < K) {
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < kPer; ++m) {
+                const int li = threadIdx.x + m * THREADS;
+                if (li < n) A[li] = res[m];
+            }
+            __syncthreads();
         }
     }
 }

@@ -28,7 +28,7 @@ def calc_distance(v0, v1):
 
 class ErosionGraph:
     """Device neighbour table: col int32 [6,V] and FP64 elen [6,V],
-    plus the tile plan that lets erosion run two steps per launch.
+    plus the tile plan that lets erosion run several steps per launch.
 
     Built on the host from a reference-style neighbour table ([V,6] int32,
     -1 padded) and FP64 vertex coordinates.  `order` renumbers the vertices
@@ -36,11 +36,12 @@ class ErosionGraph:
     (caller order) or an explicit order[new] = old.  Fields passed to
     ErosionState use the device numbering (``permute``/``unpermute``).
     With `tiled`, the tile plan is built when the graph cuts into tiles
-    (nx_tile_plan_build); otherwise, or for graphs without locality, every
-    step is its own launch.
+    (nx_tile_plan_build_steps); otherwise, or for graphs without locality,
+    every step is its own launch.  `steps` (2 or 4) is the number of steps
+    one launch covers: a 4-step plan carries four rings per tile.
     """
 
-    def __init__(self, verts, neighbors, order="sfc", device=None, tiled=True):
+    def __init__(self, verts, neighbors, order="sfc", device=None, tiled=True, steps=2):
         import torch
         verts = np.ascontiguousarray(verts, dtype=np.float64)
         nb = np.ascontiguousarray(neighbors, dtype=np.int32)
@@ -67,11 +68,12 @@ class ErosionGraph:
         self.device = self.col.device
         self.plan = None
         self._plan_blob = None
+        self.steps = int(steps)
         if tiled:
             self._build_plan(col, elen)
 
     @classmethod
-    def from_device(cls, verts, neighbors, order="sfc", tiled=True, stream=None):
+    def from_device(cls, verts, neighbors, order="sfc", tiled=True, stream=None, steps=2):
         """The same graph from device tensors (verts float64 [V,3], neighbours
         int32 [V,6]), built on the device (nx_sfc_order_device,
         nx_ell_from_adjacency_device); the tile plan is cut on the host from
@@ -103,6 +105,7 @@ class ErosionGraph:
         self.order = order.cpu().numpy() if order is not None else None
         self.plan = None
         self._plan_blob = None
+        self.steps = int(steps)
         if tiled:
             self._build_plan(col.cpu().numpy(), elen.cpu().numpy())
         return self
@@ -111,14 +114,14 @@ class ErosionGraph:
         import torch
         L = _lib.lib()
         plan = _lib.NxTilePlan()
-        rc = L.nx_tile_plan_build(_lib.ptr(col), _lib.ptr(elen), self.V, None, 0,
-                                  ctypes.byref(plan))
+        rc = L.nx_tile_plan_build_steps(_lib.ptr(col), _lib.ptr(elen), self.V, self.steps, None,
+                                        0, ctypes.byref(plan))
         if rc == _lib.NX_ERANGE:
             return
         _lib.check(rc)
         img = np.zeros(plan.bytes, dtype=np.uint8)
-        _lib.check(L.nx_tile_plan_build(_lib.ptr(col), _lib.ptr(elen), self.V, _lib.ptr(img),
-                                        plan.bytes, ctypes.byref(plan)))
+        _lib.check(L.nx_tile_plan_build_steps(_lib.ptr(col), _lib.ptr(elen), self.V, self.steps,
+                                              _lib.ptr(img), plan.bytes, ctypes.byref(plan)))
         blob = torch.from_numpy(img).to(self.device)
         if blob.data_ptr() % 256:
             raise RuntimeError("tile plan copy is not 256-byte aligned")

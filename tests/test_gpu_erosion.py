@@ -224,3 +224,56 @@ def test_pair_launches_match_single_steps(oracle, d, iters, chunk):
         assert (x == y).all() and (x == r).all()
     print(f"\nd={d} iters={iters} chunk={chunk}: tiles {gt.plan.ntiles}, "
           f"pair vs single {_rel(a[0], b[0]):.1e}, vs fp64 {_rel(a[0], href):.1e}")
+
+
+@pytest.mark.parametrize("rule", [3, 4])
+def test_four_step_plans_match_oracle(oracle, rule):
+    """Four steps per launch (tile + four rings in LDS) with the remainder
+    as single steps: 1..9 steps equal the oracle bit for bit, for every
+    kernel variant."""
+    pts, nb, h = _planet(32)
+    V = h.size
+    rng = np.random.default_rng(rule)
+    w0 = rng.uniform(0.0, 0.02, V)
+    s0 = rng.uniform(0.0, 0.001, V)
+    g = erosion.ErosionGraph(pts, nb, steps=4)
+    assert g.plan is not None and g.plan.steps == 4
+    ref_step = oracle.erosion_iteration3 if rule == 3 else oracle.erosion_iteration4
+    for variant in range(4):
+        g_ctx = _lib.Context.get(g.device)
+        g_ctx.set_multi_kernel(variant)
+        for n in (1, 4, 5, 9):
+            print("rule", rule, "variant", variant, "steps", n, flush=True)
+            st = erosion.ErosionState(V)
+            st.load(*[torch.from_numpy(g.permute(a)).cuda() for a in (h, w0, s0)])
+            st.run(g, n, 0.0, rule=rule)
+            out = [torch.empty(V, dtype=torch.float64, device="cuda") for _ in range(3)]
+            st.store(*out)
+            st.ctx.sync_check()
+            hr, wr, sr = h.copy(), w0.copy(), s0.copy()
+            for _ in range(n):
+                ref_step(pts, nb, hr, wr, sr)
+            for t, r in zip(out, (hr, wr, sr)):
+                assert np.array_equal(g.unpermute(t.cpu().numpy()), r)
+    _lib.Context.get(g.device).set_multi_kernel(0)
+
+
+def test_four_step_launches_equal_two_step_and_single_at_scale():
+    """d=1000 (10,000,002 vertices), 23 steps with rain: four-step plan,
+    two-step plan and one step per launch give the same fields bit for bit."""
+    pts, nb, h = _planet(1000, n_oct=8)
+    outs = []
+    for kw in ({"steps": 4}, {"steps": 2}, {"tiled": False}):
+        g = erosion.ErosionGraph(pts, nb, **kw)
+        st = erosion.ErosionState(g.V)
+        st.load(torch.from_numpy(g.permute(h)).cuda())
+        st.run(g, 23, 0.3 / 320)
+        o = [torch.empty(g.V, dtype=torch.float64, device="cuda") for _ in range(3)]
+        st.store(*o)
+        st.ctx.sync_check()
+        outs.append([g.unpermute(t.cpu().numpy()) for t in o])
+        print("plan", kw, "done", flush=True)
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a, b)
+    for a, b in zip(outs[0], outs[2]):
+        assert np.array_equal(a, b)

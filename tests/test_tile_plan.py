@@ -1,11 +1,12 @@
-"""The tile plan for two erosion steps per launch (nx_tile_plan_build, host).
+"""The tile plan for K erosion steps per launch (nx_tile_plan_build_steps,
+host), K = 2 and 4.
 
-Checked on the CPU: tiles cover the vertices in order, each tile's rings are
-exactly its outside neighbours and their outside neighbours, every local
+Checked on the CPU: tiles cover the vertices in order, each tile's rings
+are exactly its successive layers of outside neighbours, every local
 neighbour index names the same vertex as the global table, edge lengths are
-carried over, and two Jacobi steps computed from the plan's local view (tile
-+ ring 1 first, then the tile) equal two steps of the FP64 oracle bit for
-bit.  A numbering without locality is refused.
+carried over, and K Jacobi steps computed from the plan's local view (step j
+on the tile and rings 1..K-j) equal K steps of the FP64 oracle bit for bit.
+A numbering without locality is refused.
 """
 import ctypes
 
@@ -28,15 +29,17 @@ def _ell(d, order="sfc"):
     return pts, nb, o, col, elen
 
 
-def _plan(col, elen):
+def _plan(col, elen, steps=2):
     V = col.shape[1]
     L = _lib.lib()
     q = _lib.NxTilePlan()
-    rc = L.nx_tile_plan_build(_lib.ptr(col), _lib.ptr(elen), V, None, 0, ctypes.byref(q))
+    rc = L.nx_tile_plan_build_steps(_lib.ptr(col), _lib.ptr(elen), V, steps, None, 0,
+                                    ctypes.byref(q))
     _lib.check(rc)
     img = np.zeros(q.bytes, np.uint8)
-    _lib.check(L.nx_tile_plan_build(_lib.ptr(col), _lib.ptr(elen), V, _lib.ptr(img), q.bytes,
-                                    ctypes.byref(q)))
+    _lib.check(L.nx_tile_plan_build_steps(_lib.ptr(col), _lib.ptr(elen), V, steps, _lib.ptr(img),
+                                          q.bytes, ctypes.byref(q)))
+    assert q.steps == steps
     nt = q.ntiles
 
     def arr(off, dt, n):
@@ -50,11 +53,12 @@ def _plan(col, elen):
     cbase = arr(q.off_cbase, np.int64, nt + 1)
     lcol = arr(q.off_lcol, np.uint16, 6 * int(cbase[-1]))
     lel = arr(q.off_lel, np.float64, 6 * int(cbase[-1]))
-    return q, start, cs, load, rbase, ring, cbase, lcol, lel
+    pref = arr(q.off_pref, np.int32, (steps + 1) * nt).reshape(nt, steps + 1)
+    return q, start, cs, load, rbase, ring, cbase, lcol, lel, pref
 
 
-def _tiles(col, elen):
-    q, start, cs, load, rbase, ring, cbase, lcol, lel = _plan(col, elen)
+def _tiles(col, elen, steps=2):
+    q, start, cs, load, rbase, ring, cbase, lcol, lel, pref = _plan(col, elen, steps)
     for t in range(q.ntiles):
         a, b = int(start[t]), int(start[t + 1])
         T = b - a
@@ -63,7 +67,8 @@ def _tiles(col, elen):
         c = int(cs[t])
         lc = lcol[6 * cbase[t]:6 * cbase[t + 1]].reshape(6, c)
         le = lel[6 * cbase[t]:6 * cbase[t + 1]].reshape(6, c)
-        yield q, a, b, T, c, int(load[t]), glob, lc, le
+        assert pref[t, 0] == T and pref[t, -2] == c and pref[t, -1] == int(load[t])
+        yield q, a, b, T, c, int(load[t]), glob, lc, le, pref[t]
 
 
 @pytest.mark.parametrize("d", [2, 8, 40])
@@ -71,7 +76,7 @@ def test_plan_covers_and_rings_are_exact(d):
     pts, nb, o, col, elen = _ell(d)
     V = col.shape[1]
     nxt = 0
-    for q, a, b, T, c, L, glob, lc, le in _tiles(col, elen):
+    for q, a, b, T, c, L, glob, lc, le, _ in _tiles(col, elen):
         assert a == nxt and 0 < T <= 1024 and L <= 2048 and q.max_tile <= 1024
         nxt = b
         inside = set(range(a, b))
@@ -82,6 +87,28 @@ def test_plan_covers_and_rings_are_exact(d):
         assert (glob[lc] == col[:, glob[:c]]).all()
         assert (le == elen[:, glob[:c]]).all()
     assert nxt == V
+
+
+def _step_rows(n_rows, Hs, Ws, Ss, lc, d):
+    """One step of the first n_rows local vertices (erosion.py:209-267)."""
+    Hn, Wn, Sn = Hs[:n_rows].copy(), Ws[:n_rows].copy(), Ss[:n_rows].copy()
+    for i in range(n_rows):
+        S, W = Ss[i], Ws[i]
+        for k in range(6):
+            n = lc[k, i]
+            if Hs[n] > Hs[i]:
+                S += SOL * Ws[n]
+                W += Ws[n] * d[k, i]
+            elif Hs[n] < Hs[i]:
+                S -= SOL * Ws[n]
+                W -= Ws[n] * d[k, i]
+        Hn[i] = Hs[i] - S
+        Sn[i] = Ss[i] + S
+        Wn[i] = Ws[i] + (W - W * EVAP)
+        if Sn[i] > CAP * Wn[i]:
+            Hn[i] += Sn[i] - CAP * Wn[i]
+            Sn[i] -= Sn[i] - CAP * Wn[i]
+    return Hn, Wn, Sn
 
 
 def _two_steps_local(h, w, s, glob, c, T, lc, d):
@@ -130,9 +157,51 @@ def test_two_steps_from_the_plan_equal_the_oracle(oracle):
     hr, wr, sr = h.copy(), w.copy(), s.copy()
     for _ in range(2):
         oracle.erosion_iteration3(P, nbd, hr, wr, sr)
-    for q, a, b, T, c, L, glob, lc, le in _tiles(col, elen):
+    for q, a, b, T, c, L, glob, lc, le, _ in _tiles(col, elen):
         H2, W2, S2 = _two_steps_local(h, w, s, glob, c, T, lc, le)
         assert (H2 == hr[a:b]).all() and (W2 == wr[a:b]).all() and (S2 == sr[a:b]).all()
+
+
+@pytest.mark.parametrize("d", [3, 12, 40])
+def test_four_step_rings_are_exact_layers(d):
+    pts, nb, o, col, elen = _ell(d)
+    V = col.shape[1]
+    nxt = 0
+    for q, a, b, T, c, L, glob, lc, le, pref in _tiles(col, elen, 4):
+        assert a == nxt and L <= 3072
+        nxt = b
+        seen = set(range(a, b))
+        layer = list(range(a, b))
+        for m in range(4):
+            nxt_layer = sorted(set(col[:, layer].ravel()) - seen) if layer else []
+            assert list(glob[pref[m]:pref[m + 1]]) == nxt_layer
+            seen |= set(nxt_layer)
+            layer = nxt_layer
+        assert (glob[lc] == col[:, glob[:c]]).all() and (le == elen[:, glob[:c]]).all()
+    assert nxt == V
+
+
+def test_four_steps_from_the_plan_equal_the_oracle(oracle):
+    d = 10
+    pts, nb, o, col, elen = _ell(d)
+    V = col.shape[1]
+    P = pts[o]
+    nbd = col.T.copy()
+    nbd[nbd == np.arange(V)[:, None]] = -1
+    rng = np.random.default_rng(4)
+    h = rng.uniform(-9000, 9000, V)
+    w = rng.uniform(0, 1, V)
+    s = rng.uniform(0, 0.01, V)
+    hr, wr, sr = h.copy(), w.copy(), s.copy()
+    for _ in range(4):
+        oracle.erosion_iteration3(P, nbd, hr, wr, sr)
+    for q, a, b, T, c, L, glob, lc, le, pref in _tiles(col, elen, 4):
+        H, W, S = h[glob].copy(), w[glob].copy(), s[glob].copy()
+        for j in range(1, 5):
+            n = int(pref[4 - j])
+            Hn, Wn, Sn = _step_rows(n, H, W, S, lc, le)
+            H[:n], W[:n], S[:n] = Hn, Wn, Sn
+        assert (H[:T] == hr[a:b]).all() and (W[:T] == wr[a:b]).all() and (S[:T] == sr[a:b]).all()
 
 
 def test_numbering_without_locality_is_refused():
