@@ -60,7 +60,9 @@ inline TilePlanDev plan_dev(const nx_tile_plan &q)
 // Two steps of tile `tile`, reading buffer p and writing buffer p ^ 1.
 // THREADS is the block size; KEEP holds the tile's neighbour indices and
 // edge lengths in registers between the steps (otherwise the second step
-// reads them again).  A is the block's LDS staging area.
+// reads them again); the kept vertices' data is loaded before the first
+// barrier, while the field gathers are in flight.  A is the block's LDS
+// staging area.
 template <int THREADS, bool KEEP, int RULE = 3>
 __device__ __forceinline__ void erode_tile_pair(double2 *A, double2 *hw0, double2 *hw1, double *s0,
                                                 double *s1, int p, const TilePlanDev &P,
@@ -89,11 +91,26 @@ __device__ __forceinline__ void erode_tile_pair(double2 *A, double2 *hw0, double
         const int li = threadIdx.x + m * THREADS;
         sed[m] = li < cs ? s_src[li < T ? a + li : ring[li - T]] : 0.0;
     }
+    // The kept vertices' neighbour slots and lengths are loaded now, while
+    // the gathers above are in flight, rather than after the barrier.
+    int nbr[kKeep][6];
+    double len[kKeep][6];
+    if (KEEP) {
+#pragma unroll
+        for (int m = 0; m < kKeep; ++m) {
+            const int li = threadIdx.x + m * THREADS;
+            if (li < cs && m < kOwnPer) {
+#pragma unroll
+                for (int q = 0; q < 6; ++q) {
+                    nbr[m][q] = lcol[q * cs + li];
+                    len[m][q] = lel[q * cs + li];
+                }
+            }
+        }
+    }
     __syncthreads();
 
     double2 res[kLoadPer];
-    int nbr[kKeep][6];
-    double len[kKeep][6];
 #pragma unroll
     for (int m = 0; m < kLoadPer; ++m) {
         const int li = threadIdx.x + m * THREADS;
@@ -103,12 +120,9 @@ __device__ __forceinline__ void erode_tile_pair(double2 *A, double2 *hw0, double
             double S = sed[m], W = me.y;
 #pragma unroll
             for (int q = 0; q < 6; ++q) {
-                const int n = lcol[q * cs + li];
-                const double d = lel[q * cs + li];
-                if (KEEP && m < kOwnPer) {
-                    nbr[m < kKeep ? m : 0][q] = n;
-                    len[m < kKeep ? m : 0][q] = d;
-                }
+                const bool kept = KEEP && m < kOwnPer;
+                const int n = kept ? nbr[m < kKeep ? m : 0][q] : lcol[q * cs + li];
+                const double d = kept ? len[m < kKeep ? m : 0][q] : lel[q * cs + li];
                 const double2 nb = A[n];
                 erode_accumulate<RULE>(me.x, nb.x, nb.y, d, S, W);
             }
