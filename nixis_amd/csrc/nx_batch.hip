@@ -281,6 +281,7 @@ __global__ __launch_bounds__(1024) void k_erode_batch_pair(const BatchPlanetDev 
                                                            int64_t k)
 {
     __shared__ double2 A[kLoadMax];
+    __shared__ double2 B[kLoadMax];
     const int64_t b = xcd_tile(blockIdx.x, nt_total);
     if (b >= nt_total) return;
     // The planet is the same for the whole block: keep its descriptor in
@@ -295,7 +296,8 @@ __global__ __launch_bounds__(1024) void k_erode_batch_pair(const BatchPlanetDev 
     const int p = (D.meta->buf + (int)(k & 1)) & 1;
     const double rain_mid = t + 1 < D.total_iters ? D.rain : 0.0;
     const double rain_next = t + 2 < D.total_iters ? D.rain : 0.0;
-    erode_tile_pair<1024, true>(A, hw0, hw1, s0, s1, p, plan, b - prefix[r], rain_mid, rain_next);
+    erode_tile_pair<1024, true>(A, B, hw0, hw1, s0, s1, p, plan, b - prefix[r], rain_mid,
+                                rain_next);
 }
 
 // After both phases: planet p took steps[p] steps in launches[p] launches.

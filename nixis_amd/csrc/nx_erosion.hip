@@ -84,10 +84,11 @@ __global__ __launch_bounds__(THREADS) void k_erode_pair(const ErosionMeta *__res
                                                        int j, double rain_mid, double rain_next)
 {
     __shared__ double2 A[kLoadMax];
+    __shared__ double2 B[THREADS == 1024 ? kLoadMax : 1];
     const int64_t tile = xcd_tile(blockIdx.x, P.ntiles);
     if (tile >= P.ntiles) return;
-    erode_tile_pair<THREADS, KEEP, RULE>(A, hw0, hw1, s0, s1, (meta->buf + j) & 1, P, tile,
-                                         rain_mid, rain_next);
+    erode_tile_pair<THREADS, KEEP, RULE>(A, THREADS == 1024 ? B : nullptr, hw0, hw1, s0, s1,
+                                         (meta->buf + j) & 1, P, tile, rain_mid, rain_next);
 }
 
 // Kernel variants (NX_OPT_PAIR_KERNEL): block size and register use.

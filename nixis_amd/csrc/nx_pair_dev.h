@@ -62,11 +62,13 @@ inline TilePlanDev plan_dev(const nx_tile_plan &q)
 // edge lengths in registers between the steps (otherwise the second step
 // reads them again); the kept vertices' data is loaded before the first
 // barrier, while the field gathers are in flight.  A is the block's LDS
-// staging area.
+// staging area; B, when given, receives step one's h and w (one barrier
+// fewer than writing them back into A).
 template <int THREADS, bool KEEP, int RULE = 3>
-__device__ __forceinline__ void erode_tile_pair(double2 *A, double2 *hw0, double2 *hw1, double *s0,
-                                                double *s1, int p, const TilePlanDev &P,
-                                                int64_t tile, double rain_mid, double rain_next)
+__device__ __forceinline__ void erode_tile_pair(double2 *A, double2 *B, double2 *hw0, double2 *hw1,
+                                                double *s0, double *s1, int p,
+                                                const TilePlanDev &P, int64_t tile,
+                                                double rain_mid, double rain_next)
 {
     constexpr int kLoadPer = (kLoadMax + THREADS - 1) / THREADS;
     constexpr int kOwnPer = (kTileMax + THREADS - 1) / THREADS;
@@ -128,29 +130,35 @@ __device__ __forceinline__ void erode_tile_pair(double2 *A, double2 *hw0, double
             }
             double ho, wo, so;
             erode_finish<RULE>(me.x, me.y, sed[m], S, W, rain_mid, ho, wo, so);
-            res[m] = make_double2(ho, wo);
+            if (B)
+                B[li] = make_double2(ho, wo);
+            else
+                res[m] = make_double2(ho, wo);
             sed[m] = so;
         }
     }
-    __syncthreads();
+    if (!B) {
+        __syncthreads();
 #pragma unroll
-    for (int m = 0; m < kLoadPer; ++m) {
-        const int li = threadIdx.x + m * THREADS;
-        if (li < cs) A[li] = res[m];
+        for (int m = 0; m < kLoadPer; ++m) {
+            const int li = threadIdx.x + m * THREADS;
+            if (li < cs) A[li] = res[m];
+        }
     }
     __syncthreads();
+    const double2 *A2 = B ? B : A;
 
 #pragma unroll
     for (int m = 0; m < kOwnPer; ++m) {
         const int li = threadIdx.x + m * THREADS;
         if (li < T) {
-            const double2 me = A[li];
+            const double2 me = A2[li];
             double S = sed[m], W = me.y;
 #pragma unroll
             for (int q = 0; q < 6; ++q) {
                 const int n = KEEP ? nbr[m < kKeep ? m : 0][q] : lcol[q * cs + li];
                 const double d = KEEP ? len[m < kKeep ? m : 0][q] : lel[q * cs + li];
-                const double2 nb = A[n];
+                const double2 nb = A2[n];
                 erode_accumulate<RULE>(me.x, nb.x, nb.y, d, S, W);
             }
             double ho, wo, so;
