@@ -54,6 +54,7 @@ def parse():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-export", action="store_true")
+    ap.add_argument("--no-climate", action="store_true")
     ap.add_argument("--export-d", type=int, default=2500)
     ap.add_argument("--served-jobs", type=int, default=120)
     ap.add_argument("--served-load", type=float, default=0.8,
@@ -311,6 +312,31 @@ def bench_export(args):
     return out
 
 
+def bench_climate(args):
+    """climate.py:449-590 on the device for a d=1000 planet (10,000,002
+    vertices): a year of insolation (360 daily latitude tables, each 181
+    latitudes x 360 rotation steps, then 360 interpolations per vertex) and
+    one day of 360 rotation steps per vertex (brute_daily_insolation)."""
+    import torch
+    from nixis_amd import climate, util
+    pts, _ = util.create_mesh_device(1000, cells=False)
+    V = pts.shape[0]
+    ws = climate._ws(pts.device)
+    out = torch.empty(V, dtype=torch.float32, device=pts.device)
+    climate.yearly_insolation_device(pts, 1.0, 23.44, out=out, workspace=ws)
+    y_ms = _events_ms(lambda: climate.yearly_insolation_device(pts, 1.0, 23.44, out=out,
+                                                              workspace=ws))
+    out.zero_()
+    climate.sample_insolation_device(out, pts, 1.0, -180.0, 23.44, 360, 1.0, workspace=ws)
+    out.zero_()
+    d_ms = _events_ms(lambda: climate.sample_insolation_device(out, pts, 1.0, -180.0, 23.44, 360,
+                                                               1.0, workspace=ws))
+    log(f"[climate] V={V}: year {y_ms:.2f} ms, day of 360 rotations {d_ms:.2f} ms")
+    return {"workload": f"insolation on a d=1000 icosphere ({V} vertices), tilt 23.44",
+            "yearly_ms": round(y_ms, 3), "daily_360_rotations_ms": round(d_ms, 3),
+            "Gvertex_rotations_per_s": round(V * 360 / (d_ms * 1e-3) / 1e9, 2)}
+
+
 def cpu_erosion_baseline(er, iters=4):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -500,6 +526,8 @@ def main():
         line["served"] = bench_served(args)
     if not args.no_export:
         line["export"] = bench_export(args)
+    if not args.no_climate:
+        line["climate"] = bench_climate(args)
     print(json.dumps(line), flush=True)
 
 
