@@ -272,8 +272,11 @@ NX_ND float noise3(double x, double y, double z, const NoiseTables &T, bool chec
     if (check)
         bad = bad | !((fabs(xs) < kNoiseCoordLimit) & (fabs(ys) < kNoiseCoordLimit) &
                       (fabs(zs) < kNoiseCoordLimit));
-    const int xsb = (int)floor(xs), ysb = (int)floor(ys), zsb = (int)floor(zs);
-    const double fx = (double)xsb, fy = (double)ysb, fz = (double)zsb;
+    // floor(xs) is the integer the reference's int(floor(xs)) holds, as a
+    // double (exact inside the checked domain |xs| < 2^30), so it serves the
+    // FP64 arithmetic directly and the int only the hashing.
+    const double fx = floor(xs), fy = floor(ys), fz = floor(zs);
+    const int xsb = (int)fx, ysb = (int)fy, zsb = (int)fz;
     const double sq = (fx + fy + fz) * SQUISH;
     const double xins = xs - fx, yins = ys - fy, zins = zs - fz;
     const double in_sum = xins + yins + zins;
