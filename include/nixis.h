@@ -276,11 +276,12 @@ int nx_erosion_load(nx_ctx *ctx, void *state, int64_t V, const double *h,
 int nx_erosion_run(nx_ctx *ctx, void *state, int64_t V, const int32_t *col,
                    const double *elen, int iters, double rain, void *stream);
 
-/* Tile plan for running erosion steps two per launch.  The vertices (in
- * device order) are cut into contiguous tiles of at most max_tile vertices
- * whose neighbours (ring 1) and neighbours' neighbours (ring 2) bring the
- * total to at most max_load; each tile carries its rings' global indices
- * and, for tile + ring 1, the neighbour slots renumbered into that local
+/* Tile plan for running erosion steps several per launch (2, or 4 with
+ * nx_tile_plan_build_steps).  The vertices (in device order) are cut into
+ * contiguous tiles of at most max_tile vertices whose successive layers of
+ * outside neighbours (ring 1, ring 2, ...) bring the total to at most
+ * max_load; each tile carries its rings' global indices and, for the tile
+ * and all rings but the last, the neighbour slots renumbered into that local
  * set plus their edge lengths.  The plan is a byte image: nx_tile_plan_build
  * fills it on the host, the caller copies it to device memory (256-byte
  * aligned) and stores that address in `blob`.  The same results as
@@ -292,7 +293,7 @@ typedef struct nx_tile_plan {
     int64_t V;
     int64_t ntiles;
     int32_t max_tile;    /* largest tile (owned vertices), <= 1024 */
-    int32_t max_load;    /* largest tile + ring 1 + ring 2, <= 2048 */
+    int32_t max_load;    /* largest tile + rings, <= 2048 (2 steps), 3072 (4) */
     int64_t off_start, off_cs, off_load, off_rbase, off_ring, off_cbase,
             off_lcol, off_lel;   /* byte offsets of the arrays in the image */
     int32_t steps;       /* steps per launch the plan serves: 2 or 4 */
@@ -313,14 +314,15 @@ int nx_tile_plan_build(const int32_t *col, const double *elen, int64_t V,
  * `steps` rings (tile + rings at most 2048 vertices for 2 steps, 3072 for
  * 4); cs counts the tile and rings 1..steps-1, load all of them.  A 4-step
  * plan runs four steps per launch in nx_erosion_run_tiled / _rule (the
- * neighbour data read once per launch serves four steps); the batched entry
- * points take 2-step plans only. */
+ * neighbour data read once per launch serves four steps; on an icosphere
+ * with 1024-vertex tiles no faster than 2 steps, as the rings grow); the
+ * batched entry points take 2-step plans only. */
 int nx_tile_plan_build_steps(const int32_t *col, const double *elen, int64_t V,
                              int steps, void *out, int64_t cap,
                              nx_tile_plan *plan);
 
 /* nx_erosion_run with a tile plan (NULL plan: exactly nx_erosion_run).
- * Steps go two per launch; an odd last step runs alone. */
+ * Steps go plan->steps per launch; the remainder runs one per launch. */
 int nx_erosion_run_tiled(nx_ctx *ctx, void *state, int64_t V,
                          const int32_t *col, const double *elen,
                          const nx_tile_plan *plan, int iters, double rain,
