@@ -33,14 +33,13 @@
 namespace nx {
 
 int pack_tables(const int32_t *perm, const int32_t *pgi, NoiseTablesPacked &p);
-int octave_weights(const double *rough, const double *strength, double radius, int n_oct,
-                   double *r_out, double *w_out, int *check_oct);
+int octave_plan(const double *rough, const double *strength, double radius, int n_oct,
+                OctavePlan &p);
 
 struct BatchPlanetDev {
     const double *verts;
     double *elev;
     int64_t V;
-    int32_t n_oct;
     int32_t accumulate;
     double radius;
     double lower, upper;
@@ -56,9 +55,7 @@ struct BatchPlanetDev {
     int64_t nblocks;  // ceil(V / 256)
     TilePlanDev plan;
     int32_t has_plan;
-    double rough[NX_MAX_OCTAVES];
-    double weight[NX_MAX_OCTAVES];    // 0.5 * strength * radius
-    int32_t check_oct;                 // octave with the largest |rough|
+    OctavePlan oct;
     NoiseTablesPacked tables;
 };
 
@@ -131,11 +128,12 @@ __device__ __forceinline__ void load_seed_tables(NoiseTables &T, const NoiseTabl
         T.perm[i] = pb[(i + 255) & 255];
         T.grad[i] = gb[(i + 255) & 255];
     }
-    uint32_t *hw = &T.g16[0][0];
+    uint32_t *hw = reinterpret_cast<uint32_t *>(T.g16);
     for (int i = threadIdx.x; i < 48; i += blockDim.x) hw[i] = P.g16[i];
 }
 
-__global__ __launch_bounds__(256) void k_noise_batch(const BatchPlanetDev *__restrict__ P,
+// 8 waves per SIMD (at most 64 VGPRs), as for the single-planet noise launch.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_noise_batch(const BatchPlanetDev *__restrict__ P,
                                                      const int64_t *__restrict__ prefix, int np,
                                                      int64_t total_tiles, int tile,
                                                      unsigned *__restrict__ counter,
@@ -147,7 +145,7 @@ __global__ __launch_bounds__(256) void k_noise_batch(const BatchPlanetDev *__res
     {
         const float4 *src = reinterpret_cast<const float4 *>(kExtLutDev.e);
         float4 *dst = reinterpret_cast<float4 *>(T.ext);
-        for (int i = threadIdx.x; i < kExtKeys * 2; i += blockDim.x) dst[i] = src[i];
+        for (int i = threadIdx.x; i < kExtIndex * 2; i += blockDim.x) dst[i] = src[i];
     }
     int cur = -1;
     bool bad = false;
@@ -173,14 +171,9 @@ __global__ __launch_bounds__(256) void k_noise_batch(const BatchPlanetDev *__res
         const auto elev = gmem(D.elev);
         for (int64_t i = v0 + threadIdx.x; i < v1; i += blockDim.x) {
             const double x = verts[3 * i + 0], y = verts[3 * i + 1], z = verts[3 * i + 2];
-            double acc = D.accumulate ? elev[i] : 0.0;
-            for (int o = 0; o < D.n_oct; ++o) {
-                const double r = D.rough[o];
-                const float n = noise3<false>(x * r, y * r, z * r, T, o == D.check_oct, bad,
-                                              nullptr);
-                acc = fma((double)n + 1.0, D.weight[o], acc);
-            }
-            elev[i] = acc;
+            const double acc0 = D.accumulate ? elev[i] : 0.0;
+            elev[i] = octave_sum<false>(x, y, z, acc0, D.oct, T, bad,
+                                        [](int, float, const NoiseSel &) {});
         }
         __syncthreads();
     }
@@ -341,13 +334,10 @@ static int fill_desc(const nx_planet &q, BatchPlanetDev &d)
     d.verts = q.verts;
     d.elev = q.elev;
     d.V = q.V;
-    d.n_oct = q.n_octaves;
     d.accumulate = q.accumulate ? 1 : 0;
     d.radius = q.radius;
-    int check = 0;
-    rc = octave_weights(q.rough, q.strength, q.radius, q.n_octaves, d.rough, d.weight, &check);
+    rc = octave_plan(q.rough, q.strength, q.radius, q.n_octaves, d.oct);
     if (rc) return rc;
-    d.check_oct = check;
     d.lower = q.lower;
     d.upper = q.upper;
     d.do_rescale = q.lower < q.upper;

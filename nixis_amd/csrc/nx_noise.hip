@@ -4,8 +4,10 @@
 // once per octave; each call rescales all coordinates, evaluates noise3d at
 // every vertex and adds ((e + 1) * 0.5 * strength) * radius to the
 // elevations.  Here one lane owns one vertex for all octaves: coordinates
-// are read once, each octave's coordinate scale, lattice selection and the
-// weighting/accumulation stay in FP64 in the reference's order, and only the
+// are read once, each octave's coordinate scale, skew and lattice selection
+// stay in FP64 in the reference's order (or, when the frequencies double,
+// the skewed coordinates are doubled from the previous octave, which gives
+// the same bits), the weighted octaves accumulate in FP64, and only the
 // final elevation is written.  Tables live in LDS (see nx_noise_dev.h).
 #include <hip/hip_runtime.h>
 
@@ -19,22 +21,20 @@ namespace nx {
 
 struct OctaveArgs {
     NoiseTablesPacked tables;
-    double rough[NX_MAX_OCTAVES];
-    double weight[NX_MAX_OCTAVES];   // 0.5 * strength * radius
-    int n_oct;
+    OctavePlan oct;
     int accumulate;
-    int check_oct;                   // octave with the largest |rough|
 };
 
 // keys (nullable): order-preserving min/max keys of the elevations written,
 // reduced here so a rescale after noise needs no pass of its own.
 template <bool DBG>
-__global__ __launch_bounds__(256) void k_sample_octaves(OctaveArgs a, const double *__restrict__ verts,
-                                                        int64_t V, double *__restrict__ elev,
-                                                        float *__restrict__ dbg_noise,
-                                                        int32_t *__restrict__ dbg_sel,
-                                                        int *__restrict__ err,
-                                                        unsigned long long *__restrict__ keys)
+__device__ __forceinline__ void sample_octaves_body(const OctaveArgs &a,
+                                                    const double *__restrict__ verts, int64_t V,
+                                                    double *__restrict__ elev,
+                                                    float *__restrict__ dbg_noise,
+                                                    int32_t *__restrict__ dbg_sel,
+                                                    int *__restrict__ err,
+                                                    unsigned long long *__restrict__ keys)
 {
     unsigned long long lo = ~0ull, hi = 0ull;
     __shared__ NoiseTables T;
@@ -44,22 +44,17 @@ __global__ __launch_bounds__(256) void k_sample_octaves(OctaveArgs a, const doub
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < V; i += stride) {
         const double x = verts[3 * i + 0], y = verts[3 * i + 1], z = verts[3 * i + 2];
-        double acc = a.accumulate ? elev[i] : 0.0;
-        for (int o = 0; o < a.n_oct; ++o) {
-            const double r = a.rough[o];
-            NoiseSel sel;
-            const float n = noise3<DBG>(x * r, y * r, z * r, T, o == a.check_oct, bad, &sel);
-            // terrain.py:28 then terrain.py:43: ((n + 1) * 0.5 * strength) * radius,
-            // with the constant factors folded on the host
-            acc = fma((double)n + 1.0, a.weight[o], acc);
+        const double acc0 = a.accumulate ? elev[i] : 0.0;
+        const double acc = octave_sum<DBG>(x, y, z, acc0, a.oct, T, bad,
+                                           [&](int o, float v, const NoiseSel &sel) {
             if (DBG) {
-                if (dbg_noise) dbg_noise[(int64_t)o * V + i] = n;
+                if (dbg_noise) dbg_noise[(int64_t)o * V + i] = v * (1.0f / 103.0f);
                 if (dbg_sel) {
-                    int32_t *s = dbg_sel + ((int64_t)o * V + i) * 5;
-                    for (int k = 0; k < 5; ++k) s[k] = sel.v[k];
+                    int32_t *d = dbg_sel + ((int64_t)o * V + i) * 5;
+                    for (int k = 0; k < 5; ++k) d[k] = sel.v[k];
                 }
             }
-        }
+        });
         elev[i] = acc;
         if (keys) {
             const unsigned long long k = dkey(acc);
@@ -77,6 +72,23 @@ __global__ __launch_bounds__(256) void k_sample_octaves(OctaveArgs a, const doub
     }
 }
 
+// The VALU-bound launch asks for 8 waves per SIMD (at most 64 VGPRs), which
+// keeps enough waves in flight to cover the dependent LDS hash lookups.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_sample_octaves(
+    OctaveArgs a, const double *__restrict__ verts, int64_t V, double *__restrict__ elev,
+    int *__restrict__ err, unsigned long long *__restrict__ keys)
+{
+    sample_octaves_body<false>(a, verts, V, elev, nullptr, nullptr, err, keys);
+}
+
+// Debug launch: per-octave noise values and selection records.
+__global__ __launch_bounds__(256) void k_sample_octaves_dbg(
+    OctaveArgs a, const double *__restrict__ verts, int64_t V, double *__restrict__ elev,
+    float *__restrict__ dbg_noise, int32_t *__restrict__ dbg_sel, int *__restrict__ err)
+{
+    sample_octaves_body<true>(a, verts, V, elev, dbg_noise, dbg_sel, err, nullptr);
+}
+
 __global__ __launch_bounds__(256) void k_noise3d(NoiseTablesPacked tp, const double *__restrict__ x,
                                                  const double *__restrict__ y,
                                                  const double *__restrict__ z, int64_t n,
@@ -88,7 +100,7 @@ __global__ __launch_bounds__(256) void k_noise3d(NoiseTablesPacked tp, const dou
     bool bad = false;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-        out[i] = (double)noise3<false>(x[i], y[i], z[i], T, true, bad, nullptr);
+        out[i] = (double)noise3<false>(x[i], y[i], z[i], T, bad, nullptr);
     if (bad) __hip_atomic_fetch_or(err, NX_ERR_RANGE_BIT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
@@ -106,7 +118,7 @@ int pack_tables(const int32_t *perm, const int32_t *pgi, NoiseTablesPacked &p)
         if (pgi[i] < 0 || pgi[i] > 69 || pgi[i] % 3 != 0)
             return set_error(NX_EINVAL, "pgi[%d]=%d is not a gradient index", i, pgi[i]);
         pb[i] = (uint8_t)perm[i];
-        gb[i] = (uint8_t)(pgi[i] / 3);
+        gb[i] = (uint8_t)(8 * (pgi[i] / 3));  // byte offset of the gradient's word
     }
     std::memcpy(p.perm, pb, 256);
     std::memcpy(p.grad, gb, 256);
@@ -125,23 +137,30 @@ int pack_tables(const int32_t *perm, const int32_t *pgi, NoiseTablesPacked &p)
     return NX_OK;
 }
 
-// Per-octave frequency, folded weight 0.5 * strength * radius, and the
-// octave with the largest |frequency| (its domain check bounds the others).
-int octave_weights(const double *rough, const double *strength, double radius, int n_oct,
-                   double *r_out, double *w_out, int *check_oct)
+// Per-octave frequency and weights (OctavePlan), the octave with the largest
+// |frequency| (its domain check bounds the others) and whether the
+// frequencies double exactly from one octave to the next (terrain.py:44,
+// lacunarity 2).
+int octave_plan(const double *rough, const double *strength, double radius, int n_oct,
+                OctavePlan &p)
 {
-    *check_oct = 0;
+    std::memset(&p, 0, sizeof(p));
     double best = -1.0;
+    p.doubling = 1;
     for (int o = 0; o < n_oct; ++o) {
         if (!std::isfinite(rough[o]))
             return set_error(NX_EINVAL, "octave %d frequency is not finite", o);
-        r_out[o] = rough[o];
-        w_out[o] = (0.5 * strength[o]) * radius;
+        const double w = (0.5 * strength[o]) * radius;
+        p.rough[o] = rough[o];
+        p.wsum[o] = w / 103.0;
+        p.base += w;
         if (std::fabs(rough[o]) > best) {
             best = std::fabs(rough[o]);
-            *check_oct = o;
+            p.check_oct = o;
         }
+        if (o > 0 && !(rough[o] == 2.0 * rough[o - 1])) p.doubling = 0;
     }
+    p.n_oct = n_oct;
     return NX_OK;
 }
 
@@ -176,19 +195,20 @@ int sample_octaves_keys(nx_ctx *ctx, const double *verts, int64_t V, const int32
     std::memset(&a, 0, sizeof(a));
     int rc = pack_tables(perm, pgi, a.tables);
     if (rc) return rc;
-    rc = octave_weights(rough, strength, radius, n_oct, a.rough, a.weight, &a.check_oct);
+    rc = octave_plan(rough, strength, radius, n_oct, a.oct);
     if (rc) return rc;
-    a.n_oct = n_oct;
     a.accumulate = accumulate ? 1 : 0;
     if (V == 0) return NX_OK;
     NX_HIP(hipSetDevice(ctx->device));
     const int grid = grid_for(ctx, V, 8);
-    if (dbg_noise || dbg_sel)
-        hipLaunchKernelGGL(k_sample_octaves<true>, dim3(grid), dim3(256), 0, stream, a, verts, V,
-                           elev, dbg_noise, dbg_sel, ctx->d_err, keys);
-    else
-        hipLaunchKernelGGL(k_sample_octaves<false>, dim3(grid), dim3(256), 0, stream, a, verts, V,
-                           elev, nullptr, nullptr, ctx->d_err, keys);
+    if (dbg_noise || dbg_sel) {
+        if (keys) return set_error(NX_EINVAL, "min/max keys are not reduced with debug output");
+        hipLaunchKernelGGL(k_sample_octaves_dbg, dim3(grid), dim3(256), 0, stream, a, verts, V,
+                           elev, dbg_noise, dbg_sel, ctx->d_err);
+    } else {
+        hipLaunchKernelGGL(k_sample_octaves, dim3(grid), dim3(256), 0, stream, a, verts, V, elev,
+                           ctx->d_err, keys);
+    }
     NX_HIP(hipGetLastError());
     return NX_OK;
 }

@@ -26,6 +26,8 @@
 #include <cmath>
 #include <cstring>
 
+#include "nixis.h"
+
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define NX_ND __host__ __device__ __forceinline__
@@ -35,11 +37,11 @@
 
 namespace nx {
 
-// Extra-vertex table.  The region logic of opensimplex.py:302-686 reduces
-// to one of seven cases and the bit masks of the closest points; `key`
-// (case * 64 + mask) selects the two extra lattice offsets (each component
-// in -1..2, packed as offset+1 in two bits: ext0 in bits 0-5, ext1 in bits
-// 6-11) and their FP32 delta constants o + (ox + oy + oz) / 3.
+// Extra vertices.  The region logic of opensimplex.py:302-686 ends in one of
+// seven cases and the bit masks of the closest points, which fix the two
+// extra lattice offsets (each component in -1..2, packed as offset+1 in two
+// bits: ext0 in bits 0-5, ext1 in bits 6-11) and their FP32 delta constants
+// o + (ox + oy + oz) / 3:
 //   case 0: tetrahedron at 000, (0,0,0) among the closest two (mask = c)
 //   case 1: tetrahedron at 000, otherwise (mask = a | b)
 //   case 2: tetrahedron at 111, (1,1,1) among the closest two (mask = c)
@@ -53,10 +55,20 @@ struct ExtEntry {
     int32_t pad;
     float c1x, c1y, c1z;
 };
-constexpr int kExtKeys = 7 * 64;
+// The case and mask are functions of five comparison outcomes per region,
+// so the table is indexed by those bits (6-bit index):
+//   bit 5 clear (tetrahedra, opensimplex.py:302-317, 417-434):
+//     bit 0 = first score replacement (z replaces b), bit 1 = second (z
+//     replaces a), bit 2 = a lattice point is closer than the two scores
+//     ("near"), bit 3 = b's score beats a's (b > a at 000, b < a at 111),
+//     bit 4 = tetrahedron at 111;
+//   bit 5 set (octahedron, opensimplex.py:537-575): bit 0 = x + y > 1,
+//     bit 1 = x + z > 1, bit 2 = y + z > 1, bit 3 = y + z replaces a, bit 4
+//     = y + z replaces b.
+constexpr int kExtIndex = 64;
 
 struct ExtLut {
-    ExtEntry e[kExtKeys];
+    ExtEntry e[kExtIndex];
 };
 
 constexpr int first_axis(int m) { return (m & 1) ? 0 : ((m & 2) ? 1 : 2); }
@@ -128,26 +140,51 @@ constexpr ExtEntry ext_entry(int k, int m)
     return r;
 }
 
+// Case and mask of a table index (opensimplex.py:302-686 restated over the
+// comparison bits; combinations the comparisons cannot produce, such as both
+// replacements, map to some valid entry).
+constexpr ExtEntry ext_of_index(int i)
+{
+    const bool b0 = i & 1, b1 = (i >> 1) & 1, b2 = (i >> 2) & 1, b3 = (i >> 3) & 1,
+               b4 = (i >> 4) & 1;
+    if (!((i >> 5) & 1)) {
+        // a = (x, point 1) unless z replaced it, b = (y, point 2) unless z
+        // replaced it; at 111 the points are the complements (6, 5, 3).
+        const int ap = b1 && !b0 ? 4 : 1, bp = b0 ? 4 : 2;
+        if (!b4) return b2 ? ext_entry(0, b3 ? bp : ap) : ext_entry(1, ap | bp);
+        const int cap = 7 ^ ap, cbp = 7 ^ bp;
+        return b2 ? ext_entry(2, b3 ? cbp : cap) : ext_entry(3, cap & cbp);
+    }
+    const bool f1 = b0, f2 = b1, f3 = b2, repA = b3, repB = b4 && !b3;
+    const int ap = repA ? (f3 ? 6 : 1) : (f1 ? 3 : 4);
+    const int bp = repB ? (f3 ? 6 : 1) : (f2 ? 5 : 2);
+    const bool af = repA ? f3 : f1, bf = repB ? f3 : f2;
+    if (af == bf) return af ? ext_entry(4, ap & bp) : ext_entry(5, ap | bp);
+    return af ? ext_entry(6, ap * 8 + bp) : ext_entry(6, bp * 8 + ap);
+}
+
 constexpr ExtLut make_ext_lut()
 {
     ExtLut t{};
-    for (int k = 0; k < 7; ++k)
-        for (int m = 0; m < 64; ++m) t.e[k * 64 + m] = ext_entry(k, m);
+    for (int i = 0; i < kExtIndex; ++i) t.e[i] = ext_of_index(i);
     return t;
 }
 
-// Tables in LDS.  perm and the gradient index (pgi / 3, 0..23) of every hash
-// value are stored as bytes with wrap-around padding: entry i + 1 holds
+// Tables in LDS.  The 24 gradient vectors come first, as 64-bit words (fp16
+// gx | fp16 gy << 16 in the low half, the FP32 bits of gz in the high half),
+// so a gradient's byte offset is its LDS address; the extra-vertex table
+// follows (its fields fit the immediate offsets of the LDS loads), then perm
+// and the gradient table as bytes with wrap-around padding: entry i + 1 holds
 // table[i mod 256] for i in -1..(kWrap - 2), so every hash index used below
 // (lattice coordinate mod 256 plus an offset in -1..2, plus a table value)
-// lands in range without masking.  Then the 24 gradient vectors as
-// (fp16 gx | fp16 gy << 16, FP32 bits of gz) and the extra-vertex table.
+// lands in range without masking.  The gradient table holds 8 * (pgi / 3),
+// the byte offset of the gradient's word, so a hash is an address.
 constexpr int kWrap = 520;
 struct NoiseTables {
+    uint64_t g16[24];
+    ExtEntry ext[kExtIndex];
     uint8_t perm[kWrap];
     uint8_t grad[kWrap];
-    uint32_t g16[24][2];
-    ExtEntry ext[kExtKeys];
 };
 
 // Seed-dependent part passed in kernel arguments.
@@ -188,7 +225,7 @@ __device__ __forceinline__ void load_tables(NoiseTables &T, const NoiseTablesPac
 {
     const uint8_t *pb = reinterpret_cast<const uint8_t *>(P.perm);
     const uint8_t *gb = reinterpret_cast<const uint8_t *>(P.grad);
-    uint32_t *hw = &T.g16[0][0];
+    uint32_t *hw = reinterpret_cast<uint32_t *>(T.g16);
     for (int i = threadIdx.x; i < kWrap; i += blockDim.x) {
         T.perm[i] = pb[(i + 255) & 255];
         T.grad[i] = gb[(i + 255) & 255];
@@ -196,7 +233,7 @@ __device__ __forceinline__ void load_tables(NoiseTables &T, const NoiseTablesPac
     for (int i = threadIdx.x; i < 48; i += blockDim.x) hw[i] = P.g16[i];
     const float4 *src = reinterpret_cast<const float4 *>(kExtLutDev.e);
     float4 *dst = reinterpret_cast<float4 *>(T.ext);
-    for (int i = threadIdx.x; i < kExtKeys * 2; i += blockDim.x) dst[i] = src[i];
+    for (int i = threadIdx.x; i < kExtIndex * 2; i += blockDim.x) dst[i] = src[i];
 }
 #endif
 
@@ -225,20 +262,22 @@ NX_ND float float_of_bits(uint32_t b)
 #endif
 }
 
-NX_ND float contrib(float v, float dx, float dy, float dz, int gi, const NoiseTables &T,
+// `go` is a gradient table entry: the byte offset of the gradient's word.
+NX_ND float contrib(float v, float dx, float dy, float dz, int go, const NoiseTables &T,
                     float cap)
 {
     float attn = fmaf(-dz, dz, fmaf(-dy, dy, fmaf(-dx, dx, 2.0f)));
     attn = clamp0(attn, cap);
     const float a2 = attn * attn;
-    const uint32_t w0 = T.g16[gi][0], w1 = T.g16[gi][1];
+    const uint64_t w = *reinterpret_cast<const uint64_t *>(reinterpret_cast<const uint8_t *>(T.g16) + go);
+    const uint32_t w0 = (uint32_t)w, w1 = (uint32_t)(w >> 32);
     const float dot = fmaf(half_to_float(w0 & 0xffffu), dx,
                            fmaf(half_to_float(w0 >> 16), dy, float_of_bits(w1) * dz));
     return fmaf(a2 * a2, dot, v);
 }
 
-// Hash of lattice point (x, y, z) with x, y, z in -1..257 (a coordinate mod
-// 256 plus an offset).
+// Gradient table entry (byte offset of the gradient's word) of lattice point
+// (x, y, z) with x, y, z in -1..257 (a coordinate mod 256 plus an offset).
 NX_ND int hash_full(const NoiseTables &T, int x, int y, int z)
 {
     const int a = T.perm[1 + x];
@@ -254,73 +293,128 @@ struct NoiseSel {
     int v[5];
 };
 
-// Evaluate noise3d at (x, y, z) (already multiplied by the octave's
-// frequency).  Returns value / 103 in FP32.  With `check`, `bad` is set when
-// the point is outside the exact domain (the value is then meaningless); the
-// caller checks the octave with the largest frequency, which bounds the
-// others.  Every branch of the reference's region logic is evaluated with
-// selects, so a wave never diverges.
-template <bool SEL>
-NX_ND float noise3(double x, double y, double z, const NoiseTables &T, bool check, bool &bad,
-                   NoiseSel *sel)
+// Skewed coordinates of (x, y, z) (opensimplex.py:272-275, FP64, reference
+// operation order).
+struct Skewed {
+    double x, y, z;
+};
+
+NX_ND Skewed skew3(double x, double y, double z)
 {
     const double STRETCH = -1.0 / 6;
-    const double SQUISH = 1.0 / 3;
-    // opensimplex.py:272-299 (FP64, reference operation order)
     const double so = (x + y + z) * STRETCH;
-    const double xs = x + so, ys = y + so, zs = z + so;
-    if (check)
-        bad = bad | !((fabs(xs) < kNoiseCoordLimit) & (fabs(ys) < kNoiseCoordLimit) &
-                      (fabs(zs) < kNoiseCoordLimit));
+    return Skewed{x + so, y + so, z + so};
+}
+
+// Whether skewed coordinates lie in the exact domain of the FP64 section
+// (false for NaN).  Outside it the noise value is meaningless; callers check
+// the octave with the largest frequency, which bounds the others.
+NX_ND bool in_domain(const Skewed &s)
+{
+    return (fabs(s.x) < kNoiseCoordLimit) & (fabs(s.y) < kNoiseCoordLimit) &
+           (fabs(s.z) < kNoiseCoordLimit);
+}
+
+// Comparison outcomes as wave-wide lane masks, so the region logic below
+// runs as scalar mask operations (one bit per lane) and only the final index
+// bits are expanded per lane.  The host build keeps all-or-nothing masks.
+typedef uint64_t LaneMask;
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __forceinline__ LaneMask lanes(bool b) { return __builtin_amdgcn_ballot_w64(b); }
+__device__ __forceinline__ bool lane_bit(LaneMask m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
+#else
+inline LaneMask lanes(bool b) { return b ? ~0ull : 0ull; }
+inline bool lane_bit(LaneMask m) { return m != 0; }
+#endif
+
+// Region of the in-cell coordinates (in_sum <= 1: tetrahedron at 000, >= 2:
+// tetrahedron at 111, else octahedron) and the extra-vertex table index
+// (ExtLut), from FP64 comparisons in the reference's form.  A NaN coordinate
+// is outside the domain and flagged, so a complemented comparison stands for
+// the opposite one.
+struct RegionSel {
+    int idx;
+    bool inA, r1, octa;  // region 0, region 1, region 2
+};
+
+NX_ND RegionSel select_region(double xins, double yins, double zins, double in_sum)
+{
+    const LaneMask xy_ge = lanes(xins >= yins), xy_le = lanes(xins <= yins);
+    const LaneMask zy_gt = lanes(zins > yins), zy_lt = lanes(zins < yins);
+    const LaneMask zx_gt = lanes(zins > xins), zx_lt = lanes(zins < xins);
+    // tetrahedron at (0,0,0): opensimplex.py:302-317.  a = (x, point 1),
+    // b = (y, point 2); t1: z replaces b, t2: z replaces a.
+    const double wA = 1 - in_sum;
+    const LaneMask a1 = xy_ge & zy_gt;
+    const LaneMask a2 = ~xy_ge & zx_gt;
+    const LaneMask a0 = ~(a1 | a2);
+    const LaneMask wx = lanes(wA > xins), wy = lanes(wA > yins), wz = lanes(wA > zins);
+    const LaneMask nearA = (a1 & (wx | wz)) | (a2 & (wz | wy)) | (a0 & (wx | wy));
+    // b's score beats a's: z > x (t1), y > z (t2), y > x
+    const LaneMask gtA = (a1 & zx_gt) | (a2 & zy_lt) | (a0 & ~xy_ge);
+    // tetrahedron at (1,1,1): opensimplex.py:417-434, comparisons reversed.
+    const double wB = 3 - in_sum;
+    const LaneMask b1 = xy_le & zy_lt;
+    const LaneMask b2 = ~xy_le & zx_lt;
+    const LaneMask b0 = ~(b1 | b2);
+    const LaneMask vx = lanes(wB < xins), vy = lanes(wB < yins), vz = lanes(wB < zins);
+    const LaneMask nearB = (b1 & (vx | vz)) | (b2 & (vz | vy)) | (b0 & (vx | vy));
+    // b's score below a's: z < x (t1), y < z (t2), y < x
+    const LaneMask ltB = (b1 & zx_lt) | (b2 & zy_gt) | (b0 & ~xy_le);
+    // octahedron: opensimplex.py:537-575.  p - 1 when p > 1, else 1 - p
+    // (1 - p equals -(p - 1) exactly in IEEE arithmetic, so this is |p - 1|).
+    const double p1 = xins + yins, p2 = xins + zins, p3 = yins + zins;
+    const LaneMask f1 = lanes(p1 > 1), f2 = lanes(p2 > 1), f3 = lanes(p3 > 1);
+    const double asC = fabs(p1 - 1), bsC = fabs(p2 - 1), scC = fabs(p3 - 1);
+    const LaneMask ab_le = lanes(asC <= bsC);
+    const LaneMask repA = ab_le & lanes(asC < scC);
+    const LaneMask repB = ~ab_le & lanes(bsC < scC);
+    const LaneMask inA = lanes(in_sum <= 1), inB = lanes(in_sum >= 2);
+    const LaneMask octa = ~(inA | inB);
+    const LaneMask r1 = ~inA & inB;
+    const LaneMask r0 = ~octa & ~r1;
+    // table index bits (ExtLut)
+    const LaneMask i0 = (octa & f1) | (r1 & b1) | (r0 & a1);
+    const LaneMask i1 = (octa & f2) | (r1 & b2) | (r0 & a2);
+    const LaneMask i2 = (octa & f3) | (r1 & nearB) | (r0 & nearA);
+    const LaneMask i3 = (octa & repA) | (r1 & ltB) | (r0 & gtA);
+    const LaneMask i4 = (octa & repB) | r1;
+    RegionSel r;
+    r.idx = (lane_bit(i0) ? 1 : 0) + (lane_bit(i1) ? 2 : 0) + (lane_bit(i2) ? 4 : 0) +
+            (lane_bit(i3) ? 8 : 0) + (lane_bit(i4) ? 16 : 0) + (lane_bit(octa) ? 32 : 0);
+    r.inA = lane_bit(inA);
+    r.r1 = lane_bit(r1);
+    r.octa = lane_bit(octa);
+    return r;
+}
+
+// Evaluate noise3d at the skewed coordinates (xs, ys, zs) of a point already
+// multiplied by the octave's frequency.  Returns the FP32 sum of the
+// contributions: noise3d's value is that divided by 103 (opensimplex.py:759).
+// Every branch of the reference's region logic is evaluated with selects, so
+// a wave never diverges.
+template <bool SEL>
+NX_ND float noise3_sum(double xs, double ys, double zs, const NoiseTables &T, NoiseSel *sel)
+{
+    const double SQUISH = 1.0 / 3;
     // floor(xs) is the integer the reference's int(floor(xs)) holds, as a
     // double (exact inside the checked domain |xs| < 2^30), so it serves the
     // FP64 arithmetic directly and the int only the hashing.
     const double fx = floor(xs), fy = floor(ys), fz = floor(zs);
     const int xsb = (int)fx, ysb = (int)fy, zsb = (int)fz;
-    const double sq = (fx + fy + fz) * SQUISH;
     const double xins = xs - fx, yins = ys - fy, zins = zs - fz;
     const double in_sum = xins + yins + zins;
-    // The deltas to the cell origin, x - (fx + sq) (opensimplex.py:297-299),
-    // only feed the FP32 section: xins - (so + sq) is the same value up to
-    // FP64 rounding, with one shared addition.
-    const double t0 = so + sq;
-    const double dx0d = xins - t0, dy0d = yins - t0, dz0d = zins - t0;
+    // The deltas to the cell origin, x - (xsb + sq) with sq = (xsb + ysb +
+    // zsb) / 3 (opensimplex.py:283-299), only feed the FP32 section.  Since
+    // x = xs - so and so = -(x + y + z) / 6, in_sum = -3 (so + sq), so the
+    // delta is xins + in_sum / 3: the same value up to FP64 rounding, from
+    // one shared product.
+    const double u = in_sum * SQUISH;
+    const double dx0d = xins + u, dy0d = yins + u, dz0d = zins + u;
 
-    // ---- region logic, FP64 comparisons in the reference's form ----------
-    // tetrahedron at (0,0,0): opensimplex.py:302-317
-    const bool a1 = (xins >= yins) & (zins > yins);
-    const bool a2 = !a1 & (xins < yins) & (zins > xins);
-    const double asA = a2 ? zins : xins, bsA = a1 ? zins : yins;
-    const int apA = a2 ? 4 : 1, bpA = a1 ? 4 : 2;
-    const double wA = 1 - in_sum;
-    const bool nearA = (wA > asA) | (wA > bsA);
-    const bool bA = bsA > asA;
-    const int keyA = nearA ? (bA ? bpA : apA) : 64 + (apA | bpA);
-    // tetrahedron at (1,1,1): opensimplex.py:417-434
-    const bool b1 = (xins <= yins) & (zins < yins);
-    const bool b2 = !b1 & (xins > yins) & (zins < xins);
-    const double asB = b2 ? zins : xins, bsB = b1 ? zins : yins;
-    const int apB = b2 ? 3 : 6, bpB = b1 ? 3 : 5;
-    const double wB = 3 - in_sum;
-    const bool nearB = (wB < asB) | (wB < bsB);
-    const bool bB = bsB < asB;
-    const int keyB = nearB ? 128 + (bB ? bpB : apB) : 192 + (apB & bpB);
-    // octahedron: opensimplex.py:537-575
-    const double p1 = xins + yins, p2 = xins + zins, p3 = yins + zins;
-    const bool f1 = p1 > 1, f2 = p2 > 1, f3 = p3 > 1;
-    // p - 1 when p > 1, else 1 - p (opensimplex.py:539-559); 1 - p equals
-    // -(p - 1) exactly in IEEE arithmetic, so this is |p - 1|.
-    const double asC = fabs(p1 - 1), bsC = fabs(p2 - 1), scC = fabs(p3 - 1);
-    const bool repA = (asC <= bsC) & (asC < scC);
-    const bool repB = !repA & (asC > bsC) & (bsC < scC);
-    const int rp = f3 ? 6 : 1;
-    const int apC = repA ? rp : (f1 ? 3 : 4), bpC = repB ? rp : (f2 ? 5 : 2);
-    const bool afC = repA ? f3 : f1, bfC = repB ? f3 : f2;
-    const int kFF = 256 + (apC & bpC), kNN = 320 + (apC | bpC);
-    const int kMix = 384 + (afC ? apC * 8 + bpC : bpC * 8 + apC);
-    const int keyC = (afC == bfC) ? (afC ? kFF : kNN) : kMix;
-    const int region = in_sum <= 1 ? 0 : (in_sum >= 2 ? 1 : 2);
-    const int key = region == 0 ? keyA : (region == 1 ? keyB : keyC);
+    const RegionSel rs = select_region(xins, yins, zins, in_sum);
+    const bool inA = rs.inA, r1 = rs.r1, octa = rs.octa;
+    const int region = inA ? 0 : (r1 ? 1 : 2);
 
     // ---- FP32 section -----------------------------------------------------
     const float d0x = (float)dx0d, d0y = (float)dy0d, d0z = (float)dz0d;
@@ -341,8 +435,8 @@ NX_ND float noise3(double x, double y, double z, const NoiseTables &T, bool chec
     const int g101 = T.grad[2 + p10 + zm];
     const int g011 = T.grad[2 + p01 + zm];
 
-    const float oneBits = region != 1 ? 2.0f : 0.0f;  // 100, 010, 001
-    const float twoBits = region != 0 ? 2.0f : 0.0f;  // 110, 101, 011
+    const float oneBits = r1 ? 0.0f : 2.0f;    // 100, 010, 001
+    const float twoBits = inA ? 0.0f : 2.0f;   // 110, 101, 011
 
     // Corner c: delta = d0 - c - S * popcount(c).
     const float ax = d0x - S, ay = d0y - S, az = d0z - S;
@@ -354,12 +448,11 @@ NX_ND float noise3(double x, double y, double z, const NoiseTables &T, bool chec
     v = contrib(v, bx - 1.0f, by, bz - 1.0f, g101, T, twoBits);
     v = contrib(v, bx, by - 1.0f, bz - 1.0f, g011, T, twoBits);
     // Origin slot: 000 in region 0, 111 in region 1.
-    const bool r1 = region == 1;
     const int go = T.grad[1 + (r1 ? p11 + 1 : p00) + zm];
     const float k = r1 ? 2.0f : 0.0f;  // 1 + 3S
-    v = contrib(v, d0x - k, d0y - k, d0z - k, go, T, region != 2 ? 2.0f : 0.0f);
+    v = contrib(v, d0x - k, d0y - k, d0z - k, go, T, octa ? 0.0f : 2.0f);
     // Extra vertices from the table.
-    const ExtEntry &E = T.ext[key];
+    const ExtEntry &E = T.ext[rs.idx];
     const int code = E.code;
     const int ge0 = hash_full(T, xm + (code & 3) - 1, ym + ((code >> 2) & 3) - 1,
                               zm + ((code >> 4) & 3) - 1);
@@ -370,29 +463,120 @@ NX_ND float noise3(double x, double y, double z, const NoiseTables &T, bool chec
 
     if (SEL) {
         // Reference evaluation order of the selected points; the checksum
-        // accumulates pgi = 3 * gradient index.
+        // accumulates pgi = 3 * gradient index = 3 * (table entry / 8).
         uint32_t cs = 0;
+        auto add = [&cs](int g) { cs = cs * 31u + 3u * (uint32_t)(g >> 3); };
         if (region == 0) {
-            const int g000 = T.grad[1 + p00 + zm];
-            const int order[4] = {g000, g100, g010, g001};
-            for (int i = 0; i < 4; ++i) cs = cs * 31u + 3u * (uint32_t)order[i];
+            add(T.grad[1 + p00 + zm]);
+            add(g100);
+            add(g010);
+            add(g001);
         } else if (region == 1) {
-            const int g111 = T.grad[2 + p11 + zm];
-            const int order[4] = {g110, g101, g011, g111};
-            for (int i = 0; i < 4; ++i) cs = cs * 31u + 3u * (uint32_t)order[i];
+            add(g110);
+            add(g101);
+            add(g011);
+            add(T.grad[2 + p11 + zm]);
         } else {
-            const int order[6] = {g100, g010, g001, g110, g101, g011};
-            for (int i = 0; i < 6; ++i) cs = cs * 31u + 3u * (uint32_t)order[i];
+            add(g100);
+            add(g010);
+            add(g001);
+            add(g110);
+            add(g101);
+            add(g011);
         }
-        cs = cs * 31u + 3u * (uint32_t)ge0;
-        cs = cs * 31u + 3u * (uint32_t)ge1;
+        add(ge0);
+        add(ge1);
         sel->v[0] = xsb;
         sel->v[1] = ysb;
         sel->v[2] = zsb;
         sel->v[3] = region | (code & 63) << 2 | ((code >> 6) & 63) << 8;
         sel->v[4] = (int)cs;
     }
-    return v * (1.0f / 103.0f);
+    return v;
 }
+
+// noise3d (opensimplex.py:266-759) at (x, y, z): the value in FP32; `bad` is
+// set when the point is outside the exact domain.
+template <bool SEL>
+NX_ND float noise3(double x, double y, double z, const NoiseTables &T, bool &bad, NoiseSel *sel)
+{
+    const Skewed s = skew3(x, y, z);
+    bad = bad | !in_domain(s);
+    return noise3_sum<SEL>(s.x, s.y, s.z, T, sel) * (1.0f / 103.0f);
+}
+
+// Per-octave scalars of one sample_octaves call (terrain.py:36-45): octave o
+// adds ((noise + 1) * 0.5 * strength) * radius (terrain.py:28, 43), which is
+// w_o + (w_o / 103) * sum_o with w_o = 0.5 * strength * radius and sum_o the
+// FP32 contribution sum of noise3_sum.
+struct OctavePlan {
+    double rough[NX_MAX_OCTAVES];  // frequency / radius of each octave
+    double wsum[NX_MAX_OCTAVES];   // w_o / 103
+    double base;                   // sum of w_o over the octaves
+    int32_t n_oct;
+    int32_t check_oct;             // octave with the largest |rough|
+    int32_t doubling;              // rough[o + 1] == 2 * rough[o] for every o
+    int32_t pad;
+};
+
+// Whether doubling octave o's skewed coordinates gives octave o + 1's bit for
+// bit, for a vertex (x, y, z) whose first octave has frequency r0, when the
+// frequencies double exactly.  It does when none of the first octave's
+// products rounds in the subnormal range, i.e. x * r0, y * r0, z * r0 and
+// ((x + y) + z) * STRETCH are zero or at least 2^-1021 in magnitude: every
+// later product is then the first's times a power of two with the same
+// rounding, and sums scale exactly.  NaN fails the test.
+NX_ND bool doubles_exactly(double x, double y, double z, double r0)
+{
+    const double lim = 0x1p-1021;
+    const double X = x * r0, Y = y * r0, Z = z * r0;
+    const double s = X + Y + Z;
+    const double so = s * (-1.0 / 6);
+    return ((x == 0) | (fabs(X) >= lim)) & ((y == 0) | (fabs(Y) >= lim)) &
+           ((z == 0) | (fabs(Z) >= lim)) & ((s == 0) | (fabs(so) >= lim));
+}
+
+#if defined(__HIPCC__)
+// The octave loop of terrain.py:40-45 for one vertex: returns acc plus every
+// octave's weighted noise; `bad` is set when the octave with the largest
+// frequency leaves the exact domain.  `on_octave(o, sum, sel)` sees each octave's
+// contribution sum (the noise value times 103) and selection record.  The
+// doubling shortcut is taken per wave (every lane doubles or none does), so
+// the branch is uniform.
+template <bool SEL, typename F>
+__device__ __forceinline__ double octave_sum(double x, double y, double z, double acc,
+                                             const OctavePlan &P, const NoiseTables &T,
+                                             bool &bad, F &&on_octave)
+{
+    const int n_oct = P.n_oct;
+    if (n_oct <= 0) return acc;
+    const double r0 = P.rough[0];
+    const bool dbl = P.doubling && __all(doubles_exactly(x, y, z, r0));
+    const double rc = P.rough[P.check_oct];
+    bad = bad | !in_domain(skew3(x * rc, y * rc, z * rc));
+    acc += P.base;
+    auto octave = [&](int o, const Skewed &s) {
+        NoiseSel sel;
+        const float v = noise3_sum<SEL>(s.x, s.y, s.z, T, &sel);
+        acc = fma((double)v, P.wsum[o], acc);
+        on_octave(o, v, sel);
+    };
+    if (dbl) {
+        Skewed s = skew3(x * r0, y * r0, z * r0);
+        for (int o = 0; o < n_oct; ++o) {
+            octave(o, s);
+            s.x += s.x;
+            s.y += s.y;
+            s.z += s.z;
+        }
+    } else {
+        for (int o = 0; o < n_oct; ++o) {
+            const double r = P.rough[o];
+            octave(o, skew3(x * r, y * r, z * r));
+        }
+    }
+    return acc;
+}
+#endif
 
 }  // namespace nx

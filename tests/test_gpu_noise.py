@@ -112,6 +112,33 @@ def test_noise_out_of_domain_raises():
     terrain.sample_octaves(pts[:1], None, perm, pgi, 1)
 
 
+def test_doubling_shortcut_is_bit_exact():
+    """With frequencies that double (lacunarity 2) a wave doubles the
+    previous octave's skewed coordinates, unless one of its vertices has a
+    first-octave product in the subnormal range.  Interleaving such a vertex
+    with every real one forces the direct path in every wave; selection,
+    per-octave noise and elevations must be the same bit for bit."""
+    pts = util.create_points(24)
+    mixed = np.empty((2 * pts.shape[0], 3))
+    mixed[0::2] = pts
+    mixed[1::2] = [1e-310, 0.3, -0.2]
+    a = _run(pts, 3, 12, 1.5, 0.4, 2.0, 0.5)
+    b = _run(mixed, 3, 12, 1.5, 0.4, 2.0, 0.5)
+    assert np.array_equal(a[0], b[0][0::2])
+    assert np.array_equal(a[1], b[1][:, 0::2])
+    assert np.array_equal(a[2], b[2][:, 0::2])
+
+
+def test_domain_check_uses_the_highest_octave():
+    """x = 2^16: octave 8 (frequency 1.5 * 2^7) stays inside |skewed| < 2^30,
+    octave 16 (1.5 * 2^15) does not."""
+    pts = np.array([[1.0, 0.0, 0.0], [2.0**16, 0.0, 0.0]])
+    perm, pgi = osx.init(0)
+    terrain.sample_octaves(pts, None, perm, pgi, 8)
+    with pytest.raises(_lib.NixisRangeError):
+        terrain.sample_octaves(pts, None, perm, pgi, 16)
+
+
 def test_noise_empty_and_zero_octaves():
     perm, pgi = osx.init(0)
     assert terrain.sample_octaves(np.zeros((0, 3)), None, perm, pgi, 3).shape == (0,)
