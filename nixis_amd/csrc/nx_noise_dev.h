@@ -16,8 +16,8 @@
 //     keeps all six);
 //   - one "origin" slot holds 000 (region 0) or 111 (region 1);
 //   - two slots hold the extra vertices chosen by the region logic, whose
-//     lattice offsets and delta constants come from a small table keyed by
-//     the region's case and closest-point masks.
+//     lattice offsets and delta constants come from a 64-entry table indexed
+//     by the region's comparison outcomes.
 // The hash prefixes perm[x], perm[perm[x]+y] are shared between corners.
 #pragma once
 
