@@ -2,7 +2,7 @@
 
 Jobs (nixis_amd.batch.Job) arrive over time.  ``Admission`` decides, in
 arrival order, which jobs fit the HBM budget for per-job device fields: the
-FP64 heights and, for a job that erodes, its float-float erosion workspace.
+FP64 heights and, for a job that erodes, its FP64 erosion state.
 Meshes (vertices and neighbour tables) depend only on the subdivision count;
 they are shared assets provisioned before the trace starts.  A job whose
 fields exceed the whole budget is rejected; others wait, first come first
@@ -161,6 +161,8 @@ class PlanetServer:
             remaining = r.job.erosion_iters - r.steps_done
             n = remaining if self.steps_per_tick == 0 else min(remaining, self.steps_per_tick)
             n = min(n, max(2, left // r.V))
+            if n < remaining and n % 2:
+                n -= 1   # whole pairs; an odd (single-step) launch only at the end
             tick.append(r)
             caps.append(n)
             left -= n * r.V

@@ -60,3 +60,32 @@ def test_served_trace_shape():
     # mean inter-arrival ~ 1/rate
     gaps = [b - a for a, b in zip([0.0] + arr[:-1], arr)]
     assert abs(sum(gaps) / len(gaps) - 1 / 50.0) < 0.005
+
+
+class _Run:
+    def __init__(self, V, iters):
+        self.V = V
+        self.steps_done = 0
+        self.job = Job(d=2, seed=0, n_octaves=1, erosion_iters=iters)
+
+
+@pytest.mark.parametrize("budget, per_tick", [(9_000, 0), (4_000, 0), (50_000, 5)])
+def test_erosion_ticks_hand_out_whole_pairs(budget, per_tick):
+    """Every tick gives a job an even number of steps (two-step launches)
+    except the tick that finishes it, and the steps add up to the job's."""
+    from nixis_amd.serve import PlanetServer
+    srv = PlanetServer.__new__(PlanetServer)
+    srv.tick_vertex_steps, srv.steps_per_tick = budget, per_tick
+    runs = [_Run(1000, 31), _Run(400, 7), _Run(3000, 10)]
+    eroding, ticks = list(runs), 0
+    while eroding:
+        tick, caps = srv._tick(eroding)
+        assert tick and all(n >= 1 for n in caps)
+        for r, n in zip(tick, caps):
+            left = r.job.erosion_iters - r.steps_done
+            assert n <= left and (n % 2 == 0 or n == left)
+            r.steps_done += n
+        eroding = [r for r in eroding if r.steps_done < r.job.erosion_iters]
+        ticks += 1
+        assert ticks < 100
+    assert [r.steps_done for r in runs] == [31, 7, 10]
