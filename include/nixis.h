@@ -292,7 +292,7 @@ typedef struct nx_tile_plan {
     int64_t bytes;       /* size of the plan image */
     int64_t V;
     int64_t ntiles;
-    int32_t max_tile;    /* largest tile (owned vertices), <= 1024 */
+    int32_t max_tile;    /* largest tile (owned vertices): <= 768 (2 steps), 1024 (4) */
     int32_t max_load;    /* largest tile + rings, <= 2048 (2 steps), 3072 (4) */
     int64_t off_start, off_cs, off_load, off_rbase, off_ring, off_cbase,
             off_lcol, off_lel;   /* byte offsets of the arrays in the image */
@@ -312,7 +312,9 @@ int nx_tile_plan_build(const int32_t *col, const double *elen, int64_t V,
 
 /* nx_tile_plan_build for `steps` (2 or 4) steps per launch: tiles carry
  * `steps` rings (tile + rings at most 2048 vertices for 2 steps, 3072 for
- * 4); cs counts the tile and rings 1..steps-1, load all of them.  A 4-step
+ * 4; for 2 steps also tile + ring 1 at most 1024, one thread each in the
+ * two-step kernel); cs counts the tile and rings 1..steps-1, load all of
+ * them.  A 4-step
  * plan runs four steps per launch in nx_erosion_run_tiled / _rule (the
  * neighbour data read once per launch serves four steps; on an icosphere
  * with 1024-vertex tiles no faster than 2 steps, as the rings grow); the

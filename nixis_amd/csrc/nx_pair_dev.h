@@ -19,6 +19,12 @@ namespace nx {
 
 constexpr int kTileMax = 1024;      // owned vertices per tile
 constexpr int kLoadMax = 2048;      // tile + both rings (two steps per launch)
+// Two-step plans start from tiles of kPairChunk vertices and keep tile + ring
+// 1 (the vertices of the first step) within kStepOneMax, the pair kernel's
+// block size, so every first-step vertex has its own thread and its
+// neighbour slots and lengths load before the block's first barrier.
+constexpr int kPairChunk = 768;
+constexpr int kStepOneMax = 1024;
 constexpr int kTileMin = 32;        // smallest tile the planner cuts to
 constexpr int kMultiSteps = 4;      // steps per launch of the multi-step kernel
 constexpr int kLoadMaxMulti = 3072; // tile + its kMultiSteps rings

@@ -5,10 +5,11 @@
 // outside the tile and the earlier rings.  K Jacobi steps of the tile need
 // the fields of the tile and K rings (erosion.py:197-279 reads only the
 // one-ring of each vertex): step j runs on the tile and rings 1..K-j.  Tiles
-// start at kTileMax vertices and are halved until tile + rings fit the
-// kernel's load limit, so a space-filling-curve order (compact tiles, short
-// perimeters) gives few, full tiles; a graph without locality fails with
-// NX_ERANGE.
+// start at kPairChunk vertices for two steps (kTileMax for more) and are
+// halved until tile + rings fit the kernel's load limit and, for two steps,
+// tile + ring 1 fits kStepOneMax; a space-filling-curve order (compact tiles,
+// short perimeters) gives few, full tiles; a graph without locality fails
+// with NX_ERANGE.
 //
 // Plan image (offsets 256-aligned, recorded in nx_tile_plan):
 //   start  int32 [ntiles + 1]   first vertex of each tile (and V)
@@ -59,8 +60,10 @@ bool in_rings(const Tile &t, int m, int32_t n)
     return false;
 }
 
-// K rings of [a, a+T); false if tile + rings exceed `limit`.
-bool rings(const int32_t *col, int64_t V, int64_t a, int32_t T, int K, int limit, Tile &t)
+// K rings of [a, a+T); false if tile + rings exceed `limit` or tile + ring 1
+// exceeds `step1_limit`.
+bool rings(const int32_t *col, int64_t V, int64_t a, int32_t T, int K, int limit, int step1_limit,
+           Tile &t)
 {
     const int64_t b = a + T;
     t.a = a;
@@ -82,23 +85,24 @@ bool rings(const int32_t *col, int64_t V, int64_t a, int32_t T, int K, int limit
         }
         sort_unique(cur);
         total += (int64_t)cur.size();
-        if (total > limit) return false;
+        if (total > limit || (m == 0 && total > step1_limit)) return false;
     }
     return true;
 }
 
 // Cut [a, a+len) into tiles that fit, halving as needed.
 bool cut(const int32_t *col, int64_t V, int64_t a, int64_t len, int K, int limit,
-         std::vector<Tile> &out)
+         int step1_limit, std::vector<Tile> &out)
 {
     Tile t;
-    if (rings(col, V, a, (int32_t)len, K, limit, t)) {
+    if (rings(col, V, a, (int32_t)len, K, limit, step1_limit, t)) {
         out.push_back(std::move(t));
         return true;
     }
     if (len / 2 < kTileMin) return false;
     const int64_t h = len / 2;
-    return cut(col, V, a, h, K, limit, out) && cut(col, V, a + h, len - h, K, limit, out);
+    return cut(col, V, a, h, K, limit, step1_limit, out) &&
+           cut(col, V, a + h, len - h, K, limit, step1_limit, out);
 }
 
 inline int64_t a256(int64_t x) { return (x + 255) & ~(int64_t)255; }
@@ -113,24 +117,28 @@ int tile_plan_build(const int32_t *col, const double *elen, int64_t V, int K, vo
     if (K != 2 && K != kMultiSteps)
         return set_error(NX_EINVAL, "tile plan for %d steps per launch: 2 or %d", K, kMultiSteps);
     const int limit = K == 2 ? kLoadMax : kLoadMaxMulti;
+    const int step1_limit = K == 2 ? kStepOneMax : kLoadMaxMulti;
+    const int64_t chunk = K == 2 ? kPairChunk : kTileMax;
     for (int64_t i = 0; i < 6 * V; ++i)
         if (col[i] < 0 || col[i] >= V) return set_error(NX_EINVAL, "col[%lld] = %d outside 0..V-1", (long long)i, col[i]);
 
-    // Chunks of kTileMax, each cut independently (in parallel), in order.
-    const int64_t nchunks = (V + kTileMax - 1) / kTileMax;
+    // Chunks of `chunk` vertices, each cut independently (in parallel), in order.
+    const int64_t nchunks = (V + chunk - 1) / chunk;
     std::vector<std::vector<Tile>> parts((size_t)nchunks);
     std::vector<char> ok((size_t)nchunks, 1);
     parallel_for(nchunks, [&](int64_t c0, int64_t c1) {
         for (int64_t c = c0; c < c1; ++c) {
-            const int64_t a = c * kTileMax;
-            ok[(size_t)c] = cut(col, V, a, std::min<int64_t>(kTileMax, V - a), K, limit,
-                                parts[(size_t)c]);
+            const int64_t a = c * chunk;
+            ok[(size_t)c] = cut(col, V, a, std::min<int64_t>(chunk, V - a), K, limit,
+                                step1_limit, parts[(size_t)c]);
         }
     }, 16);
     for (int64_t c = 0; c < nchunks; ++c)
         if (!ok[(size_t)c])
-            return set_error(NX_ERANGE, "vertices %lld.. do not cut into tiles whose %d rings fit %d vertices",
-                             (long long)(c * kTileMax), K, limit);
+            return set_error(NX_ERANGE,
+                             "vertices %lld.. do not cut into tiles whose %d rings fit %d vertices "
+                             "(tile + ring 1 within %d)",
+                             (long long)(c * chunk), K, limit, step1_limit);
     std::vector<const Tile *> tiles;
     for (auto &p : parts)
         for (auto &t : p) tiles.push_back(&t);
