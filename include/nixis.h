@@ -297,7 +297,7 @@ typedef struct nx_tile_plan {
     int64_t off_start, off_cs, off_load, off_rbase, off_ring, off_cbase,
             off_lcol, off_lel;   /* byte offsets of the arrays in the image */
     int32_t steps;       /* steps per launch the plan serves: 2 or 4 */
-    int32_t pad;
+    int32_t max_step1;   /* largest tile + ring 1 (first-step vertices) */
     int64_t off_pref;    /* int32 [ntiles][steps + 1]: tile, tile + ring 1, ... */
 } nx_tile_plan;
 

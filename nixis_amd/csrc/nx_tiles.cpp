@@ -145,12 +145,13 @@ int tile_plan_build(const int32_t *col, const double *elen, int64_t V, int K, vo
     const int64_t nt = (int64_t)tiles.size();
 
     int64_t nring = 0, ncs = 0;
-    int32_t max_tile = 0, max_load = 0;
+    int32_t max_tile = 0, max_load = 0, max_step1 = 0;
     for (const Tile *t : tiles) {
         nring += t->ring_total();
         ncs += t->T + t->ring_total() - (int64_t)t->r.back().size();
         max_tile = std::max(max_tile, t->T);
         max_load = std::max<int32_t>(max_load, t->T + t->ring_total());
+        max_step1 = std::max<int32_t>(max_step1, t->T + (int32_t)t->r[0].size());
     }
     // Rings larger than the tiles themselves read more than K single steps
     // would: such a graph has no locality to exploit.
@@ -163,6 +164,7 @@ int tile_plan_build(const int32_t *col, const double *elen, int64_t V, int K, vo
     q.ntiles = nt;
     q.max_tile = max_tile;
     q.max_load = max_load;
+    q.max_step1 = max_step1;
     int64_t o = 0;
     q.off_start = o; o = a256(o + 4 * (nt + 1));
     q.off_cs = o;    o = a256(o + 4 * nt);

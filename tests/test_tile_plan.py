@@ -76,8 +76,12 @@ def test_plan_covers_and_rings_are_exact(d):
     pts, nb, o, col, elen = _ell(d)
     V = col.shape[1]
     nxt = 0
+    step1 = 0
     for q, a, b, T, c, L, glob, lc, le, _ in _tiles(col, elen):
-        assert a == nxt and 0 < T <= 1024 and L <= 2048 and q.max_tile <= 1024
+        # two-step tiles: at most 768 owned, tile + ring 1 within one
+        # 1024-thread block, tile + both rings within 2048
+        assert a == nxt and 0 < T <= 768 and c <= 1024 and L <= 2048 and q.max_tile <= 768
+        step1 = max(step1, c)
         nxt = b
         inside = set(range(a, b))
         r1 = sorted(set(col[:, a:b].ravel()) - inside)
@@ -86,7 +90,7 @@ def test_plan_covers_and_rings_are_exact(d):
         # every local neighbour of tile + ring 1 names the global neighbour
         assert (glob[lc] == col[:, glob[:c]]).all()
         assert (le == elen[:, glob[:c]]).all()
-    assert nxt == V
+    assert nxt == V and q.max_step1 == step1
 
 
 def _step_rows(n_rows, Hs, Ws, Ss, lc, d):
