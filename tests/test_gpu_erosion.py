@@ -208,7 +208,8 @@ def _run_graph(g, h, iters, rain, chunk=32):
     return [g.unpermute(t.cpu().numpy()) for t in out]
 
 
-@pytest.mark.parametrize("d,iters,chunk", [(20, 101, 32), (24, 64, 0), (20, 75, 7)])
+@pytest.mark.parametrize("d,iters,chunk", [(20, 101, 32), (24, 64, 0), (20, 75, 7), (2, 9, 32),
+                                          (8, 10, 0)])
 def test_pair_launches_match_single_steps(oracle, d, iters, chunk):
     """Two steps per launch over the tile plan give the one-step-per-launch
     results and the FP64 oracle's, bit for bit."""
