@@ -313,10 +313,11 @@ def bench_export(args):
 
 
 def bench_climate(args):
-    """climate.py:449-590 on the device for a d=1000 planet (10,000,002
+    """climate.py:351-590 on the device for a d=1000 planet (10,000,002
     vertices): a year of insolation (360 daily latitude tables, each 181
-    latitudes x 360 rotation steps, then 360 interpolations per vertex) and
-    one day of 360 rotation steps per vertex (brute_daily_insolation)."""
+    latitudes x 360 rotation steps, then 360 interpolations per vertex), one
+    day of 360 rotation steps per vertex (brute_daily_insolation) and the
+    starting surface temperatures (assign_surface_temp)."""
     import torch
     from nixis_amd import climate, util
     pts, _ = util.create_mesh_device(1000, cells=False)
@@ -331,10 +332,13 @@ def bench_climate(args):
     out.zero_()
     d_ms = _events_ms(lambda: climate.sample_insolation_device(out, pts, 1.0, -180.0, 23.44, 360,
                                                                1.0, workspace=ws))
-    log(f"[climate] V={V}: year {y_ms:.2f} ms, day of 360 rotations {d_ms:.2f} ms")
+    t_ms = _events_ms(lambda: climate.assign_surface_temp_device(pts, 1.0, 23.44, out=out))
+    log(f"[climate] V={V}: year {y_ms:.2f} ms, day of 360 rotations {d_ms:.2f} ms, "
+        f"surface temperatures {t_ms:.3f} ms")
     return {"workload": f"insolation on a d=1000 icosphere ({V} vertices), tilt 23.44",
             "yearly_ms": round(y_ms, 3), "daily_360_rotations_ms": round(d_ms, 3),
-            "Gvertex_rotations_per_s": round(V * 360 / (d_ms * 1e-3) / 1e9, 2)}
+            "Gvertex_rotations_per_s": round(V * 360 / (d_ms * 1e-3) / 1e9, 2),
+            "surface_temp_ms": round(t_ms, 4)}
 
 
 def cpu_erosion_baseline(er, iters=4):

@@ -243,6 +243,12 @@ int nx_insolation_slice(nx_ctx *ctx, double radius, double tilt, float *out,
 int nx_interpolate_insolation(nx_ctx *ctx, const double *verts, int64_t V,
                               const float *table, double radius, float *out,
                               void *stream);
+/* climate.py:351-412 (assign_surface_temp): out float32 [V] =
+ * max(cos(|lat - tilt| * pi / 180), 0), the branch every vertex takes with
+ * the reference's altitude weight of 0.  The caller raises the reference's
+ * ZeroDivisionError for constant altitudes (its rescale, climate.py:374). */
+int nx_assign_surface_temp(nx_ctx *ctx, const double *verts, int64_t V,
+                           double radius, double tilt, float *out, void *stream);
 /* climate.py:536-544 (calc_daily_insolation): slice + interpolation. */
 int nx_daily_insolation(nx_ctx *ctx, const double *verts, int64_t V,
                         double radius, double tilt, float *out, void *ws,

@@ -107,6 +107,8 @@ SIGNATURES = {
                                      _c_void_p]),
     "nx_interpolate_insolation": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p, _c_double,
                                            _c_void_p, _c_void_p]),
+    "nx_assign_surface_temp": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_double, _c_double,
+                                        _c_void_p, _c_void_p]),
     "nx_daily_insolation": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_double, _c_double,
                                      _c_void_p, _c_void_p, _c_void_p]),
     "nx_yearly_insolation": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_double, _c_double,

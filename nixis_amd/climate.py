@@ -1,5 +1,5 @@
 """Drop-ins for the reference's climate.py insolation (climate.py:193-201,
-416-590) on the GPU.
+416-590) and starting surface temperatures (climate.py:351-412) on the GPU.
 
 Vertex arrays are float64 [V, 3]; results are float32 as in the reference.
 The `altitudes` arguments of the daily functions only feed a rescale whose
@@ -133,6 +133,31 @@ def calc_daily_insolation(verts, altitudes, radius, tilt):
     """climate.py:536-544."""
     _altitude_check(altitudes)
     out = daily_insolation_device(verts, radius, tilt)
+    _lib.Context.get(out.device).sync_check()
+    return out.cpu().numpy()
+
+
+def assign_surface_temp_device(verts, radius, tilt, out=None):
+    """climate.py:351-412 on the device -> float32 [V] tensor (see
+    assign_surface_temp)."""
+    import torch
+    v = _verts_dev(verts)
+    if out is None:
+        out = torch.empty(v.shape[0], dtype=torch.float32, device=v.device)
+    ctx = _lib.Context.get(v.device)
+    _lib.check(_lib.lib().nx_assign_surface_temp(ctx.handle, _lib.ptr(v), v.shape[0],
+                                                 float(radius), float(tilt), _lib.ptr(out),
+                                                 _lib.stream_ptr()))
+    return out
+
+
+def assign_surface_temp(verts, altitudes, radius, tilt):
+    """climate.py:351-412: starting surface temperatures, float32 [V].  The
+    reference weights altitude by 0 (h2 = rescale(altitudes, 0, 0) is 0), so
+    each vertex gets max(cos(|lat - tilt| * pi / 180), 0); the altitudes
+    only raise ZeroDivisionError when constant, as its rescale does."""
+    _altitude_check(altitudes)
+    out = assign_surface_temp_device(verts, radius, tilt)
     _lib.Context.get(out.device).sync_check()
     return out.cpu().numpy()
 

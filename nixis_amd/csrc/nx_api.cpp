@@ -260,6 +260,12 @@ int nx_interpolate_insolation(nx_ctx *ctx, const double *verts, int64_t V, const
     return interpolate_insolation(ctx, verts, V, table, radius, out, (hipStream_t)stream);
 }
 
+int nx_assign_surface_temp(nx_ctx *ctx, const double *verts, int64_t V, double radius,
+                           double tilt, float *out, void *stream)
+{
+    return surface_temp(ctx, verts, V, radius, tilt, out, (hipStream_t)stream);
+}
+
 int nx_yearly_insolation(nx_ctx *ctx, const double *verts, int64_t V, double radius,
                          double axial_tilt, float *out, void *ws, void *stream)
 {

@@ -71,6 +71,21 @@ __device__ __forceinline__ double insol_value(double x, double y, double z, doub
     return py_max(cos(fabs(lat) * pi / 180), 0.0) * py_max(cos(lon * pi / 180), 0.0);
 }
 
+// climate.py:351-412 (assign_surface_temp).  The reference's altitude weight
+// is 0 (climate.py:373-374), so h2 = rescale(altitudes, 0, 0) is 0 at every
+// vertex and every vertex takes the else branch (climate.py:408):
+// max(cos(|lat - tilt| * pi / 180), 0), stored as float32.
+__global__ __launch_bounds__(256) void k_surface_temp(const double *__restrict__ verts,
+                                                      int64_t V, double r, double tilt,
+                                                      float *__restrict__ out)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const double pi = 3.141592653589793;
+    const double dist = lat_of(verts[3 * v + 2], r) - tilt;
+    out[v] = (float)py_max(cos(fabs(dist) * pi / 180), 0.0);
+}
+
 }  // namespace
 
 // nsteps rotations of each vertex, accumulated into arr (float32).
@@ -251,6 +266,19 @@ int interpolate_insolation(nx_ctx *ctx, const double *verts, int64_t V, const fl
     NX_HIP(hipSetDevice(ctx->device));
     hipLaunchKernelGGL(k_interpolate, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, stream,
                        verts, V, radius, table, 1, out);
+    NX_HIP(hipGetLastError());
+    return NX_OK;
+}
+
+int surface_temp(nx_ctx *ctx, const double *verts, int64_t V, double radius, double tilt,
+                 float *out, hipStream_t stream)
+{
+    if (!ctx) return set_error(NX_EINVAL, "null context");
+    if (V < 0 || (V > 0 && (!verts || !out))) return set_error(NX_EINVAL, "bad vertices / output");
+    if (V == 0) return NX_OK;
+    NX_HIP(hipSetDevice(ctx->device));
+    hipLaunchKernelGGL(k_surface_temp, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, stream,
+                       verts, V, radius, tilt, out);
     NX_HIP(hipGetLastError());
     return NX_OK;
 }

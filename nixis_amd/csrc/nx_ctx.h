@@ -112,6 +112,8 @@ int insolation_slice(nx_ctx *ctx, double radius, double tilt, float *out, void *
                      hipStream_t stream);
 int daily_insolation(nx_ctx *ctx, const double *verts, int64_t V, double radius, double tilt,
                      float *out, void *ws, hipStream_t stream);
+int surface_temp(nx_ctx *ctx, const double *verts, int64_t V, double radius, double tilt,
+                 float *out, hipStream_t stream);
 int interpolate_insolation(nx_ctx *ctx, const double *verts, int64_t V, const float *table,
                            double radius, float *out, hipStream_t stream);
 int yearly_insolation(nx_ctx *ctx, const double *verts, int64_t V, double radius,

@@ -1204,6 +1204,33 @@ void ora_interpolate_insolation(const double *verts, int64_t V, const float *tab
     }
 }
 
+/* climate.py:351-412 (assign_surface_temp): h2 = rescale(altitudes, 0,
+ * alt_intensity) with the reference's alt_intensity of 0 (climate.py:373-374),
+ * then climate.py:408 per vertex; -1 when the altitudes are constant (the
+ * rescale's division by zero). */
+int ora_assign_surface_temp(const double *verts, const double *altitudes, int64_t V,
+                            double radius, double tilt, float *out)
+{
+    const double pi = 3.141592653589793;
+    const double alt_intensity = 0;
+    double *h2 = (double *)malloc(sizeof(double) * (size_t)(V > 0 ? V : 1));
+    if (!h2) return -2;
+    if (ora_rescale(altitudes, V, 0, alt_intensity, h2)) {
+        free(h2);
+        return -1;
+    }
+#pragma omp parallel for schedule(static)
+    for (int64_t v = 0; v < V; ++v) {
+        double lat, lon;
+        ora_xyz2latlon(verts[3 * v], verts[3 * v + 1], verts[3 * v + 2], radius, &lat, &lon);
+        const double dist = lat - tilt;
+        const double c = py_max(cos(fabs(dist) * pi / 180), 0.0);
+        out[v] = (float)(h2[v] > 0 ? c - fabs(h2[v]) + 0.1 : c);
+    }
+    free(h2);
+    return 0;
+}
+
 void ora_yearly_insolation(const double *verts, int64_t V, double radius, double axial_tilt,
                            float *out)
 {

@@ -130,6 +130,9 @@ void ora_sample_insolation(float *arr, const double *verts, int64_t V, double ra
 /* climate.py:503-533 (calc_insolation_slice): out[181], index = latitude
  * with negative latitudes at 181 + lat. */
 void ora_insolation_slice(double radius, double tilt, float *out);
+/* climate.py:351-412 (assign_surface_temp); -1 for constant altitudes */
+int ora_assign_surface_temp(const double *verts, const double *altitudes, int64_t V,
+                            double radius, double tilt, float *out);
 /* climate.py:547-573 (interpolate_insolation) */
 void ora_interpolate_insolation(const double *verts, int64_t V, const float *table, float *out,
                                 double radius);

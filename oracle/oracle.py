@@ -70,6 +70,8 @@ def lib():
         L.ora_insolation_slice.argtypes = [_d, _d, _p]
         L.ora_interpolate_insolation.argtypes = [_p, _i64, _p, _p, _d]
         L.ora_yearly_insolation.argtypes = [_p, _i64, _d, _d, _p]
+        L.ora_assign_surface_temp.argtypes = [_p, _p, _i64, _d, _d, _p]
+        L.ora_assign_surface_temp.restype = _i32
         _lib = L
     return _lib
 
@@ -349,6 +351,20 @@ def interpolate_insolation(verts, table, radius):
     t = np.ascontiguousarray(table, dtype=np.float32)
     out = np.empty(v.shape[0], np.float32)
     lib().ora_interpolate_insolation(_ptr(v), v.shape[0], _ptr(t), _ptr(out), float(radius))
+    return out
+
+
+def assign_surface_temp(verts, altitudes, radius, tilt):
+    """climate.py:351-412 -> float32 [V]; ZeroDivisionError for constant
+    altitudes, as the reference's rescale raises."""
+    v = _f64(verts)
+    a = _f64(altitudes).reshape(-1)
+    out = np.empty(v.shape[0], np.float32)
+    rc = lib().ora_assign_surface_temp(_ptr(v), _ptr(a), v.shape[0], float(radius), float(tilt),
+                                       _ptr(out))
+    if rc == -1:
+        raise ZeroDivisionError("rescale: altitude range is zero")
+    assert rc == 0
     return out
 
 

@@ -105,3 +105,19 @@ def test_yearly_full_size_properties():
     assert (y >= 0).all() and y.max() <= 360 * eq_day * (1 + 1e-5)
     z = pts[:, 2].cpu().numpy()
     assert y[np.abs(z) < 0.01].mean() > y[np.abs(z) > 0.99].mean()
+
+
+@pytest.mark.parametrize("tilt", [0.0, 23.44, -61.0])
+def test_surface_temp_matches_oracle(oracle, tilt):
+    """climate.py:351-412: the device's asin and cos against glibc's, so one
+    float32 ulp of the result; values in [0, 1]."""
+    pts, _ = util.create_mesh(64)
+    alts = np.linspace(-9000, 9000, pts.shape[0])
+    got = climate.assign_surface_temp(pts, alts, 1.0, tilt)
+    ref = oracle.assign_surface_temp(pts, alts, 1.0, tilt)
+    _report(f"surface temp tilt {tilt}", got, ref, 1)
+    assert got.min() >= 0 and got.max() <= 1
+    with pytest.raises(ZeroDivisionError):
+        climate.assign_surface_temp(pts, np.full(pts.shape[0], 3.0), 1.0, tilt)
+    dev = climate.assign_surface_temp_device(torch.from_numpy(pts).cuda(), 1.0, tilt)
+    assert np.array_equal(dev.cpu().numpy(), got)

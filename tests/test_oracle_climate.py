@@ -70,3 +70,27 @@ def test_year_at_zero_tilt_is_360_days(oracle):
     y = oracle.yearly_insolation(verts, 1.0, 0.0)
     day = oracle.interpolate_insolation(verts, oracle.insolation_slice(1.0, 0.0), 1.0)
     assert np.allclose(y, 360.0 * day, rtol=1e-5, atol=0)
+
+
+def test_surface_temp_closed_forms(oracle):
+    """climate.py:351-412: max(cos(|lat - tilt| deg), 0) whatever the
+    altitudes (their weight is 0): 1 where the latitude equals the tilt,
+    cos(lat - tilt) elsewhere, 0 beyond 90 degrees from it, north and south
+    mirrored about the tilt."""
+    lats = [-80.0, -30.0, 0.0, 23.44, 45.0, 89.0]
+    verts = np.array([_ll(a, 11.0 * i) for i, a in enumerate(lats)])
+    for alts in (np.linspace(-5, 5, len(lats)), np.array([1e4, -3, 0, 7, 2, 1e-3])):
+        t = oracle.assign_surface_temp(verts, alts, 1.0, 23.44)
+        assert t.dtype == np.float32
+        for a, v in zip(lats, t):
+            assert abs(v - max(math.cos(math.radians(abs(a - 23.44))), 0.0)) < 1e-6
+    # exactly at the tilt and exactly 90 degrees away
+    t = oracle.assign_surface_temp(np.array([_ll(30.0), _ll(-60.0), _ll(-70.0)]),
+                                   np.array([0.0, 1.0, 2.0]), 1.0, 30.0)
+    assert abs(t[0] - 1.0) < 1e-6 and 0 <= t[1] < 1e-6 and t[2] == 0.0
+
+
+def test_surface_temp_constant_altitudes_raise(oracle):
+    import pytest
+    with pytest.raises(ZeroDivisionError):
+        oracle.assign_surface_temp(np.array([_ll(0.0), _ll(10.0)]), np.zeros(2), 1.0, 0.0)
