@@ -267,6 +267,43 @@ __global__ __launch_bounds__(256) void k_erode_batch(const BatchPlanetDev *__res
 
 // Launch k of the two-step phase: steps it + 2k and it + 2k + 1 of every
 // tile of the first `active` planets of `order` (one block per tile).
+// A two-step launch over at most kArgPlanets planets takes what each block
+// needs in its kernel arguments (tile prefix, plan, buffers, step counter,
+// rain), so a block finds its planet among scalar argument loads instead of
+// a search over device arrays followed by the descriptor.
+constexpr int kArgPlanets = 16;
+struct PairPlanetArgs {
+    TilePlanDev plan;
+    double2 *hw0, *hw1;
+    double *s0, *s1;
+    const ErosionMeta *meta;
+    int64_t total_iters;
+    double rain;
+};
+struct PairLaunchArgs {
+    int64_t prefix[kArgPlanets + 1];
+    PairPlanetArgs pl[kArgPlanets];
+    int32_t active;
+};
+
+__global__ __launch_bounds__(1024) void k_erode_batch_pair_args(const PairLaunchArgs a,
+                                                                int64_t nt_total, int64_t k)
+{
+    __shared__ double2 A[kLoadMax];
+    __shared__ double2 B[kLoadMax];
+    const int64_t b = xcd_tile(blockIdx.x, nt_total);
+    if (b >= nt_total) return;
+    int r = 0;
+    while (r + 1 < a.active && a.prefix[r + 1] <= b) ++r;
+    const PairPlanetArgs &Q = a.pl[r];
+    const int64_t t = Q.meta->it + 2 * k;
+    const int p = (Q.meta->buf + (int)(k & 1)) & 1;
+    const double rain_mid = t + 1 < Q.total_iters ? Q.rain : 0.0;
+    const double rain_next = t + 2 < Q.total_iters ? Q.rain : 0.0;
+    erode_tile_pair<1024, true>(A, B, Q.hw0, Q.hw1, Q.s0, Q.s1, p, Q.plan, b - a.prefix[r],
+                                rain_mid, rain_next);
+}
+
 __global__ __launch_bounds__(1024) void k_erode_batch_pair(const BatchPlanetDev *__restrict__ P,
                                                            const int32_t *__restrict__ order,
                                                            const int64_t *__restrict__ prefix,
@@ -513,13 +550,28 @@ int batch_erode(nx_ctx *ctx, nx_planet *planets, int n, int max_steps, void *ws,
     const auto *SB = (const int64_t *)(w + L.one_base);
     const auto *ST = (const int64_t *)(w + L.steps);
     const auto *LN = (const int32_t *)(w + L.launches);
+    // Kernel-argument form of the first kArgPlanets planets in pair order.
+    PairLaunchArgs pargs;
+    std::memset(&pargs, 0, sizeof(pargs));
+    for (int r = 0; r < std::min(activeP0, kArgPlanets); ++r) {
+        const BatchPlanetDev &d = desc[ordP[r]];
+        pargs.prefix[r] = tpre[r];
+        pargs.pl[r] = PairPlanetArgs{d.plan, d.hw0, d.hw1, d.s0, d.s1, d.meta, d.total_iters, d.rain};
+    }
     auto launch_all = [&](hipStream_t s) {
         int active = activeP0;
         for (int64_t k = 0; k < KP; ++k) {
             while (active > 0 && pairs[ordP[active - 1]] <= k) --active;
             const int64_t nt = tpre[active];
-            hipLaunchKernelGGL(k_erode_batch_pair, dim3((unsigned)(((nt + 7) / 8) * 8)), dim3(1024), 0, s,
-                               D, OP, TP, active, nt, k);
+            const dim3 grid((unsigned)(((nt + 7) / 8) * 8));
+            if (active <= kArgPlanets) {
+                pargs.active = active;
+                pargs.prefix[active] = nt;
+                hipLaunchKernelGGL(k_erode_batch_pair_args, grid, dim3(1024), 0, s, pargs, nt, k);
+            } else {
+                hipLaunchKernelGGL(k_erode_batch_pair, grid, dim3(1024), 0, s, D, OP, TP, active, nt,
+                                   k);
+            }
         }
         active = activeS0;
         for (int64_t k = 0; k < KS; ++k) {
