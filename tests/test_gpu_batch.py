@@ -72,6 +72,25 @@ def test_batch_graph_replays_and_chunks(meshes, plain):
         assert (a == b).all(), f"chunked planet {k}"
 
 
+@pytest.mark.parametrize("chunk", [5, 6])
+def test_batch_one_planet_chunks_under_graphs(meshes, plain, chunk):
+    """A planet alone in a pair launch takes the single-planet launch, whose
+    rain comes from the host's step count: graph-captured chunks of 5 steps
+    (two pairs and a single step) or 6 (ending on a pair with no rain after
+    it), run twice so the second pass replays the cached graphs, give the
+    single-planet path's heights."""
+    job = _jobs()[2]   # 40 steps
+    runs = _runs([job], meshes)
+    br = batch.BatchRunner()
+    for rep in range(2):
+        br.noise(runs)
+        while runs[0].steps_done < job.erosion_iters:
+            br.erode(runs, chunk, use_graph=True)
+        br.store(runs)
+        br.ctx.sync_check()
+        assert (_heights(runs)[0] == plain[2]).all(), f"pass {rep}"
+
+
 def test_batch_alone_equals_in_mix(meshes, plain):
     """A planet's result is the same alone and inside a larger batch."""
     jobs = _jobs()
