@@ -1,12 +1,12 @@
-"""Why the erosion fields are float-float: a CPU emulation on a small planet.
+"""Why the erosion fields are not FP32: a CPU emulation on a small planet.
 
 erosion.py:197-279 doubles the water (and with it the heights' spread) about
 every step, and the flow direction at each edge is decided by h[n] > h[i].
 Emulating the step with every operation rounded to K significant bits shows
 that K=24 (FP32) flips those comparisons and ends far from the FP64 result,
-while K=48 (the float-float pairs the kernels use) stays at the 1e-12 level.
-This is the evidence behind the kernels' field format; the kernels
-themselves are checked against the oracle in tests/test_gpu_erosion.py.
+while K=48 (what float-float pairs would keep) stays at the 1e-12 level.
+The kernels keep FP64 fields (K=53) in the reference's operation order and
+are checked against the oracle bit for bit in tests/test_gpu_erosion.py.
 """
 import numpy as np
 import pytest

@@ -103,7 +103,7 @@ def test_erosion_flat_field_closed_form():
 
 def test_erosion_mass_per_vertex_conserved():
     """h + s is conserved per vertex by every step (erosion.py:262-267); the
-    FP64 mass check sees it to within the fields' float-float rounding."""
+    FP64 mass check sees it to within FP64 rounding."""
     pts, nb, h = _planet(16)
     g = erosion.ErosionGraph(pts, nb)
     st = erosion.ErosionState(g.V)
