@@ -290,7 +290,10 @@ __global__ __launch_bounds__(1024) void k_erode_batch_pair_one(const ErosionMeta
 // A two-step launch over at most kArgPlanets planets takes what each block
 // needs in its kernel arguments (tile prefix, plan, buffers, step counter,
 // rain), so a block finds its planet among scalar argument loads instead of
-// a search over device arrays followed by the descriptor.
+// a search over device arrays followed by the descriptor.  The prefix is
+// int32 and padded past the launch's planets with values no tile index
+// reaches, so the planet is a count of prefix entries <= the tile: fixed
+// argument offsets, loaded together and counted with scalar arithmetic.
 constexpr int kArgPlanets = 16;
 struct PairPlanetArgs {
     TilePlanDev plan;
@@ -301,20 +304,23 @@ struct PairPlanetArgs {
     double rain;
 };
 struct PairLaunchArgs {
-    int64_t prefix[kArgPlanets + 1];
+    int32_t prefix[kArgPlanets];   // prefix[q] = first tile of planet q (q < active)
     PairPlanetArgs pl[kArgPlanets];
-    int32_t active;
 };
 
 __global__ __launch_bounds__(1024) void k_erode_batch_pair_args(const PairLaunchArgs a,
-                                                                int64_t nt_total, int64_t k)
+                                                                int32_t nt_total, int64_t k)
 {
     __shared__ double2 A[kLoadMax];
     __shared__ double2 B[kLoadMax];
-    const int64_t b = xcd_tile(blockIdx.x, nt_total);
+    const int b = (int)xcd_tile(blockIdx.x, nt_total);
     if (b >= nt_total) return;
-    int r = 0;
-    while (r + 1 < a.active && a.prefix[r + 1] <= b) ++r;
+    // b and the prefix entries lie in [0, 2^31): b - prefix[q] is negative,
+    // its sign bit set, exactly when planet q starts after tile b.
+    int after = 0;
+#pragma unroll
+    for (int q = 1; q < kArgPlanets; ++q) after += (int)(((uint32_t)b - (uint32_t)a.prefix[q]) >> 31);
+    const int r = (kArgPlanets - 1) - after;
     const PairPlanetArgs &Q = a.pl[r];
     const int64_t t = Q.meta->it + 2 * k;
     const int p = (Q.meta->buf + (int)(k & 1)) & 1;
@@ -573,11 +579,13 @@ int batch_erode(nx_ctx *ctx, nx_planet *planets, int n, int max_steps, void *ws,
     // Kernel-argument form of the first kArgPlanets planets in pair order.
     PairLaunchArgs pargs;
     std::memset(&pargs, 0, sizeof(pargs));
+    for (int r = 0; r < kArgPlanets; ++r) pargs.prefix[r] = INT32_MAX;
     for (int r = 0; r < std::min(activeP0, kArgPlanets); ++r) {
         const BatchPlanetDev &d = desc[ordP[r]];
-        pargs.prefix[r] = tpre[r];
+        pargs.prefix[r] = (int32_t)tpre[r];   // tpre[kArgPlanets] <= INT32_MAX, checked below
         pargs.pl[r] = PairPlanetArgs{d.plan, d.hw0, d.hw1, d.s0, d.s1, d.meta, d.total_iters, d.rain};
     }
+    const bool args_fit = tpre[std::min(activeP0, kArgPlanets)] <= INT32_MAX;
     auto launch_all = [&](hipStream_t s) {
         int active = activeP0;
         for (int64_t k = 0; k < KP; ++k) {
@@ -592,10 +600,10 @@ int batch_erode(nx_ctx *ctx, nx_planet *planets, int n, int max_steps, void *ws,
                 const double rain_next = t + 2 < d.total_iters ? d.rain : 0.0;
                 hipLaunchKernelGGL(k_erode_batch_pair_one, grid, dim3(1024), 0, s, d.meta, d.hw0,
                                    d.hw1, d.s0, d.s1, d.plan, (int)(k & 1), rain_mid, rain_next);
-            } else if (active <= kArgPlanets) {
-                pargs.active = active;
-                pargs.prefix[active] = nt;
-                hipLaunchKernelGGL(k_erode_batch_pair_args, grid, dim3(1024), 0, s, pargs, nt, k);
+            } else if (active <= kArgPlanets && args_fit) {
+                // entries past the active planets start at or after nt
+                hipLaunchKernelGGL(k_erode_batch_pair_args, grid, dim3(1024), 0, s, pargs,
+                                   (int32_t)nt, k);
             } else {
                 hipLaunchKernelGGL(k_erode_batch_pair, grid, dim3(1024), 0, s, D, OP, TP, active, nt,
                                    k);
