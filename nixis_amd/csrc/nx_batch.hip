@@ -63,8 +63,17 @@ constexpr int kRsTile = 4096;  // vertices per block in the rescale / load / sto
 
 // Workspace layout (offsets in bytes, all 256-aligned).
 struct BatchLayout {
-    size_t counter, keys, desc, noise_prefix, rs_prefix, pair_prefix, pair_order, one_prefix,
-        one_order, one_base, steps, launches, total;
+    size_t counter, keys, desc, noise_prefix, rs_prefix, pair_prefix, pair_desc, pair_bucket,
+        one_prefix, one_order, one_base, steps, launches, total;
+};
+
+// Planet lookup of the many-planet pair kernel: tile b falls in bucket
+// b >> shift; the bucket holds the planet (in pair order) of its first tile
+// and the first tile of the next planet.
+constexpr int kPairBuckets = 1024;
+struct PairBucket {
+    int32_t r;
+    int32_t next;
 };
 
 static inline size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -84,7 +93,8 @@ static BatchLayout layout(int n)
     L.noise_prefix = take(sizeof(int64_t) * (n + 1));
     L.rs_prefix = take(sizeof(int64_t) * (n + 1));
     L.pair_prefix = take(sizeof(int64_t) * (n + 1));
-    L.pair_order = take(sizeof(int32_t) * n);
+    L.pair_desc = take(sizeof(BatchPlanetDev) * n);
+    L.pair_bucket = take(sizeof(PairBucket) * kPairBuckets);
     L.one_prefix = take(sizeof(int64_t) * (n + 1));
     L.one_order = take(sizeof(int32_t) * n);
     L.one_base = take(sizeof(int64_t) * n);
@@ -330,8 +340,10 @@ __global__ __launch_bounds__(1024) void k_erode_batch_pair_args(const PairLaunch
                                 rain_mid, rain_next);
 }
 
-__global__ __launch_bounds__(1024) void k_erode_batch_pair(const BatchPlanetDev *__restrict__ P,
-                                                           const int32_t *__restrict__ order,
+// PD: descriptors in pair order; prefix[r] = first tile of planet r.
+__global__ __launch_bounds__(1024) void k_erode_batch_pair(const BatchPlanetDev *__restrict__ PD,
+                                                           const PairBucket *__restrict__ bucket,
+                                                           int shift,
                                                            const int64_t *__restrict__ prefix,
                                                            int active, int64_t nt_total,
                                                            int64_t k)
@@ -342,9 +354,16 @@ __global__ __launch_bounds__(1024) void k_erode_batch_pair(const BatchPlanetDev 
     if (b >= nt_total) return;
     // The planet is the same for the whole block: keep its descriptor in
     // scalar registers (a local copy, so stores to the fields cannot be
-    // taken to alias it).
-    const int r = __builtin_amdgcn_readfirstlane(find_slot(prefix, active, b));
-    const BatchPlanetDev &D = P[order[r]];
+    // taken to alias it).  Its bucket gives it unless the bucket holds the
+    // start of another planet at or before b.
+    const PairBucket q = bucket[b >> shift];
+    int r = q.r;
+    if (b >= q.next) {
+        ++r;
+        while (r + 1 < active && prefix[r + 1] <= b) ++r;
+    }
+    r = __builtin_amdgcn_readfirstlane(r);
+    const BatchPlanetDev &D = PD[r];
     const TilePlanDev plan = D.plan;
     double2 *hw0 = D.hw0, *hw1 = D.hw1;
     double *s0 = D.s0, *s1 = D.s1;
@@ -555,12 +574,27 @@ int batch_erode(nx_ctx *ctx, nx_planet *planets, int n, int max_steps, void *ws,
         bpre[r + 1] = bpre[r] + (r < activeS0 ? desc[ordS[r]].nblocks : 0);
         sbase[r] = pairs[ordS[r]];
     }
+    // Pair-ordered descriptors and the bucket table of the many-planet pair
+    // kernel (tile b is in bucket b >> shift, at most kPairBuckets of them).
+    std::vector<BatchPlanetDev> descP(n);
+    for (int r = 0; r < n; ++r) descP[r] = desc[ordP[r]];
+    const int64_t ntP = tpre[activeP0];
+    if (ntP > INT32_MAX) return set_error(NX_EINVAL, "%lld tiles in one batch exceed 2^31 - 1", (long long)ntP);
+    int shift = 0;
+    while (((ntP - 1) >> shift) >= kPairBuckets) ++shift;
+    std::vector<PairBucket> buckets(kPairBuckets, PairBucket{0, INT32_MAX});
+    for (int j = 0, r = 0; j < kPairBuckets && ((int64_t)j << shift) < ntP; ++j) {
+        const int64_t first = (int64_t)j << shift;
+        while (r + 1 < activeP0 && tpre[r + 1] <= first) ++r;
+        buckets[j] = PairBucket{r, r + 1 < activeP0 ? (int32_t)tpre[r + 1] : INT32_MAX};
+    }
     char *w = (char *)ws;
     NX_HIP(hipSetDevice(ctx->device));
     int rc = upload(ctx, w, L,
                     {{L.desc, desc.data(), sizeof(BatchPlanetDev) * n},
                      {L.pair_prefix, tpre.data(), sizeof(int64_t) * (n + 1)},
-                     {L.pair_order, ordP.data(), sizeof(int32_t) * n},
+                     {L.pair_desc, descP.data(), sizeof(BatchPlanetDev) * n},
+                     {L.pair_bucket, buckets.data(), sizeof(PairBucket) * kPairBuckets},
                      {L.one_prefix, bpre.data(), sizeof(int64_t) * (n + 1)},
                      {L.one_order, ordS.data(), sizeof(int32_t) * n},
                      {L.one_base, sbase.data(), sizeof(int64_t) * n},
@@ -569,7 +603,8 @@ int batch_erode(nx_ctx *ctx, nx_planet *planets, int n, int max_steps, void *ws,
                     stream);
     if (rc) return rc;
     const auto *D = (const BatchPlanetDev *)(w + L.desc);
-    const auto *OP = (const int32_t *)(w + L.pair_order);
+    const auto *PDv = (const BatchPlanetDev *)(w + L.pair_desc);
+    const auto *PB = (const PairBucket *)(w + L.pair_bucket);
     const auto *TP = (const int64_t *)(w + L.pair_prefix);
     const auto *OS = (const int32_t *)(w + L.one_order);
     const auto *BP = (const int64_t *)(w + L.one_prefix);
@@ -605,7 +640,7 @@ int batch_erode(nx_ctx *ctx, nx_planet *planets, int n, int max_steps, void *ws,
                 hipLaunchKernelGGL(k_erode_batch_pair_args, grid, dim3(1024), 0, s, pargs,
                                    (int32_t)nt, k);
             } else {
-                hipLaunchKernelGGL(k_erode_batch_pair, grid, dim3(1024), 0, s, D, OP, TP, active, nt,
+                hipLaunchKernelGGL(k_erode_batch_pair, grid, dim3(1024), 0, s, PDv, PB, shift, TP, active, nt,
                                    k);
             }
         }

@@ -91,6 +91,33 @@ def test_batch_one_planet_chunks_under_graphs(meshes, plain, chunk):
         assert (_heights(runs)[0] == plain[2]).all(), f"pass {rep}"
 
 
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batch_many_planets_equals_plain(meshes, use_graph):
+    """More than 16 eroding planets take the many-planet pair kernel, which
+    finds a tile's planet through a table of buckets over the tiles (two
+    tiles per bucket here, so some buckets hold the start of a second
+    planet and one-tile planets share buckets): every planet equals its
+    single-planet result, with and without graphs."""
+    rng = np.random.default_rng(8)
+    ds = rng.choice([6, 8, 12, 40, 64, 90, 120], size=24)
+    its = rng.integers(2, 12, size=24)
+    jobs = [batch.Job(d=int(d), seed=k, n_octaves=4, erosion_iters=int(it), rain=1.0)
+            for k, (d, it) in enumerate(zip(ds, its))]
+    ref = _runs(jobs, meshes)
+    batch.run_plain(ref)
+    want = _heights(ref)
+    runs = _runs(jobs, meshes)
+    br = batch.BatchRunner()
+    br.run(runs, use_graph=use_graph)
+    br.ctx.sync_check()
+    plans = [meshes.graph(j.d).plan for j in jobs]
+    assert all(q is not None for q in plans)
+    tiles = sum(q.ntiles for q in plans)
+    assert tiles > 1024, tiles
+    for k, (a, b) in enumerate(zip(_heights(runs), want)):
+        assert (a == b).all(), f"planet {k} (d={jobs[k].d}) differs from the single-planet path"
+
+
 def test_batch_alone_equals_in_mix(meshes, plain):
     """A planet's result is the same alone and inside a larger batch."""
     jobs = _jobs()
