@@ -278,23 +278,27 @@ __global__ __launch_bounds__(256) void k_erode_batch(const BatchPlanetDev *__res
 // Launch k of the two-step phase: steps it + 2k and it + 2k + 1 of every
 // tile of the first `active` planets of `order` (one block per tile).
 // A two-step launch that covers one planet takes the single-planet kernel's
-// arguments: the buffers and plan, the launch's parity and the two steps'
-// rain, computed on the host from the planet's step count (the step counter
-// on the device equals steps_done when a call starts).
+// arguments (buffers and plan by value) with the planet's step count and
+// rain; the step counter and buffer index come from one load of the
+// planet's meta, as in the single-planet kernel.
 __global__ __launch_bounds__(1024) void k_erode_batch_pair_one(const ErosionMeta *__restrict__ meta,
                                                                double2 *__restrict__ hw0,
                                                                double2 *__restrict__ hw1,
                                                                double *__restrict__ s0,
                                                                double *__restrict__ s1,
-                                                               TilePlanDev plan, int j,
-                                                               double rain_mid, double rain_next)
+                                                               TilePlanDev plan, int64_t k,
+                                                               int64_t total_iters, double rain)
 {
     __shared__ double2 A[kLoadMax];
     __shared__ double2 B[kLoadMax];
     const int64_t tile = xcd_tile(blockIdx.x, plan.ntiles);
     if (tile >= plan.ntiles) return;
-    erode_tile_pair<1024, true>(A, B, hw0, hw1, s0, s1, (meta->buf + j) & 1, plan, tile, rain_mid,
-                                rain_next);
+    const ErosionMeta m = *meta;
+    const int64_t t = m.it + 2 * k;
+    const double rain_mid = t + 1 < total_iters ? rain : 0.0;
+    const double rain_next = t + 2 < total_iters ? rain : 0.0;
+    erode_tile_pair<1024, true>(A, B, hw0, hw1, s0, s1, (m.buf + (int)(k & 1)) & 1, plan, tile,
+                                rain_mid, rain_next);
 }
 
 // A two-step launch over at most kArgPlanets planets takes what each block
@@ -628,13 +632,9 @@ int batch_erode(nx_ctx *ctx, nx_planet *planets, int n, int max_steps, void *ws,
             const int64_t nt = tpre[active];
             const dim3 grid((unsigned)(((nt + 7) / 8) * 8));
             if (active == 1) {
-                const nx_planet &q = planets[ordP[0]];
                 const BatchPlanetDev &d = desc[ordP[0]];
-                const int64_t t = q.steps_done + 2 * k;
-                const double rain_mid = t + 1 < d.total_iters ? d.rain : 0.0;
-                const double rain_next = t + 2 < d.total_iters ? d.rain : 0.0;
                 hipLaunchKernelGGL(k_erode_batch_pair_one, grid, dim3(1024), 0, s, d.meta, d.hw0,
-                                   d.hw1, d.s0, d.s1, d.plan, (int)(k & 1), rain_mid, rain_next);
+                                   d.hw1, d.s0, d.s1, d.plan, k, d.total_iters, d.rain);
             } else if (active <= kArgPlanets && args_fit) {
                 // entries past the active planets start at or after nt
                 hipLaunchKernelGGL(k_erode_batch_pair_args, grid, dim3(1024), 0, s, pargs,
@@ -657,14 +657,11 @@ int batch_erode(nx_ctx *ctx, nx_planet *planets, int n, int max_steps, void *ws,
         launch_all(stream);
         NX_HIP(hipGetLastError());
     } else {
-        // Cache key: workspace, the descriptor contents, the schedule and
-        // the step counts it starts from (single-planet launches take their
-        // rain from them).
+        // Cache key: workspace, the descriptor contents and the schedule.
         uint64_t h = 1469598103934665603ull;
         h = fnv(desc.data(), sizeof(BatchPlanetDev) * n, h);
         h = fnv(pairs.data(), sizeof(int64_t) * n, h);
         h = fnv(ones.data(), sizeof(int64_t) * n, h);
-        for (int p = 0; p < n; ++p) h = fnv(&planets[p].steps_done, sizeof(int64_t), h);
         GraphKey key{ws, (int64_t)n, nullptr, nullptr, (int)(KP + KS), 0.0, true, h};
         hipGraphExec_t ex = nullptr;
         {
