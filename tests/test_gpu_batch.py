@@ -74,11 +74,11 @@ def test_batch_graph_replays_and_chunks(meshes, plain):
 
 @pytest.mark.parametrize("chunk", [5, 6])
 def test_batch_one_planet_chunks_under_graphs(meshes, plain, chunk):
-    """A planet alone in a pair launch takes the single-planet launch, whose
-    rain comes from the host's step count: graph-captured chunks of 5 steps
+    """A planet alone in a pair launch takes the one-planet kernel, which
+    reads its step counter on the device: graph-captured chunks of 5 steps
     (two pairs and a single step) or 6 (ending on a pair with no rain after
-    it), run twice so the second pass replays the cached graphs, give the
-    single-planet path's heights."""
+    it), run twice so the second pass replays the cached graphs at other
+    step counts, give the single-planet path's heights."""
     job = _jobs()[2]   # 40 steps
     runs = _runs([job], meshes)
     br = batch.BatchRunner()
