@@ -275,8 +275,10 @@ __global__ __launch_bounds__(256) void k_erode_batch(const BatchPlanetDev *__res
     if (i < D.V) erode_vertex_step(i, D.V, hw_src, s_src, hw_dst, s_dst, D.col, D.elen, rain);
 }
 
-// Launch k of the two-step phase: steps it + 2k and it + 2k + 1 of every
-// tile of the first `active` planets of `order` (one block per tile).
+// Launch k of the two-step phase runs steps it + 2k and it + 2k + 1 of every
+// tile of the first `active` planets in pair order (one block per tile), in
+// one of three kernels by the number of planets it covers.
+//
 // A two-step launch that covers one planet takes the single-planet kernel's
 // arguments (buffers and plan by value) with the planet's step count and
 // rain; the step counter and buffer index come from one load of the
@@ -344,7 +346,8 @@ __global__ __launch_bounds__(1024) void k_erode_batch_pair_args(const PairLaunch
                                 rain_mid, rain_next);
 }
 
-// PD: descriptors in pair order; prefix[r] = first tile of planet r.
+// A launch over more planets reads their descriptors (PD, in pair order;
+// prefix[r] = first tile of planet r) and finds its planet in the buckets.
 __global__ __launch_bounds__(1024) void k_erode_batch_pair(const BatchPlanetDev *__restrict__ PD,
                                                            const PairBucket *__restrict__ bucket,
                                                            int shift,
@@ -621,10 +624,9 @@ int batch_erode(nx_ctx *ctx, nx_planet *planets, int n, int max_steps, void *ws,
     for (int r = 0; r < kArgPlanets; ++r) pargs.prefix[r] = INT32_MAX;
     for (int r = 0; r < std::min(activeP0, kArgPlanets); ++r) {
         const BatchPlanetDev &d = desc[ordP[r]];
-        pargs.prefix[r] = (int32_t)tpre[r];   // tpre[kArgPlanets] <= INT32_MAX, checked below
+        pargs.prefix[r] = (int32_t)tpre[r];   // <= ntP <= INT32_MAX, checked above
         pargs.pl[r] = PairPlanetArgs{d.plan, d.hw0, d.hw1, d.s0, d.s1, d.meta, d.total_iters, d.rain};
     }
-    const bool args_fit = tpre[std::min(activeP0, kArgPlanets)] <= INT32_MAX;
     auto launch_all = [&](hipStream_t s) {
         int active = activeP0;
         for (int64_t k = 0; k < KP; ++k) {
@@ -635,13 +637,13 @@ int batch_erode(nx_ctx *ctx, nx_planet *planets, int n, int max_steps, void *ws,
                 const BatchPlanetDev &d = desc[ordP[0]];
                 hipLaunchKernelGGL(k_erode_batch_pair_one, grid, dim3(1024), 0, s, d.meta, d.hw0,
                                    d.hw1, d.s0, d.s1, d.plan, k, d.total_iters, d.rain);
-            } else if (active <= kArgPlanets && args_fit) {
+            } else if (active <= kArgPlanets) {
                 // entries past the active planets start at or after nt
                 hipLaunchKernelGGL(k_erode_batch_pair_args, grid, dim3(1024), 0, s, pargs,
                                    (int32_t)nt, k);
             } else {
-                hipLaunchKernelGGL(k_erode_batch_pair, grid, dim3(1024), 0, s, PDv, PB, shift, TP, active, nt,
-                                   k);
+                hipLaunchKernelGGL(k_erode_batch_pair, grid, dim3(1024), 0, s, PDv, PB, shift, TP,
+                                   active, nt, k);
             }
         }
         active = activeS0;
