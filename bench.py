@@ -147,8 +147,26 @@ def cpu_noise_baseline(nz, budget_s=25.0):
                            nz["lac"], nz["pers"], 1.0)
     gpu = nz["out"][::101]
     bound = 0.5 * 0.4 * sum(nz["pers"] ** k for k in range(nz["n_oct"])) * 1e-5
+    # per-octave noise and lattice selection of the same vertices: a debug
+    # launch of the GPU kernel against the oracle's per-octave values
+    import torch
+    from nixis_amd import _lib, terrain
+    sub = np.ascontiguousarray(pts[::101])
+    rough, strength = terrain.octave_scalars(nz["n_oct"], nz["f0"], 0.4, nz["lac"], nz["pers"], 1.0)
+    dv = torch.from_numpy(sub).to("cuda")
+    dn = torch.empty((nz["n_oct"], sub.shape[0]), dtype=torch.float32, device="cuda")
+    ds = torch.empty((nz["n_oct"], sub.shape[0], 5), dtype=torch.int32, device="cuda")
+    terrain.sample_octaves_device(dv, None, nz["perm"], nz["pgi"], rough, strength, 1.0, dn, ds)
+    _lib.Context.get(dv.device).sync_check()
+    _, raw, sel = O.sample_octaves(sub, None, nz["perm"], nz["pgi"], nz["n_oct"], nz["f0"], 0.4,
+                                   nz["lac"], nz["pers"], 1.0, raw=True, sel=True)
+    per_oct = np.abs(dn.cpu().numpy().astype(np.float64) - raw).max(axis=1)
+    mism = int((ds.cpu().numpy() != sel).any(axis=2).sum())
     parity = {"vertices_checked": int(ref.size), "max_abs_elev_err": float(np.abs(gpu - ref).max()),
-              "bound": bound}
+              "bound": bound, "per_octave_max_abs_noise_err": [float(f"{x:.3e}") for x in per_oct],
+              "noise_tolerance": 1e-5, "selection_mismatches": mism}
+    log(f"[cpu] per-octave max |noise - fp64| over {sub.shape[0]} vertices: "
+        + " ".join(f"{x:.1e}" for x in per_oct) + f"; selection mismatches {mism}")
     log(f"[cpu] oracle noise {val:.3f} Gvo/s over {len(times)} runs, {O.num_threads()} threads")
     return {"value": round(val, 4), "unit": "Gvertex-octaves/s", "cores": O.num_threads(),
             "kind": "port",
