@@ -365,15 +365,27 @@ def cpu_erosion_baseline(er, iters=4):
     from nixis_amd import util
     d = int(round(((er["_V"] - 2) / 10) ** 0.5))
     pts, nb = util.icosphere_with_neighbors(d)
+    import numpy as np
+    from nixis_amd import erosion
     h = er["_h"].cpu().numpy()
+    href = h.copy()
     t = time.perf_counter()
-    O.erode_terrain3(pts, nb, h.copy(), iters, 1.0)
+    O.erode_terrain3(pts, nb, href, iters, 1.0)
     dt = time.perf_counter() - t
     val = er["_V"] * iters / dt / 1e9
-    log(f"[cpu] oracle erosion {val:.4f} Gvi/s")
+    # the same iterations through the GPU drop-in, against the oracle's heights
+    hg = h.copy()
+    erosion.erode_terrain3(pts, nb, hg, iters, rain=1.0)
+    span = float(href.max() - href.min())
+    parity = {"vertices": int(h.size), "iterations": iters,
+              "mismatches": int((hg != href).sum()),
+              "max_abs_err_over_range": float(np.abs(hg - href).max() / span)}
+    log(f"[cpu] oracle erosion {val:.4f} Gvi/s; GPU drop-in after {iters} iterations: "
+        f"{parity['mismatches']} of {h.size} heights differ from the oracle")
     return {"value": round(val, 5), "unit": "Gvertex-iters/s", "cores": O.num_threads(),
             "kind": "port", "sample": f"{iters} of the {er['_V']}-vertex erosion iterations, "
-                                      "FP64 oracle with OpenMP"}
+                                      "FP64 oracle with OpenMP",
+            "parity_check": parity}
 
 
 def _events_ms(fn, stream=None):
