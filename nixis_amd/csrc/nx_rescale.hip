@@ -38,6 +38,22 @@ __device__ __forceinline__ void stream4(int64_t n, Load load, Body body)
     for (; i < n; i += stride) body(i, load(i));
 }
 
+// The same with eight loads in flight per lane, for the read-only min/max.
+template <typename T, typename Load, typename Body>
+__device__ __forceinline__ void stream8(int64_t n, Load load, Body body)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 7 * stride < n; i += 8 * stride) {
+        T v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = load(i + u * stride);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) body(i + u * stride, v[u]);
+    }
+    for (; i < n; i += stride) body(i, load(i));
+}
+
 // vec: the arrays are aligned for 16-byte accesses of value pairs; pairs
 // (2j, 2j+1) stream with stream4 and block 0 thread 0 takes an odd last value.
 __device__ __forceinline__ bool tail_thread() { return blockIdx.x == 0 && threadIdx.x == 0; }
@@ -61,14 +77,14 @@ __global__ __launch_bounds__(256) void k_minmax(const double *__restrict__ x, in
     };
     if (vec) {
         const double2 *x2 = (const double2 *)x;
-        stream4<double2>(n >> 1, [&](int64_t j) { return x2[j]; },
+        stream8<double2>(n >> 1, [&](int64_t j) { return x2[j]; },
                          [&](int64_t, double2 d) {
                              upd(d.x);
                              upd(d.y);
                          });
         if ((n & 1) && tail_thread()) upd(x[n - 1]);
     } else {
-        stream4<double>(n, [&](int64_t i) { return x[i]; }, [&](int64_t, double v) { upd(v); });
+        stream8<double>(n, [&](int64_t i) { return x[i]; }, [&](int64_t, double v) { upd(v); });
     }
     block_minmax(lo, hi);
     if (threadIdx.x == 0) {
@@ -113,6 +129,15 @@ __global__ __launch_bounds__(256) void k_rescale_map(const double *x, int64_t n,
 static int vec16(const void *p) { return (uintptr_t)p % 16 == 0 ? 1 : 0; }
 static int vec2(const void *p) { return (uintptr_t)p % 2 == 0 ? 1 : 0; }
 
+// The read-only min/max runs one block per CU with eight loads in flight per
+// lane: every block ends in two atomics on the same keys, and with eight
+// blocks per CU those atomics at the end of the pass cost a large part of it
+// (40 M values: 80 against 54 us alone, 89 against 79 us in the height chain).
+static unsigned mm_grid(nx_ctx *ctx, int64_t n)
+{
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, (int64_t)ctx->num_cus));
+}
+
 int rescale(nx_ctx *ctx, const double *x, int64_t n, double lower, double upper, double *out,
             double *minmax, hipStream_t stream)
 {
@@ -125,7 +150,7 @@ int rescale(nx_ctx *ctx, const double *x, int64_t n, double lower, double upper,
     unsigned long long *keys = ctx->d_keys + 2 * (ctx->key_slot.fetch_add(1) % kKeySlots);
     const unsigned grid = (unsigned)std::min<int64_t>((n + 255) / 256, (int64_t)ctx->num_cus * 8);
     hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(64), 0, stream, keys);
-    hipLaunchKernelGGL(k_minmax, dim3(grid), dim3(256), 0, stream, x, n, keys, vec16(x));
+    hipLaunchKernelGGL(k_minmax, dim3(mm_grid(ctx, n)), dim3(256), 0, stream, x, n, keys, vec16(x));
     hipLaunchKernelGGL(k_rescale_map, dim3(grid), dim3(256), 0, stream, x, n, keys, lower, upper,
                        out, minmax, ctx->d_err, vec16(x) & vec16(out));
     NX_HIP(hipGetLastError());
@@ -211,7 +236,7 @@ static int rescale_ex_launch(nx_ctx *ctx, const double *x, int64_t n, const Resc
     unsigned long long *keys = key_slot(ctx);
     const unsigned grid = red_grid(ctx, n);
     hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(64), 0, stream, keys);
-    hipLaunchKernelGGL(k_minmax, dim3(grid), dim3(256), 0, stream, x, n, keys, vec16(x));
+    hipLaunchKernelGGL(k_minmax, dim3(mm_grid(ctx, n)), dim3(256), 0, stream, x, n, keys, vec16(x));
     hipLaunchKernelGGL(k_rescale_ex, dim3(grid), dim3(256), 0, stream, x, n, keys, a, out,
                        minmax, ctx->d_err, vec16(x) & vec16(out));
     NX_HIP(hipGetLastError());
@@ -708,7 +733,8 @@ int power_rescale(nx_ctx *ctx, const double *x, const uint8_t *mask, int64_t n, 
     const PrGrid g = pr_grid(ctx, n);
     const unsigned grid = red_grid(ctx, n);
     hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(64), 0, stream, head->keys0);
-    hipLaunchKernelGGL(k_minmax, dim3(grid), dim3(256), 0, stream, x, n, head->keys0, vec16(x));
+    hipLaunchKernelGGL(k_minmax, dim3(mm_grid(ctx, n)), dim3(256), 0, stream, x, n, head->keys0,
+                       vec16(x));
     hipLaunchKernelGGL(k_pr_summary<false>, dim3((unsigned)g.blocks), dim3(kPrThreads), 0, stream,
                        const_cast<double *>(x), const_cast<uint8_t *>(mask), n, g.nunits, g.cpw,
                        pr_vec(x, mask), sum, head, ChainArgs{0.0, 0.0, 0.0, nullptr}, ctx->d_err);
@@ -783,7 +809,7 @@ int height_chain(nx_ctx *ctx, double *h, int64_t n, double min_alt, double max_a
     NX_HIP(hipSetDevice(ctx->device));
     auto *head = (PrHead *)ws;
     hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(64), 0, stream, head->keys0);
-    hipLaunchKernelGGL(k_minmax, dim3(red_grid(ctx, n)), dim3(256), 0, stream, h, n, head->keys0,
+    hipLaunchKernelGGL(k_minmax, dim3(mm_grid(ctx, n)), dim3(256), 0, stream, h, n, head->keys0,
                        vec16(h));
     return chain_after_keys(ctx, h, n, min_alt, max_alt, ocean_percent, mask, level, ws, stream);
 }
