@@ -406,7 +406,7 @@ def bench_batched(args):
     rescale/load, one graph-captured launch per erosion step) against the
     same jobs one planet per call."""
     from nixis_amd import autotune, batch, terrain
-    jobs = batch.batched_jobs()
+    jobs = batch.load_jobs(batch.BATCHED_64_PATH)
     t0 = time.perf_counter()
     meshes = batch.MeshCache()
     runs = [batch.PlanetRun(j, meshes) for j in jobs]
@@ -435,7 +435,7 @@ def bench_batched(args):
     return {
         "workload": "64 planets: d log-uniform in [64,1024] (even), seeds 0..63, octaves "
                     "U{4..16}, erosion steps U{0..200}, rescale to [-9000,9000], rain 0.3/320; "
-                    "job list from numpy default_rng(20261202)",
+                    "job list nixis_amd/jobs/batched_64planets.json (numpy default_rng(20261202))",
         "planets": len(jobs), "vertices": sum(j.V for j in jobs),
         "vertex_octaves": vo, "vertex_iters": vi,
         "batched_ms": round(ms_b, 3), "plain_ms": round(ms_p, 3),
@@ -456,7 +456,12 @@ def bench_served(args):
     form's capacity, measured by running the jobs back to back."""
     from nixis_amd import batch, serve
     n = args.served_jobs
-    proto = serve.served_jobs(n, rate=1.0)
+
+    def trace(rate):
+        if n == 120:
+            return serve.load_trace(batch.SERVED_120_PATH, rate)
+        return serve.served_jobs(n, rate=rate)
+    proto = trace(1.0)
     t0 = time.perf_counter()
     log(f"[served] preparing meshes for {len(set(j.d for j in proto))} subdivisions")
     plain = serve.PlainServer()
@@ -470,7 +475,7 @@ def bench_served(args):
     outs = plain.serve(b2b, prepare_meshes=False)
     busy = max(o.done for o in outs)
     rate = args.served_load * n / busy
-    jobs = serve.served_jobs(n, rate=rate)
+    jobs = trace(rate)
     log(f"[served] plain back-to-back busy {busy:.2f}s -> rate {rate:.1f} jobs/s")
     out_plain = plain.serve(jobs, prepare_meshes=False)
     out_served = served.serve(jobs, prepare_meshes=False)
@@ -490,8 +495,8 @@ def bench_served(args):
     return {
         "workload": f"{n} jobs, Poisson arrivals at {rate:.2f} jobs/s ({args.served_load:.0%} of "
                     "the plain form's back-to-back capacity), d log-uniform in [32,2048], "
-                    "octaves U{4..16}, 30% with U{100..300} erosion steps; trace from numpy "
-                    "default_rng(20261203)",
+                    "octaves U{4..16}, 30% with U{100..300} erosion steps; trace "
+                    "nixis_amd/jobs/served_120jobs_rate1.json (numpy default_rng(20261203))",
         "rate_jobs_per_s": round(rate, 3), "hbm_budget_GB": 64,
         "served": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in res_served.items()},
         "plain": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in res_plain.items()},

@@ -10,7 +10,9 @@ for comparison.  Meshes (vertices and neighbour tables) depend only on the
 subdivision count and are shared between jobs through ``MeshCache``.
 """
 import ctypes
-from dataclasses import dataclass, field
+import json
+import os
+from dataclasses import asdict, dataclass
 
 import numpy as np
 
@@ -327,6 +329,23 @@ def run_plain(runs, stream=None):
             r.state.load(r.elev, stream=stream)
             r.state.run(r.graph, j.erosion_iters, j.rain, stream=stream)
             r.state.store(r.elev, stream=stream)
+
+
+JOBS_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "jobs")
+BATCHED_64_PATH = os.path.join(JOBS_DIR, "batched_64planets.json")
+SERVED_120_PATH = os.path.join(JOBS_DIR, "served_120jobs_rate1.json")
+
+
+def save_jobs(jobs, path):
+    """Write a job list as JSON (one object per job, every Job field)."""
+    with open(path, "w") as f:
+        f.write("[\n" + ",\n".join(json.dumps(asdict(j)) for j in jobs) + "\n]\n")
+
+
+def load_jobs(path):
+    """Read a job list written by save_jobs."""
+    with open(path) as f:
+        return [Job(**e) for e in json.load(f)]
 
 
 def batched_jobs(n=64, seed=20261202, d_range=(64, 1024), oct_range=(4, 16),

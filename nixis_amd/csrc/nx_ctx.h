@@ -28,7 +28,7 @@ struct GraphKey {
     int n;
     double rain;
     bool last;
-    uint64_t tag;  // content hash for batched schedules, 0 otherwise
+    uint64_t tag;  // hash of what the captured launches take by value (plan, schedule)
     bool operator<(const GraphKey &o) const
     {
         return std::tie(state, V, col, elen, n, rain, last, tag) <

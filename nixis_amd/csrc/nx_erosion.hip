@@ -321,19 +321,44 @@ static void launch_steps(const ErosionBufs &b, int64_t V, const int32_t *col, co
     hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, stream, b.meta, (int64_t)n, j);
 }
 
-// Captured chunk of steps, cached per (workspace, graph, plan, length, rain,
-// last).  Replays stay correct because every launch reads the step counter,
-// buffer and exponent state from the workspace rather than from its
+// Everything a captured launch takes from the plan by value (TilePlanDev:
+// the blob address, the section offsets, the tile count) plus the kernel
+// variant and rule, folded into one 64-bit tag (FNV-1a).  Two plans that
+// differ in any of these get different graphs even when a later plan's
+// image lands at a freed plan's address.
+static uint64_t plan_tag(const nx_tile_plan *plan, int variant, int rule)
+{
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&h](uint64_t v) {
+        for (int k = 0; k < 8; ++k) {
+            h ^= (v >> (8 * k)) & 0xff;
+            h *= 1099511628211ull;
+        }
+    };
+    mix((uint64_t)rule);
+    mix((uint64_t)variant);
+    if (plan) {
+        mix((uint64_t)(uintptr_t)plan->blob);
+        const int64_t f[] = {plan->bytes,   plan->V,        plan->ntiles,    plan->max_tile,
+                             plan->max_load, plan->off_start, plan->off_cs,   plan->off_load,
+                             plan->off_rbase, plan->off_ring, plan->off_cbase, plan->off_lcol,
+                             plan->off_lel,  plan->steps,    plan->max_step1, plan->off_pref};
+        for (int64_t v : f) mix((uint64_t)v);
+    }
+    return h;
+}
+
+// Captured chunk of steps, cached per (workspace, graph, plan contents,
+// length, rain, last).  Replays stay correct because every launch reads the
+// step counter and current buffer from the workspace rather than from its
 // arguments.  No memset is recorded into the graph.
 static int chunk_graph(nx_ctx *ctx, const ErosionBufs &b, void *state, int64_t V,
                        const int32_t *col, const double *elen, const nx_tile_plan *plan, int rule,
                        int n, double rain, bool last, hipGraphExec_t *out)
 {
-    const uint64_t ptag =
-        plan ? (uint64_t)(uintptr_t)plan->blob * 16 +
-                   (uint64_t)(plan->steps == kMultiSteps ? 8 + ctx->multi_variant : ctx->pair_variant)
-             : 0;
-    GraphKey key{state, V, col, elen, n, rain, last, ptag * 8 + (uint64_t)rule};
+    const int variant =
+        plan ? (plan->steps == kMultiSteps ? 8 + ctx->multi_variant : ctx->pair_variant) : -1;
+    GraphKey key{state, V, col, elen, n, rain, last, plan_tag(plan, variant, rule)};
     std::lock_guard<std::mutex> lk(ctx->mu);
     auto it = ctx->graphs.find(key);
     if (it != ctx->graphs.end()) {

@@ -131,7 +131,7 @@ class ErosionGraph:
 
     @property
     def nbytes(self):
-        n = self.col.numel() * 4 + self.elen.numel() * 4
+        n = self.col.numel() * self.col.element_size() + self.elen.numel() * self.elen.element_size()
         return n + (self._plan_blob.numel() if self._plan_blob is not None else 0)
 
     def _order_dev(self):
@@ -222,6 +222,37 @@ class ErosionState:
         return out
 
 
+def _digest(a):
+    """Content digest of a contiguous array (128-bit xxh3)."""
+    import xxhash
+    a = np.ascontiguousarray(a)
+    return (a.shape, a.dtype.str, xxhash.xxh3_128_hexdigest(memoryview(a).cast("B")))
+
+
+_GRAPHS = {}        # (digest of verts, digest of neighbours, options) -> ErosionGraph
+_GRAPHS_MAX = 2
+
+
+def graph_for(verts, neighbors, order="sfc", tiled=True):
+    """The ErosionGraph of (verts, neighbors), built once and reused while the
+    arrays hold the same contents.  The reference's loops call the step
+    once per iteration with the same mesh (erosion.py:180-184), so the
+    drop-ins keep the device tables, vertex order and tile plan of the last
+    two meshes instead of rebuilding them per call; the key is the arrays'
+    contents, so a mesh edited in place gets a new graph."""
+    import torch
+    dev = torch.cuda.current_device()
+    key = (_digest(np.asarray(verts, dtype=np.float64)),
+           _digest(np.asarray(neighbors, dtype=np.int32)), order, bool(tiled), dev)
+    g = _GRAPHS.pop(key, None)
+    if g is None:
+        g = ErosionGraph(verts, neighbors, order=order, tiled=tiled)
+        while len(_GRAPHS) >= _GRAPHS_MAX:
+            _GRAPHS.pop(next(iter(_GRAPHS)))
+    _GRAPHS[key] = g          # most recently used last
+    return g
+
+
 def _dev(a, graph):
     import torch
     return torch.from_numpy(np.ascontiguousarray(graph.permute(a), dtype=np.float64)).to(
@@ -239,7 +270,7 @@ def erode_terrain3(nodes, neighbors, heights, num_iter=1, snapshot=False, rain=R
         raise TypeError("heights must be float64")
     if num_iter <= 0:
         num_iter = 1
-    graph = ErosionGraph(nodes, neighbors)
+    graph = graph_for(nodes, neighbors)
     st = ErosionState(graph.V, graph.device)
     st.load(_dev(heights, graph))
     st.run(graph, num_iter, rain)
@@ -256,7 +287,7 @@ def erosion_iteration3(verts, neighbors, r_buff, wat, sed):
     for a in (r_buff, wat, sed):
         if a.dtype != np.float64:
             raise TypeError("fields must be float64")
-    graph = ErosionGraph(verts, neighbors)
+    graph = graph_for(verts, neighbors)
     st = ErosionState(graph.V, graph.device)
     st.load(_dev(r_buff, graph), _dev(wat, graph), _dev(sed, graph))
     st.run(graph, 1, 0.0)
@@ -274,7 +305,7 @@ def erosion_iteration4(verts, neighbors, r_buff, wat, sed):
     for a in (r_buff, wat, sed):
         if a.dtype != np.float64:
             raise TypeError("fields must be float64")
-    graph = ErosionGraph(verts, neighbors)
+    graph = graph_for(verts, neighbors)
     st = ErosionState(graph.V, graph.device)
     st.load(_dev(r_buff, graph), _dev(wat, graph), _dev(sed, graph))
     st.run(graph, 1, 0.0, rule=4)
@@ -307,7 +338,7 @@ def erosion_iteration1(neighbors, r_buff, w_buff):
     if w_buff.dtype != np.float64:
         raise TypeError("w_buff must be float64")
     nb = np.ascontiguousarray(neighbors, dtype=np.int32)
-    graph = ErosionGraph(np.zeros((nb.shape[0], 3)), nb, order=None, tiled=False)
+    graph = graph_for(np.zeros((nb.shape[0], 3)), nb, order=None, tiled=False)
     h = _dev(r_buff, graph)
     _v1_run(graph, h, 1)
     w_buff[...] = graph.unpermute(h.cpu().numpy())
@@ -322,7 +353,7 @@ def erode_terrain1(nodes, neighbors, heights, num_iter=1, snapshot=None):
         raise TypeError("heights must be float64")
     if num_iter <= 0:
         num_iter = 1
-    graph = ErosionGraph(nodes, neighbors, tiled=False)
+    graph = graph_for(nodes, neighbors, tiled=False)
     h = _dev(heights, graph)
     _v1_run(graph, h, num_iter)
     heights[...] = graph.unpermute(h.cpu().numpy())

@@ -162,7 +162,9 @@ class PlanetServer:
             n = remaining if self.steps_per_tick == 0 else min(remaining, self.steps_per_tick)
             n = min(n, max(2, left // r.V))
             if n < remaining and n % 2:
-                n -= 1   # whole pairs; an odd (single-step) launch only at the end
+                # whole pairs; an odd (single-step) launch only at the end,
+                # and never fewer than one pair, so every tick advances
+                n = max(n - 1, min(2, remaining))
             tick.append(r)
             caps.append(n)
             left -= n * r.V
@@ -298,3 +300,10 @@ def served_jobs(n, rate, seed=20261203, d_range=(32, 2048), oct_range=(4, 16),
         jobs.append(_batch.Job(d=d, seed=seed_i, n_octaves=o, erosion_iters=it, arrival=t,
                                name=f"s{i:03d}"))
     return jobs
+
+
+def load_trace(path, rate):
+    """A served trace saved at one job per second (batch.save_jobs of
+    served_jobs(n, rate=1.0)) with its arrival times rescaled to `rate`."""
+    from dataclasses import replace
+    return [replace(j, arrival=j.arrival / rate) for j in _batch.load_jobs(path)]

@@ -278,3 +278,114 @@ def test_four_step_launches_equal_two_step_and_single_at_scale():
         assert np.array_equal(a, b)
     for a, b in zip(outs[0], outs[2]):
         assert np.array_equal(a, b)
+
+
+def test_graph_cache_equal_v_meshes_back_to_back(oracle):
+    """Two meshes with the same vertex count but different tile plans (the
+    same icosphere with jittered coordinates, so other edge lengths and
+    another space-filling-curve order), eroded one after the other with
+    captured graphs, the first mesh's graph and state freed before the next
+    is built: the second run must not replay a graph captured for the first
+    plan when its buffers land at the same addresses.  Every run equals the
+    oracle bit for bit."""
+    pts, nb, h = _planet(24)
+    rng = np.random.default_rng(5)
+    jit = pts + rng.normal(scale=0.02, size=pts.shape)
+    jit /= np.linalg.norm(jit, axis=1, keepdims=True)
+    seen = []
+    for k, p in enumerate((pts, jit, pts, jit)):
+        g = erosion.ErosionGraph(p, nb)
+        assert g.plan is not None
+        st = erosion.ErosionState(g.V)
+        st.load(torch.from_numpy(g.permute(h)).cuda())
+        st.run(g, 70, 1.0)                 # 32 + 32 + 6 steps, each a cached graph
+        out = torch.empty(g.V, dtype=torch.float64, device="cuda")
+        st.store(out)
+        st.ctx.sync_check()
+        got = g.unpermute(out.cpu().numpy())
+        seen.append((st.buf.data_ptr(), g.plan.blob, g.plan.ntiles))
+        print(f"\nrun {k}: state {seen[-1][0]:#x} plan {seen[-1][1]:#x} tiles {seen[-1][2]}",
+              flush=True)
+        del g, st, out
+        torch.cuda.synchronize()
+        href = h.copy()
+        oracle.erode_terrain3(p, nb, href, 70, 1.0)
+        assert (got == href).all(), f"run {k} differs from the oracle"
+
+
+def test_reference_main_flow_at_world_radius(oracle):
+    """The reference's own run (nixis.py:247-249, 308-364, 410): points
+    scaled to the world radius 6378100 m, 7 octaves with roughness 1.5 x 2.5
+    per octave and strength 0.4 x 0.5, the height chain to [-4000, 8850] m
+    with the ocean at 55%, then erode_terrain3(points, neighbors, height,
+    num_iter=11).  Noise within its per-octave tolerance of the oracle, the
+    chain's mask and ocean level exact, erosion bit for bit from the same
+    heights, through the drop-in erode_terrain3 on the scaled points."""
+    R = 6378100.0
+    d = 320
+    pts, nb = util.icosphere_with_neighbors(d)
+    pts = pts * R
+    perm, pgi = osx.init(1234)
+    n_oct, r0, s0, lac, pers = 7, 1.5, 0.4, 2.5, 0.5
+    h = terrain.sample_octaves(pts, None, perm, pgi, n_oct, r0, s0, lac, pers, R)
+    raw = oracle.sample_octaves(pts, None, perm, pgi, n_oct, r0, s0, lac, pers, R)
+    # octave o adds ((n + 1) / 2 * (a_o / R)) * R: the radius cancels
+    bound = 0.5 * sum(s0 * pers ** k for k in range(n_oct)) * 1e-5
+    err = np.abs(h - raw).max()
+    print(f"\nnoise at R={R}: max |elev - fp64| {err:.3e} m (bound {bound:.3e})")
+    assert err <= bound
+    hd = torch.from_numpy(h.copy()).cuda()
+    hd, mask, level = util.height_chain_device(hd, -4000.0, 8850.0, 55.0)
+    _lib.Context.get(hd.device).sync_check()
+    ref_h, ref_mask, ref_level = oracle.height_chain(h, -4000.0, 8850.0, 55.0)
+    assert level.item() == ref_level
+    assert (mask.cpu().numpy().astype(bool) == ref_mask).all()
+    hc = hd.cpu().numpy()
+    assert np.abs(hc - ref_h).max() <= 64 * 2.0 ** -52 * 12850.0
+    hg = hc.copy()
+    erosion.erode_terrain3(pts, nb, hg, num_iter=11)
+    href = hc.copy()
+    oracle.erode_terrain3(pts, nb, href, 11)
+    print(f"erosion at R={R}, 11 passes: max |dh| {np.abs(hg - hc).max():.3e} m")
+    assert (hg == href).all(), _rel(hg, href)
+
+
+def _same_bits_or_both_nan(a, b):
+    """Equal bit patterns, except that NaNs only need to be NaN in both:
+    the sign and payload of a NaN made by an invalid operation (inf - inf)
+    are the hardware's choice (x86 makes 0xfff8..., the GPU 0x7ff8...)."""
+    na, nb_ = np.isnan(a), np.isnan(b)
+    return (na == nb_).all() and (a[~na].view(np.int64) == b[~nb_].view(np.int64)).all()
+
+
+def test_erosion_into_overflow_matches_oracle(oracle):
+    """The unit-radius planet with rain 1.0 grows about 2x per step
+    (erosion.py:214-215,262-267 start each sum at the vertex's own value)
+    until FP64 overflows near step 1002; inf - inf then makes NaN that
+    spreads over the mesh.  Heights, water and sediment equal the oracle's
+    at steps 1000..1005 and 1040 (infinities with their signs, finite values
+    bit for bit, NaNs where the oracle has NaNs)."""
+    pts, nb, h = _planet(16)
+    g = erosion.ErosionGraph(pts, nb)
+    st = erosion.ErosionState(g.V)
+    st.load(torch.from_numpy(g.permute(h)).cuda())
+    hr = h.copy()
+    wr, sr = oracle.erode_terrain3(pts, nb, hr, 1000, 1.0, return_fields=True)
+    done = 1000
+    st.run(g, 1000, 1.0)
+    for target in (1000, 1001, 1002, 1003, 1004, 1005, 1040):
+        if target > done:
+            st.run(g, target - done, 1.0)
+            for _ in range(target - done):
+                wr += 1.0
+                oracle.erosion_iteration3(pts, nb, hr, wr, sr)
+            done = target
+        out = [torch.empty(g.V, dtype=torch.float64, device="cuda") for _ in range(3)]
+        st.store(*out)
+        st.ctx.sync_check()
+        got = [g.unpermute(t.cpu().numpy()) for t in out]
+        print(f"step {target}: h finite {np.isfinite(hr).sum()} inf {np.isinf(hr).sum()} "
+              f"nan {np.isnan(hr).sum()}", flush=True)
+        for name, x, r in zip("hws", got, (hr, wr, sr)):
+            assert _same_bits_or_both_nan(x, r), f"{name} differs at step {target}"
+    assert np.isnan(hr).any() and np.isinf(hr).any()

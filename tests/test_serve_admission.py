@@ -69,10 +69,13 @@ class _Run:
         self.job = Job(d=2, seed=0, n_octaves=1, erosion_iters=iters)
 
 
-@pytest.mark.parametrize("budget, per_tick", [(9_000, 0), (4_000, 0), (50_000, 5)])
+@pytest.mark.parametrize("budget, per_tick", [(9_000, 0), (4_000, 0), (50_000, 5), (50_000, 1),
+                                              (50_000, 3)])
 def test_erosion_ticks_hand_out_whole_pairs(budget, per_tick):
     """Every tick gives a job an even number of steps (two-step launches)
-    except the tick that finishes it, and the steps add up to the job's."""
+    except the tick that finishes it, and the steps add up to the job's.
+    A per-job cap of one or three steps still advances every job by a pair
+    per tick (rounding down to whole pairs never reaches zero steps)."""
     from nixis_amd.serve import PlanetServer
     srv = PlanetServer.__new__(PlanetServer)
     srv.tick_vertex_steps, srv.steps_per_tick = budget, per_tick
