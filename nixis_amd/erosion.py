@@ -259,13 +259,45 @@ def _dev(a, graph):
         graph.device)
 
 
+def _snapshot_image(h_dev, key):
+    """erosion.py:186-192 on the device: the heights rescaled to 0..255,
+    build_image_data's gray image of them (util.py:369-438: a float64 layer
+    is rescaled to 0..255 once more, sampled per pixel by make_gray_array and
+    cast to uint8) and save_image(..., cfg.SNAP_DIR, "erosion_snapshot").
+    The reference puts the bare array into the dict where build_image_data
+    expects [array, mode]; the layer is written as the 'gray' image that
+    call is meant to produce."""
+    import os
+    import torch
+    from PIL import Image
+    from . import cfg, util
+    if cfg.SNAP_DIR is None or cfg.IMG_QUERY_DATA is None:
+        raise ValueError("erosion snapshots need cfg.SNAP_DIR and cfg.IMG_QUERY_DATA")
+    dists, nbrs = cfg.IMG_QUERY_DATA
+    if not isinstance(dists, torch.Tensor):
+        dists = torch.from_numpy(np.ascontiguousarray(dists, dtype=np.float64)).to(h_dev.device)
+        nbrs = torch.from_numpy(np.ascontiguousarray(nbrs, dtype=np.int64)).to(h_dev.device)
+        cfg.IMG_QUERY_DATA = (dists, nbrs)   # keep the device copies for later passes
+    if tuple(dists.shape[:2]) != (int(cfg.IMG_HEIGHT), int(cfg.IMG_WIDTH)):
+        raise ValueError("cfg.IMG_QUERY_DATA does not match cfg.IMG_WIDTH x cfg.IMG_HEIGHT")
+    layer, _ = util.rescale_device(h_dev, 0, 255)            # erosion.py:188
+    layer, _ = util.rescale_device(layer, 0, 255, out=layer)  # util.py:397-398
+    pixels = util.make_gray_array_device(dists, nbrs, layer).to(torch.uint8)
+    _lib.Context.get(h_dev.device).sync_check()
+    Image.fromarray(pixels.cpu().numpy()).save(
+        os.path.join(cfg.SNAP_DIR, f"erosion_snapshot_{key}.{cfg.IMG_FORMAT}"))
+
+
 def erode_terrain3(nodes, neighbors, heights, num_iter=1, snapshot=False, rain=RAIN_DEFAULT):
     """erosion.py:171-192: erode ``heights`` (float64[V]) in place.
 
     Water and sediment start at zero and are discarded, as in the reference.
-    ``snapshot`` image export is not part of this path and is ignored.
+    With ``snapshot``, each pass also writes the gray image of the heights
+    rescaled to 0..255 to cfg.SNAP_DIR as erosion_snapshot_NNN (NNN the pass
+    number from 001), sampled through cfg.IMG_QUERY_DATA.
     ``rain`` is the per-iteration rainfall (the reference hard-codes 0.3/320).
     """
+    import torch
     if heights.dtype != np.float64:
         raise TypeError("heights must be float64")
     if num_iter <= 0:
@@ -273,10 +305,15 @@ def erode_terrain3(nodes, neighbors, heights, num_iter=1, snapshot=False, rain=R
     graph = graph_for(nodes, neighbors)
     st = ErosionState(graph.V, graph.device)
     st.load(_dev(heights, graph))
-    st.run(graph, num_iter, rain)
-    import torch
     h = torch.empty(graph.V, dtype=torch.float64, device=graph.device)
-    st.store(h)
+    if snapshot:
+        for i in range(num_iter):
+            st.run(graph, 1, rain)
+            st.store(h)
+            _snapshot_image(graph.unpermute_device(h), f"{i + 1:03d}")
+    else:
+        st.run(graph, num_iter, rain)
+        st.store(h)
     st.ctx.sync_check()
     heights[...] = graph.unpermute(h.cpu().numpy())
 

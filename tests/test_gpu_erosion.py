@@ -389,3 +389,38 @@ def test_erosion_into_overflow_matches_oracle(oracle):
         for name, x, r in zip("hws", got, (hr, wr, sr)):
             assert _same_bits_or_both_nan(x, r), f"{name} differs at step {target}"
     assert np.isnan(hr).any() and np.isinf(hr).any()
+
+
+def test_erode_terrain3_snapshots(oracle, tmp_path):
+    """erosion.py:186-192: with snapshot, each pass writes the gray image of
+    the heights rescaled to 0..255 (then once more by build_image_data,
+    util.py:397-398), sampled at each pixel's three nearest vertices and cast
+    to uint8, as erosion_snapshot_NNN.png in cfg.SNAP_DIR.  Every image
+    equals the oracle's pipeline on the oracle's heights after that pass,
+    and the heights equal a run without snapshots."""
+    from PIL import Image
+    from nixis_amd import cfg
+    pts, nb, h = _planet(24)
+    W, H = 96, 48
+    dists, nbrs = util.KnnGrid(pts).query_pixels_device(W, H, 1.0, 3)
+    dn, nn = dists.cpu().numpy(), nbrs.cpu().numpy()
+    saved = (cfg.SNAP_DIR, cfg.IMG_QUERY_DATA, cfg.IMG_WIDTH, cfg.IMG_HEIGHT)
+    try:
+        cfg.SNAP_DIR, cfg.IMG_QUERY_DATA, cfg.IMG_WIDTH, cfg.IMG_HEIGHT = str(tmp_path), (dn, nn), W, H
+        hs = h.copy()
+        erosion.erode_terrain3(pts, nb, hs, 3, snapshot=True, rain=1.0)
+    finally:
+        cfg.SNAP_DIR, cfg.IMG_QUERY_DATA, cfg.IMG_WIDTH, cfg.IMG_HEIGHT = saved
+    hp = h.copy()
+    erosion.erode_terrain3(pts, nb, hp, 3, rain=1.0)
+    assert (hs == hp).all()
+    href, w, s = h.copy(), np.zeros_like(h), np.zeros_like(h)
+    for i in range(3):
+        w += 1.0
+        oracle.erosion_iteration3(pts, nb, href, w, s)
+        layer = oracle.rescale(oracle.rescale(href, 0, 255), 0, 255)
+        want = oracle.make_gray_array(W, H, dn, nn, layer).astype(np.uint8)
+        got = np.asarray(Image.open(tmp_path / f"erosion_snapshot_{i + 1:03d}.png"))
+        assert got.shape == (H, W) and got.dtype == np.uint8
+        assert (got == want).all(), f"pass {i + 1}"
+    assert (href == hs).all()
