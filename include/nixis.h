@@ -175,6 +175,24 @@ int nx_terrain_heights(nx_ctx *ctx, const double *verts, int64_t V,
                        double ocean_percent, double *h, uint8_t *mask,
                        double *ocean_level, void *ws, void *stream);
 
+/* nx_terrain_heights without a workspace argument.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid allocation).
+ * It calls no hipMalloc, hipFree or allocating library routine; the chain's
+ * scratch (its 256-byte head and at most 2048 pairs of block summaries,
+ * 196,864 bytes, enough for any V) is a static device array, one per GPU,
+ * held until the process ends, whose address the context takes when it is
+ * created.  Calls that overlap on one GPU share that array and race (also
+ * from different contexts); calls in stream order are independent.
+ * Arguments and results as nx_terrain_heights. */
+int nx_terrain_heights_noalloc(nx_ctx *ctx, const double *verts, int64_t V,
+                               const int32_t *perm, const int32_t *pgi,
+                               int n_octaves, const double *rough,
+                               const double *strength, double radius,
+                               double min_alt, double max_alt,
+                               double ocean_percent, double *h, uint8_t *mask,
+                               double *ocean_level, void *stream);
+
 /* ---- equirectangular export (device) ----------------------------------- */
 
 /* util.py:275-287 (build_KDTree) and nixis.py:282 (KDTree.query(ll, k=3)):

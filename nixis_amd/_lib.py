@@ -97,6 +97,10 @@ SIGNATURES = {
                                     _c_void_p, _c_void_p, _c_double, _c_double, _c_double,
                                     _c_double, _c_void_p, _c_void_p, _c_void_p, _c_void_p,
                                     _c_void_p]),
+    "nx_terrain_heights_noalloc": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p, _c_void_p,
+                                            _c_int, _c_void_p, _c_void_p, _c_double, _c_double,
+                                            _c_double, _c_double, _c_void_p, _c_void_p, _c_void_p,
+                                            _c_void_p]),
     "nx_height_chain": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_double, _c_double, _c_double,
                                  _c_void_p, _c_void_p, _c_void_p, _c_void_p]),
     "nx_climate_workspace_bytes": (_c_size, []),

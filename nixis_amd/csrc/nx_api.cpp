@@ -109,11 +109,14 @@ int nx_ctx_create(int device, nx_ctx **out)
     if (e == hipSuccess) e = hipMalloc((void **)&c->d_keys, sizeof(unsigned long long) * 2 * kKeySlots);
     if (e == hipSuccess) e = hipMemset(c->d_err, 0, 256);
     if (e == hipSuccess) e = hipDeviceSynchronize();
-    if (e != hipSuccess) {
+    // this device's address of the static chain scratch (device current)
+    const int src = e == hipSuccess ? chain_scratch_init(c) : NX_OK;
+    if (e != hipSuccess || src != NX_OK) {
         if (c->d_err) (void)hipFree(c->d_err);
         if (c->d_keys) (void)hipFree(c->d_keys);
         if (c->capture_stream) (void)hipStreamDestroy(c->capture_stream);
         delete c;
+        if (src != NX_OK) return src;
         return set_error(NX_EHIP, "context device allocation failed: %s", hipGetErrorString(e));
     }
     *out = c;
@@ -319,6 +322,17 @@ int nx_terrain_heights(nx_ctx *ctx, const double *verts, int64_t V, const int32_
 {
     return terrain_heights(ctx, verts, V, perm, pgi, n_octaves, rough, strength, radius, min_alt,
                            max_alt, ocean_percent, h, mask, ocean_level, ws, (hipStream_t)stream);
+}
+
+int nx_terrain_heights_noalloc(nx_ctx *ctx, const double *verts, int64_t V, const int32_t *perm,
+                               const int32_t *pgi, int n_octaves, const double *rough,
+                               const double *strength, double radius, double min_alt,
+                               double max_alt, double ocean_percent, double *h, uint8_t *mask,
+                               double *ocean_level, void *stream)
+{
+    return terrain_heights_noalloc(ctx, verts, V, perm, pgi, n_octaves, rough, strength, radius,
+                                   min_alt, max_alt, ocean_percent, h, mask, ocean_level,
+                                   (hipStream_t)stream);
 }
 
 int nx_height_chain(nx_ctx *ctx, double *h, int64_t n, double min_alt, double max_alt,

@@ -56,6 +56,7 @@ struct nx_ctx {
     int graph_chunk = 32;                  // erosion steps per captured graph; 0 = eager
     int pair_variant = 0;                  // two-step kernel variant (0 = default)
     int multi_variant = 0;                 // four-step kernel variant (0 = default)
+    void *chain_scratch = nullptr;         // this device's address of the static chain scratch
     std::map<GraphKey, hipGraphExec_t> graphs;
     std::mutex mu;
     StageSlot stage[kStageSlots];
@@ -104,6 +105,12 @@ int terrain_heights(nx_ctx *ctx, const double *verts, int64_t V, const int32_t *
                     double radius, double min_alt, double max_alt, double ocean_percent,
                     double *h, uint8_t *mask, double *level, void *ws, hipStream_t stream);
 size_t height_chain_workspace_bytes(int64_t n);
+int chain_scratch_init(nx_ctx *ctx);
+int terrain_heights_noalloc(nx_ctx *ctx, const double *verts, int64_t V, const int32_t *perm,
+                            const int32_t *pgi, int n_oct, const double *rough,
+                            const double *strength, double radius, double min_alt,
+                            double max_alt, double ocean_percent, double *h, uint8_t *mask,
+                            double *level, hipStream_t stream);
 size_t climate_workspace_bytes();
 int sample_insolation(nx_ctx *ctx, const double *verts, int64_t V, double radius, double rot0,
                       double rot_step, int nsteps, double tilt, float *arr, void *ws,

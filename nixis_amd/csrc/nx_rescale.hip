@@ -832,4 +832,45 @@ int terrain_heights(nx_ctx *ctx, const double *verts, int64_t V, const int32_t *
     return chain_after_keys(ctx, h, V, min_alt, max_alt, ocean_percent, mask, level, ws, stream);
 }
 
+// ---- terrain_heights without a caller workspace ---------------------------
+//
+// The chain's workspace (the 256-byte head and at most kPrMaxBlocks pairs of
+// block summaries) as a file-scope device array: one copy per GPU, held
+// until the process ends.  Its device address is taken once per context,
+// with the context's device current (chain_scratch_init, called when the
+// context is created), and passed to the kernels exactly as terrain_heights
+// passes a caller's workspace.  Every reduction key in the head is reset by a
+// fill kernel at the start of its pass and every summary is written before
+// it is read, so nothing carries over from the previous call; calls that
+// overlap on one GPU share the array and race.
+constexpr size_t kChainScratchBytes = 256 + ((sizeof(PSum) * 2 * (size_t)kPrMaxBlocks + 255) & ~(size_t)255);
+static_assert(kChainScratchBytes == 196864, "include/nixis.h states the scratch size");
+static __device__ __align__(256) unsigned char scratch[kChainScratchBytes];
+
+int chain_scratch_init(nx_ctx *ctx)
+{
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return set_error(NX_EHIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+    ctx->chain_scratch = p;
+    return NX_OK;
+}
+
+int terrain_heights_noalloc(nx_ctx *ctx, const double *verts, int64_t V, const int32_t *perm,
+                            const int32_t *pgi, int n_oct, const double *rough,
+                            const double *strength, double radius, double min_alt,
+                            double max_alt, double ocean_percent, double *h, uint8_t *mask,
+                            double *level, hipStream_t stream)
+{
+    if (!ctx) return set_error(NX_EINVAL, "null context");
+    if (!ctx->chain_scratch) return set_error(NX_EINVAL, "context has no static scratch address");
+    const size_t need = height_chain_workspace_bytes(V);
+    if (need > kChainScratchBytes)
+        return set_error(NX_ERANGE, "V=%lld needs %zu bytes of chain scratch; the static scratch holds %zu",
+                         (long long)V, need, kChainScratchBytes);
+    return terrain_heights(ctx, verts, V, perm, pgi, n_oct, rough, strength, radius, min_alt,
+                           max_alt, ocean_percent, h, mask, level, ctx->chain_scratch, stream);
+}
+
 }  // namespace nx
