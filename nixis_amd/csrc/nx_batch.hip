@@ -152,11 +152,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
     __shared__ NoiseTables T;
     __shared__ int64_t s_item;
     __shared__ int s_p;
-    {
-        const float4 *src = reinterpret_cast<const float4 *>(kExtLutDev.e);
-        float4 *dst = reinterpret_cast<float4 *>(T.ext);
-        for (int i = threadIdx.x; i < kExtIndex * 2; i += blockDim.x) dst[i] = src[i];
-    }
+    load_ext_table(T);
     int cur = -1;
     bool bad = false;
     for (;;) {

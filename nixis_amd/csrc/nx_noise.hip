@@ -7,8 +7,9 @@
 // are read once, each octave's coordinate scale, skew and lattice selection
 // stay in FP64 in the reference's order (or, when the frequencies double,
 // the skewed coordinates are doubled from the previous octave, which gives
-// the same bits), the weighted octaves accumulate in FP64, and only the
-// final elevation is written.  Tables live in LDS (see nx_noise_dev.h).
+// the same bits), the octaves' weighted FP32 sums accumulate in FP32 and
+// join the FP64 sum of the octave offsets, and only the final elevation is
+// written.  Tables live in LDS (see nx_noise_dev.h).
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -48,7 +49,7 @@ __device__ __forceinline__ void sample_octaves_body(const OctaveArgs &a,
         const double acc = octave_sum<DBG>(x, y, z, acc0, a.oct, T, bad,
                                            [&](int o, float v, const NoiseSel &sel) {
             if (DBG) {
-                if (dbg_noise) dbg_noise[(int64_t)o * V + i] = v * (1.0f / 103.0f);
+                if (dbg_noise) dbg_noise[(int64_t)o * V + i] = v * (kSumScale / 103.0f);
                 if (dbg_sel) {
                     int32_t *d = dbg_sel + ((int64_t)o * V + i) * 5;
                     for (int k = 0; k < 5; ++k) d[k] = sel.v[k];
@@ -152,7 +153,7 @@ int octave_plan(const double *rough, const double *strength, double radius, int 
             return set_error(NX_EINVAL, "octave %d frequency is not finite", o);
         const double w = (0.5 * strength[o]) * radius;
         p.rough[o] = rough[o];
-        p.wsum[o] = w / 103.0;
+        p.wsumf[o] = (float)(w * kSumScaleD / 103.0);
         p.base += w;
         if (std::fabs(rough[o]) > best) {
             best = std::fabs(rough[o]);

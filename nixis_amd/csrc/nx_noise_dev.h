@@ -8,17 +8,18 @@
 //   * the lattice hash (opensimplex.py:125) reads byte tables in LDS;
 //   * the per-point falloff and gradient dot product run in FP32.
 //
-// Layout of the evaluation (branch-free across the three regions):
-//   - the six unit-cube corners with one or two bits set (100, 010, 001,
-//     110, 101, 011) are evaluated for every point and masked by region
-//     (region 0 = tetrahedron at 000 keeps the 1-bit corners, region 1 =
-//     tetrahedron at 111 keeps the 2-bit corners, region 2 = octahedron
-//     keeps all six);
-//   - one "origin" slot holds 000 (region 0) or 111 (region 1);
+// Layout of the evaluation (branch-free across the three regions, eight
+// contribution slots, which is what the octahedron needs):
+//   - six slots for unit-cube corners: 010, 001 (one bit set; out in region
+//     1, the tetrahedron at 111), 101, 011 (two bits set; out in region 0,
+//     the tetrahedron at 000), slot A = 100 or, in region 1, 111, and slot
+//     B = 110 or, in region 0, 000;
 //   - two slots hold the extra vertices chosen by the region logic, whose
 //     lattice offsets and delta constants come from a 64-entry table indexed
 //     by the region's comparison outcomes.
 // The hash prefixes perm[x], perm[perm[x]+y] are shared between corners.
+// The FP32 deltas are scaled by 1/sqrt(2), so attn / 2 is clamped to [0, 1]
+// by the clamp modifier of the FMA that computes it (see contrib).
 #pragma once
 
 #include <stdint.h>
@@ -49,7 +50,7 @@ namespace nx {
 //   case 4: octahedron, both closest on the (1,1,1) side (mask = a & b)
 //   case 5: octahedron, both on the (0,0,0) side (mask = a | b)
 //   case 6: octahedron, one on each side (mask = far * 8 + near)
-struct ExtEntry {
+struct alignas(16) ExtEntry {
     int32_t code;
     float c0x, c0y, c0z;
     int32_t pad;
@@ -128,15 +129,16 @@ constexpr ExtEntry ext_entry(int k, int m)
     ExtEntry r{};
     r.code = (e0[0] + 1) | (e0[1] + 1) << 2 | (e0[2] + 1) << 4 | (e1[0] + 1) << 6 |
              (e1[1] + 1) << 8 | (e1[2] + 1) << 10;
-    const float S = 1.0f / 3.0f;
-    const float t0 = S * (float)(e0[0] + e0[1] + e0[2]);
-    const float t1 = S * (float)(e1[0] + e1[1] + e1[2]);
-    r.c0x = (float)e0[0] + t0;
-    r.c0y = (float)e0[1] + t0;
-    r.c0z = (float)e0[2] + t0;
-    r.c1x = (float)e1[0] + t1;
-    r.c1y = (float)e1[1] + t1;
-    r.c1z = (float)e1[2] + t1;
+    // o + (ox + oy + oz) / 3, scaled by 1 / sqrt(2) like the deltas
+    const double R = 0.70710678118654752;
+    const double t0 = (e0[0] + e0[1] + e0[2]) / 3.0;
+    const double t1 = (e1[0] + e1[1] + e1[2]) / 3.0;
+    r.c0x = (float)((e0[0] + t0) * R);
+    r.c0y = (float)((e0[1] + t0) * R);
+    r.c0z = (float)((e0[2] + t0) * R);
+    r.c1x = (float)((e1[0] + t1) * R);
+    r.c1y = (float)((e1[1] + t1) * R);
+    r.c1z = (float)((e1[2] + t1) * R);
     return r;
 }
 
@@ -180,7 +182,7 @@ constexpr ExtLut make_ext_lut()
 // lands in range without masking.  The gradient table holds 8 * (pgi / 3),
 // the byte offset of the gradient's word, so a hash is an address.
 constexpr int kWrap = 520;
-struct NoiseTables {
+struct alignas(16) NoiseTables {
     uint64_t g16[24];
     ExtEntry ext[kExtIndex];
     uint8_t perm[kWrap];
@@ -221,6 +223,14 @@ NX_ND float half_to_float(uint32_t bits)
 #if defined(__HIPCC__)
 __device__ const ExtLut kExtLutDev = make_ext_lut();
 
+__device__ __forceinline__ void load_ext_table(NoiseTables &T)
+{
+    const float4 *src = reinterpret_cast<const float4 *>(kExtLutDev.e);
+    float4 *dst = reinterpret_cast<float4 *>(T.ext);
+    constexpr int n4 = (int)(sizeof(ExtLut) / sizeof(float4));
+    for (int i = threadIdx.x; i < n4; i += blockDim.x) dst[i] = src[i];
+}
+
 __device__ __forceinline__ void load_tables(NoiseTables &T, const NoiseTablesPacked &P)
 {
     const uint8_t *pb = reinterpret_cast<const uint8_t *>(P.perm);
@@ -231,23 +241,18 @@ __device__ __forceinline__ void load_tables(NoiseTables &T, const NoiseTablesPac
         T.grad[i] = gb[(i + 255) & 255];
     }
     for (int i = threadIdx.x; i < 48; i += blockDim.x) hw[i] = P.g16[i];
-    const float4 *src = reinterpret_cast<const float4 *>(kExtLutDev.e);
-    float4 *dst = reinterpret_cast<float4 *>(T.ext);
-    for (int i = threadIdx.x; i < kExtIndex * 2; i += blockDim.x) dst[i] = src[i];
+    load_ext_table(T);
 }
 #endif
 
-// One lattice contribution in FP32: attn = 2 - |d|^2 clamped at 0 (the
-// reference adds only when attn > 0) times attn^4 * (g . d), accumulated
-// into v.  The fp16 gradient components feed the FMAs directly.
-// `cap` is 2 for a selected point (attn <= 2 always, so the clamp is the
-// reference's attn > 0 test) and 0 for a point the region excludes.
-NX_ND float clamp0(float a, float cap)
+// Clamp to [0, 1]; on the device this folds into the clamp modifier of the
+// instruction that computes the operand.
+NX_ND float clamp01(float a)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
-    return __builtin_amdgcn_fmed3f(a, 0.0f, cap);
+    return __builtin_amdgcn_fmed3f(a, 0.0f, 1.0f);
 #else
-    return std::fmin(std::fmax(a, 0.0f), cap);
+    return std::fmin(std::fmax(a, 0.0f), 1.0f);
 #endif
 }
 
@@ -262,12 +267,19 @@ NX_ND float float_of_bits(uint32_t b)
 #endif
 }
 
-// `go` is a gradient table entry: the byte offset of the gradient's word.
+// One lattice contribution in FP32, accumulated into v.  The deltas are
+// scaled by 1/sqrt(2), so attn / 2 = 1 - |d / sqrt(2)|^2 (attn <= 2 always),
+// and clamping it to [0, 1] is the reference's attn > 0 test; `m` (1 for a
+// selected point, 0 for one the region excludes) starts the chain, so an
+// excluded point clamps to 0.  The slot adds (attn / 2)^4 * g.(d / sqrt(2)) =
+// attn^4 (g.d) / kSumScale.  The fp16 gradient components feed the FMAs
+// directly; `go` is a gradient table entry (the byte offset of the
+// gradient's word).
 NX_ND float contrib(float v, float dx, float dy, float dz, int go, const NoiseTables &T,
-                    float cap)
+                    float m)
 {
-    float attn = fmaf(-dz, dz, fmaf(-dy, dy, fmaf(-dx, dx, 2.0f)));
-    attn = clamp0(attn, cap);
+    float attn = fmaf(-dz, dz, fmaf(-dy, dy, fmaf(-dx, dx, m)));
+    attn = clamp01(attn);
     const float a2 = attn * attn;
     const uint64_t w = *reinterpret_cast<const uint64_t *>(reinterpret_cast<const uint8_t *>(T.g16) + go);
     const uint32_t w0 = (uint32_t)w, w1 = (uint32_t)(w >> 32);
@@ -284,6 +296,11 @@ NX_ND int hash_full(const NoiseTables &T, int x, int y, int z)
     const int b = T.perm[1 + a + y];
     return T.grad[1 + b + z];
 }
+
+// noise3_sum's result times kSumScale = 16 sqrt(2) is the reference's sum of
+// attn^4 (g . d) (the deltas are scaled by 1 / sqrt(2), attn by 1 / 2).
+constexpr double kSumScaleD = 22.627416997969522;
+constexpr float kSumScale = (float)kSumScaleD;
 
 // Selection record for parity tests (same encoding as the oracle):
 // sel[0..2] = xsb, ysb, zsb; sel[3] = region | ext0 offsets << 2 | ext1
@@ -413,12 +430,19 @@ NX_ND float noise3_sum(double xs, double ys, double zs, const NoiseTables &T, No
     const double dx0d = xins + u, dy0d = yins + u, dz0d = zins + u;
 
     const RegionSel rs = select_region(xins, yins, zins, in_sum);
-    const bool inA = rs.inA, r1 = rs.r1, octa = rs.octa;
+    const bool inA = rs.inA, r1 = rs.r1;
     const int region = inA ? 0 : (r1 ? 1 : 2);
 
     // ---- FP32 section -----------------------------------------------------
-    const float d0x = (float)dx0d, d0y = (float)dy0d, d0z = (float)dz0d;
-    const float S = 1.0f / 3.0f;
+    constexpr float R = 0.70710678118654752f;   // 1 / sqrt(2)
+    const float d0x = (float)dx0d * R, d0y = (float)dy0d * R, d0z = (float)dz0d * R;
+    // every constant offset below is scaled by R as well (S is SQUISH * R)
+    constexpr float S = (float)(1.0 / 3.0 * 0.70710678118654752);
+    constexpr float ONE = 0.70710678118654752f, TWO = (float)(2.0 * 0.70710678118654752);
+    // (1 + 3S = 2: 111's offset in every axis)
+    constexpr float ONE_S = (float)((1.0 + 1.0 / 3.0) * 0.70710678118654752);
+    constexpr float ONE_2S = (float)((1.0 + 2.0 / 3.0) * 0.70710678118654752);
+    constexpr float S2 = (float)(2.0 / 3.0 * 0.70710678118654752);
     const int xm = xsb & 255, ym = ysb & 255, zm = zsb & 255;
 
     // Shared hash prefixes for the unit cube (padded tables: index + 1).
@@ -435,31 +459,34 @@ NX_ND float noise3_sum(double xs, double ys, double zs, const NoiseTables &T, No
     const int g101 = T.grad[2 + p10 + zm];
     const int g011 = T.grad[2 + p01 + zm];
 
-    const float oneBits = r1 ? 0.0f : 2.0f;    // 100, 010, 001
-    const float twoBits = inA ? 0.0f : 2.0f;   // 110, 101, 011
+    const float oneBits = r1 ? 0.0f : 1.0f;    // 010, 001
+    const float twoBits = inA ? 0.0f : 1.0f;   // 101, 011
 
-    // Corner c: delta = d0 - c - S * popcount(c).
+    // Corner c: delta = d0 - c - S * popcount(c).  Slot A holds 100, or 111
+    // in region 1 (where the one-bit corners are out); slot B holds 110, or
+    // 000 in region 0 (where the two-bit corners are out): eight slots cover
+    // every region.
+    const float cAx = r1 ? TWO : ONE_S, cAyz = r1 ? TWO : S;
+    const int gA = T.grad[1 + (r1 ? p11 + 1 : p10) + zm];
+    float v = contrib(0.0f, d0x - cAx, d0y - cAyz, d0z - cAyz, gA, T, 1.0f);
     const float ax = d0x - S, ay = d0y - S, az = d0z - S;
-    float v = contrib(0.0f, ax - 1.0f, ay, az, g100, T, oneBits);
-    v = contrib(v, ax, ay - 1.0f, az, g010, T, oneBits);
-    v = contrib(v, ax, ay, az - 1.0f, g001, T, oneBits);
-    const float bx = d0x - 2.0f * S, by = d0y - 2.0f * S, bz = d0z - 2.0f * S;
-    v = contrib(v, bx - 1.0f, by - 1.0f, bz, g110, T, twoBits);
-    v = contrib(v, bx - 1.0f, by, bz - 1.0f, g101, T, twoBits);
-    v = contrib(v, bx, by - 1.0f, bz - 1.0f, g011, T, twoBits);
-    // Origin slot: 000 in region 0, 111 in region 1.
-    const int go = T.grad[1 + (r1 ? p11 + 1 : p00) + zm];
-    const float k = r1 ? 2.0f : 0.0f;  // 1 + 3S
-    v = contrib(v, d0x - k, d0y - k, d0z - k, go, T, octa ? 0.0f : 2.0f);
-    // Extra vertices from the table.
+    v = contrib(v, ax, ay - ONE, az, g010, T, oneBits);
+    v = contrib(v, ax, ay, az - ONE, g001, T, oneBits);
+    const float cBxy = inA ? 0.0f : ONE_2S, cBz = inA ? 0.0f : S2;
+    const int gB = T.grad[1 + (inA ? p00 : p11) + zm];
+    v = contrib(v, d0x - cBxy, d0y - cBxy, d0z - cBz, gB, T, 1.0f);
+    const float bx = d0x - S2, by = d0y - S2, bz = d0z - S2;
+    v = contrib(v, bx - ONE, by, bz - ONE, g101, T, twoBits);
+    v = contrib(v, bx, by - ONE, bz - ONE, g011, T, twoBits);
+    // Extra vertices from the table (a 32-byte entry, two 16-byte LDS reads).
     const ExtEntry &E = T.ext[rs.idx];
     const int code = E.code;
     const int ge0 = hash_full(T, xm + (code & 3) - 1, ym + ((code >> 2) & 3) - 1,
                               zm + ((code >> 4) & 3) - 1);
     const int ge1 = hash_full(T, xm + ((code >> 6) & 3) - 1, ym + ((code >> 8) & 3) - 1,
                               zm + ((code >> 10) & 3) - 1);
-    v = contrib(v, d0x - E.c0x, d0y - E.c0y, d0z - E.c0z, ge0, T, 2.0f);
-    v = contrib(v, d0x - E.c1x, d0y - E.c1y, d0z - E.c1z, ge1, T, 2.0f);
+    v = contrib(v, d0x - E.c0x, d0y - E.c0y, d0z - E.c0z, ge0, T, 1.0f);
+    v = contrib(v, d0x - E.c1x, d0y - E.c1y, d0z - E.c1z, ge1, T, 1.0f);
 
     if (SEL) {
         // Reference evaluation order of the selected points; the checksum
@@ -502,16 +529,17 @@ NX_ND float noise3(double x, double y, double z, const NoiseTables &T, bool &bad
 {
     const Skewed s = skew3(x, y, z);
     bad = bad | !in_domain(s);
-    return noise3_sum<SEL>(s.x, s.y, s.z, T, sel) * (1.0f / 103.0f);
+    return noise3_sum<SEL>(s.x, s.y, s.z, T, sel) * (kSumScale / 103.0f);
 }
 
 // Per-octave scalars of one sample_octaves call (terrain.py:36-45): octave o
 // adds ((noise + 1) * 0.5 * strength) * radius (terrain.py:28, 43), which is
-// w_o + (w_o / 103) * sum_o with w_o = 0.5 * strength * radius and sum_o the
-// FP32 contribution sum of noise3_sum.
+// w_o + (w_o * kSumScale / 103) * sum_o with w_o = 0.5 * strength * radius
+// and sum_o the FP32 sum of noise3_sum.  The w_o add up in FP64 (base); the
+// weighted sums add up in FP32 over the octaves and join them at the end.
 struct OctavePlan {
     double rough[NX_MAX_OCTAVES];  // frequency / radius of each octave
-    double wsum[NX_MAX_OCTAVES];   // w_o / 103
+    float wsumf[NX_MAX_OCTAVES];   // w_o * kSumScale / 103 (FP32)
     double base;                   // sum of w_o over the octaves
     int32_t n_oct;
     int32_t check_oct;             // octave with the largest |rough|
@@ -555,10 +583,11 @@ __device__ __forceinline__ double octave_sum(double x, double y, double z, doubl
     const double rc = P.rough[P.check_oct];
     bad = bad | !in_domain(skew3(x * rc, y * rc, z * rc));
     acc += P.base;
+    float accf = 0.0f;
     auto octave = [&](int o, const Skewed &s) {
         NoiseSel sel;
         const float v = noise3_sum<SEL>(s.x, s.y, s.z, T, &sel);
-        acc = fma((double)v, P.wsum[o], acc);
+        accf = fmaf(v, P.wsumf[o], accf);
         on_octave(o, v, sel);
     };
     if (dbl) {
@@ -575,7 +604,7 @@ __device__ __forceinline__ double octave_sum(double x, double y, double z, doubl
             octave(o, skew3(x * r, y * r, z * r));
         }
     }
-    return acc;
+    return acc + (double)accf;
 }
 #endif
 
