@@ -189,14 +189,31 @@ def bench_erosion(args):
     dv = torch.from_numpy(pts).cuda()
     h = terrain.sample_octaves_device(dv, None, perm, pgi, rough, strength, 1.0)
     h, _ = util.rescale_device(h, -9000, 9000)
+    from nixis_amd import autotune
     t0 = time.perf_counter()
-    graph = erosion.ErosionGraph(pts, nb)      # SFC order + tile plan
+    order = util.sfc_order(pts)
+    graph = erosion.ErosionGraph(pts, nb, order=order)      # SFC order + tile plan
     hd = graph.permute_device(h)
+    log(f"[erosion] SFC order, ELL table and tile plan in {time.perf_counter() - t0:.1f}s")
+    # tile chunk, kernel variant and graph chunk from the autotuner
+    tuning = autotune.lookup_erosion(V)
+    tuned_now = tuning is None
+    if tuned_now:
+        t0 = time.perf_counter()
+        tuning, table = autotune.tune_erosion(
+            lambda c: graph if c == 0 else erosion.ErosionGraph(pts, nb, order=order, chunk=c),
+            heights=hd, steps=20, reps=2)
+        log(f"[erosion] autotuned in {time.perf_counter() - t0:.1f}s: chunk {tuning.chunk} "
+            f"variant {tuning.variant} graph chunk {tuning.graph_chunk}; "
+            f"{len(table)} candidates, {min(table)[0]:.1f}..{max(table)[0]:.1f} us/step")
+    if tuning.chunk != graph.chunk:
+        graph = erosion.ErosionGraph(pts, nb, order=order, **tuning.graph_kwargs())
+    else:
+        graph.variant, graph.graph_chunk = tuning.variant, tuning.graph_chunk
     st = erosion.ErosionState(V)
     nnz = int((nb >= 0).sum())
     plan = graph.plan
-    log(f"[erosion] SFC order, ELL table and tile plan in {time.perf_counter() - t0:.1f}s: "
-        + (f"{plan.ntiles} tiles" if plan is not None else "no tile plan"))
+    log("[erosion] " + (f"{plan.ntiles} tiles" if plan is not None else "no tile plan"))
     # warm-up run, then the timed run from the same initial state
     st.load(hd)
     st.run(graph, min(args.warmup * 10, iters), 1.0)
@@ -241,6 +258,10 @@ def bench_erosion(args):
                      "traffic": profiled_traffic("erosion", V) if spl == 2 else None,
                      "algorithmic_bytes_per_launch": spl * B, "steps_per_launch": spl,
                      "launch_us": round(spl * per_step_ms * 1e3, 2)},
+        "tuning": {"chunk": tuning.chunk, "variant": tuning.variant,
+                   "graph_chunk": tuning.graph_chunk,
+                   "tiles": plan.ntiles if plan is not None else 0,
+                   "source": "autotuned now" if tuned_now else "autotune cache"},
         "checks": {"finite": bool(np.isfinite(hh).all()),
                    "h_plus_s_drift_max_ulps": round(drift_ulps_max, 2),
                    "h_plus_s_drift_bound_ulps": iters,

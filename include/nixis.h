@@ -347,6 +347,16 @@ int nx_tile_plan_build_steps(const int32_t *col, const double *elen, int64_t V,
                              int steps, void *out, int64_t cap,
                              nx_tile_plan *plan);
 
+/* nx_tile_plan_build_steps with the starting tile size `chunk` (kTileMin =
+ * 32 .. 1024 vertices; 0 for the default, 768 for 2 steps and 1024 for 4).
+ * Tiles are halved from there until their rings fit, so the chunk trades
+ * ring overhead against the number of blocks per launch; the erosion
+ * autotuner picks it per mesh size.  The results of erosion do not depend on
+ * it. */
+int nx_tile_plan_build_chunk(const int32_t *col, const double *elen, int64_t V,
+                             int steps, int chunk, void *out, int64_t cap,
+                             nx_tile_plan *plan);
+
 /* nx_erosion_run with a tile plan (NULL plan: exactly nx_erosion_run).
  * Steps go plan->steps per launch; the remainder runs one per launch. */
 int nx_erosion_run_tiled(nx_ctx *ctx, void *state, int64_t V,

@@ -4,6 +4,7 @@ The shared library is built in-tree by ``nixis_amd.build``.  There is no
 fallback: if the library is missing, or no HIP device is present when a
 device entry point is used, the call raises.
 """
+import contextlib
 import ctypes
 import os
 import threading
@@ -148,6 +149,8 @@ SIGNATURES = {
                                     ctypes.POINTER(NxTilePlan)]),
     "nx_tile_plan_build_steps": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_int, _c_void_p, _c_i64,
                                           ctypes.POINTER(NxTilePlan)]),
+    "nx_tile_plan_build_chunk": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_int, _c_int, _c_void_p,
+                                          _c_i64, ctypes.POINTER(NxTilePlan)]),
     "nx_erosion_run_tiled": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p, _c_void_p,
                                       ctypes.POINTER(NxTilePlan), _c_int, _c_double, _c_void_p]),
     "nx_erosion_run_rule": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p, _c_void_p,
@@ -224,6 +227,10 @@ class Context:
         check(lib().nx_ctx_create(int(device), ctypes.byref(h)))
         self.handle = h
         self.device = device
+        # the C context's option values (its defaults until set here)
+        self.graph_chunk = 32
+        self.pair_kernel = 0
+        self.multi_kernel = 0
 
     @classmethod
     def get(cls, device=None):
@@ -246,14 +253,34 @@ class Context:
     def set_graph_chunk(self, steps):
         """Erosion steps per captured hipGraph (0: eager launches)."""
         check(lib().nx_ctx_set_option(self.handle, NX_OPT_GRAPH_CHUNK, int(steps)))
+        self.graph_chunk = int(steps)
 
     def set_multi_kernel(self, variant):
         """Four-step erosion kernel variant 0..3 (0: default); same results."""
         check(lib().nx_ctx_set_option(self.handle, NX_OPT_MULTI_KERNEL, int(variant)))
+        self.multi_kernel = int(variant)
 
     def set_pair_kernel(self, variant):
         """Two-step erosion kernel variant 0..6 (0: default); same results."""
         check(lib().nx_ctx_set_option(self.handle, NX_OPT_PAIR_KERNEL, int(variant)))
+        self.pair_kernel = int(variant)
+
+    @contextlib.contextmanager
+    def options(self, graph_chunk=None, pair_kernel=None):
+        """Set the given erosion options for the body of a with-block and
+        restore the previous values after it (None leaves one alone)."""
+        old = (self.graph_chunk, self.pair_kernel)
+        try:
+            if graph_chunk is not None and graph_chunk != old[0]:
+                self.set_graph_chunk(graph_chunk)
+            if pair_kernel is not None and pair_kernel != old[1]:
+                self.set_pair_kernel(pair_kernel)
+            yield self
+        finally:
+            if self.graph_chunk != old[0]:
+                self.set_graph_chunk(old[0])
+            if self.pair_kernel != old[1]:
+                self.set_pair_kernel(old[1])
 
 
 def stream_ptr(stream=None):

@@ -148,20 +148,36 @@ class MeshCache:
             self._verts[d] = v
         return v
 
-    def graph(self, d):
+    def graph(self, d, tuning=None):
+        """The erosion graph of subdivision d, its tile plan cut with the
+        autotuned settings for its size class when the tuning cache has them
+        (`tuning` overrides; the results are the same either way)."""
         g = self._graphs.get(d)
         if g is None:
-            if self.on_device:
-                pts, cells = _util.create_mesh_device(d, self.device)
-                adj = _util.sort_adjacency_device(
-                    _util.build_adjacency_device(cells, pts.shape[0]))
-                del cells
-                g = _erosion.ErosionGraph.from_device(pts, adj, order=self._order_dev(d))
-            else:
-                pts, nb = _util.icosphere_with_neighbors(d)
-                g = _erosion.ErosionGraph(pts, nb, order=self.order(d), device=self.device)
+            if tuning is None:
+                from . import autotune
+                tuning = autotune.lookup_erosion(10 * d * d + 2)
+            kw = tuning.graph_kwargs() if tuning is not None else {}
+            g = self._build_graph(d, **kw)
             self._graphs[d] = g
         return g
+
+    def _build_graph(self, d, **kw):
+        if self.on_device:
+            pts, cells = _util.create_mesh_device(d, self.device)
+            adj = _util.sort_adjacency_device(_util.build_adjacency_device(cells, pts.shape[0]))
+            del cells
+            return _erosion.ErosionGraph.from_device(pts, adj, order=self._order_dev(d), **kw)
+        pts, nb = _util.icosphere_with_neighbors(d)
+        return _erosion.ErosionGraph(pts, nb, order=self.order(d), device=self.device, **kw)
+
+    def tune_erosion(self, d, **kw):
+        """Autotune erosion for subdivision d's size class (autotune.tune_erosion)
+        and keep the tuned graph; returns (tuning, table)."""
+        from . import autotune
+        best, table = autotune.tune_erosion(lambda c: self._build_graph(d, chunk=c), **kw)
+        self._graphs[d] = self._build_graph(d, **best.graph_kwargs())
+        return best, table
 
     def evict(self, d):
         self._verts.pop(d, None)

@@ -143,21 +143,21 @@ int nx_ctx_destroy(nx_ctx *ctx)
 int nx_ctx_set_option(nx_ctx *ctx, int option, int64_t value)
 {
     if (!ctx) return set_error(NX_EINVAL, "null context");
+    // Cached graphs stay valid: their keys hold the kernel variant and the
+    // number of steps they cover, so no graph captured under other options
+    // is replayed, and none that may still be running is destroyed here.
     std::lock_guard<std::mutex> lk(ctx->mu);
     switch (option) {
     case NX_OPT_GRAPH_CHUNK:
         if (value < 0 || value > 4096) return set_error(NX_EINVAL, "graph chunk %lld outside 0..4096", (long long)value);
-        ctx->clear_graphs();
         ctx->graph_chunk = (int)value;
         return NX_OK;
     case NX_OPT_MULTI_KERNEL:
         if (value < 0 || value > 3) return set_error(NX_EINVAL, "multi-step kernel variant %lld outside 0..3", (long long)value);
-        ctx->clear_graphs();
         ctx->multi_variant = (int)value;
         return NX_OK;
     case NX_OPT_PAIR_KERNEL:
         if (value < 0 || value > 6) return set_error(NX_EINVAL, "pair kernel variant %lld outside 0..6", (long long)value);
-        ctx->clear_graphs();
         ctx->pair_variant = (int)value;
         return NX_OK;
     default:
@@ -425,6 +425,12 @@ int nx_tile_plan_build_steps(const int32_t *col, const double *elen, int64_t V, 
                              void *out, int64_t cap, nx_tile_plan *plan)
 {
     return tile_plan_build(col, elen, V, steps, out, cap, plan);
+}
+
+int nx_tile_plan_build_chunk(const int32_t *col, const double *elen, int64_t V, int steps,
+                             int chunk, void *out, int64_t cap, nx_tile_plan *plan)
+{
+    return tile_plan_build(col, elen, V, steps, out, cap, plan, chunk);
 }
 
 int nx_erosion_store(nx_ctx *ctx, const void *state, int64_t V, double *h, double *w, double *s,

@@ -668,7 +668,7 @@ int batch_erode(nx_ctx *ctx, nx_planet *planets, int n, int max_steps, void *ws,
             if (itg != ctx->graphs.end()) {
                 ex = itg->second;
             } else {
-                if (ctx->graphs.size() >= kMaxGraphs) ctx->clear_graphs();
+                ctx->make_room_for_graph();
                 hipGraph_t g = nullptr;
                 NX_HIP(hipStreamBeginCapture(ctx->capture_stream, hipStreamCaptureModeThreadLocal));
                 launch_all(ctx->capture_stream);

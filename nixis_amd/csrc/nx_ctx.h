@@ -68,6 +68,15 @@ struct nx_ctx {
         for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
         graphs.clear();
     }
+    // A full cache is emptied once the device has finished every launch, so
+    // no graph that may still be running is destroyed.
+    void make_room_for_graph()
+    {
+        if (graphs.size() < kMaxGraphs) return;
+        (void)hipSetDevice(device);
+        (void)hipDeviceSynchronize();
+        clear_graphs();
+    }
 };
 
 #define NX_HIP(call)                                                                      \

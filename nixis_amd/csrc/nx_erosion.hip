@@ -365,7 +365,7 @@ static int chunk_graph(nx_ctx *ctx, const ErosionBufs &b, void *state, int64_t V
         *out = it->second;
         return NX_OK;
     }
-    if (ctx->graphs.size() >= kMaxGraphs) ctx->clear_graphs();
+    ctx->make_room_for_graph();
     hipGraph_t g = nullptr;
     NX_HIP(hipStreamBeginCapture(ctx->capture_stream, hipStreamCaptureModeThreadLocal));
     launch_steps(b, V, col, elen, plan, ctx->pair_variant, ctx->multi_variant, rule, n, rain, last,

@@ -45,7 +45,7 @@ int ell_from_adjacency(const int32_t *adj, int64_t V, const double *verts,
 
 // Tile plan for two steps per launch (nx_tiles.cpp).
 int tile_plan_build(const int32_t *col, const double *elen, int64_t V, int K, void *out,
-                    int64_t cap, nx_tile_plan *plan);
+                    int64_t cap, nx_tile_plan *plan, int chunk = 0);
 
 // Host-side permutation tables (nx_api.cpp).
 int init_tables(int64_t seed, int32_t *perm, int32_t *pgi);

@@ -5,7 +5,8 @@
 // outside the tile and the earlier rings.  K Jacobi steps of the tile need
 // the fields of the tile and K rings (erosion.py:197-279 reads only the
 // one-ring of each vertex): step j runs on the tile and rings 1..K-j.  Tiles
-// start at kPairChunk vertices for two steps (kTileMax for more) and are
+// start at `chunk` vertices (by default kPairChunk for two steps, kTileMax
+// for more; an autotuner may pick another per mesh size) and are
 // halved until tile + rings fit the kernel's load limit and, for two steps,
 // tile + ring 1 fits kStepOneMax; a space-filling-curve order (compact tiles,
 // short perimeters) gives few, full tiles; a graph without locality fails
@@ -110,15 +111,18 @@ inline int64_t a256(int64_t x) { return (x + 255) & ~(int64_t)255; }
 }  // namespace
 
 int tile_plan_build(const int32_t *col, const double *elen, int64_t V, int K, void *out,
-                    int64_t cap, nx_tile_plan *plan)
+                    int64_t cap, nx_tile_plan *plan, int chunk_in)
 {
     if (!col || !elen || !plan || V <= 0 || V > INT32_MAX)
         return set_error(NX_EINVAL, "tile plan needs col, elen, plan and 0 < V < 2^31");
     if (K != 2 && K != kMultiSteps)
         return set_error(NX_EINVAL, "tile plan for %d steps per launch: 2 or %d", K, kMultiSteps);
+    if (chunk_in != 0 && (chunk_in < kTileMin || chunk_in > kTileMax))
+        return set_error(NX_EINVAL, "tile chunk %d outside %d..%d (0 for the default)", chunk_in,
+                         kTileMin, kTileMax);
     const int limit = K == 2 ? kLoadMax : kLoadMaxMulti;
     const int step1_limit = K == 2 ? kStepOneMax : kLoadMaxMulti;
-    const int64_t chunk = K == 2 ? kPairChunk : kTileMax;
+    const int64_t chunk = chunk_in ? chunk_in : (K == 2 ? kPairChunk : kTileMax);
     for (int64_t i = 0; i < 6 * V; ++i)
         if (col[i] < 0 || col[i] >= V) return set_error(NX_EINVAL, "col[%lld] = %d outside 0..V-1", (long long)i, col[i]);
 

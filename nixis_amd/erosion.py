@@ -36,12 +36,18 @@ class ErosionGraph:
     (caller order) or an explicit order[new] = old.  Fields passed to
     ErosionState use the device numbering (``permute``/``unpermute``).
     With `tiled`, the tile plan is built when the graph cuts into tiles
-    (nx_tile_plan_build_steps); otherwise, or for graphs without locality,
+    (nx_tile_plan_build_chunk); otherwise, or for graphs without locality,
     every step is its own launch.  `steps` (2 or 4) is the number of steps
     one launch covers: a 4-step plan carries four rings per tile.
+    `chunk` is the plan's starting tile size (0: the library's default),
+    `variant` the two-step kernel variant and `graph_chunk` the steps per
+    captured graph that runs on this graph use (None: the context's
+    setting); autotune.tune_erosion picks the three per mesh size.  None of
+    them changes the results.
     """
 
-    def __init__(self, verts, neighbors, order="sfc", device=None, tiled=True, steps=2):
+    def __init__(self, verts, neighbors, order="sfc", device=None, tiled=True, steps=2,
+                 chunk=0, variant=None, graph_chunk=None):
         import torch
         verts = np.ascontiguousarray(verts, dtype=np.float64)
         nb = np.ascontiguousarray(neighbors, dtype=np.int32)
@@ -69,11 +75,18 @@ class ErosionGraph:
         self.plan = None
         self._plan_blob = None
         self.steps = int(steps)
+        self._tuning(chunk, variant, graph_chunk)
         if tiled:
             self._build_plan(col, elen)
 
+    def _tuning(self, chunk, variant, graph_chunk):
+        self.chunk = int(chunk)
+        self.variant = None if variant is None else int(variant)
+        self.graph_chunk = None if graph_chunk is None else int(graph_chunk)
+
     @classmethod
-    def from_device(cls, verts, neighbors, order="sfc", tiled=True, stream=None, steps=2):
+    def from_device(cls, verts, neighbors, order="sfc", tiled=True, stream=None, steps=2,
+                    chunk=0, variant=None, graph_chunk=None):
         """The same graph from device tensors (verts float64 [V,3], neighbours
         int32 [V,6]), built on the device (nx_sfc_order_device,
         nx_ell_from_adjacency_device); the tile plan is cut on the host from
@@ -106,6 +119,7 @@ class ErosionGraph:
         self.plan = None
         self._plan_blob = None
         self.steps = int(steps)
+        self._tuning(chunk, variant, graph_chunk)
         if tiled:
             self._build_plan(col.cpu().numpy(), elen.cpu().numpy())
         return self
@@ -114,14 +128,15 @@ class ErosionGraph:
         import torch
         L = _lib.lib()
         plan = _lib.NxTilePlan()
-        rc = L.nx_tile_plan_build_steps(_lib.ptr(col), _lib.ptr(elen), self.V, self.steps, None,
-                                        0, ctypes.byref(plan))
+        rc = L.nx_tile_plan_build_chunk(_lib.ptr(col), _lib.ptr(elen), self.V, self.steps,
+                                        self.chunk, None, 0, ctypes.byref(plan))
         if rc == _lib.NX_ERANGE:
             return
         _lib.check(rc)
         img = np.zeros(plan.bytes, dtype=np.uint8)
-        _lib.check(L.nx_tile_plan_build_steps(_lib.ptr(col), _lib.ptr(elen), self.V, self.steps,
-                                              _lib.ptr(img), plan.bytes, ctypes.byref(plan)))
+        _lib.check(L.nx_tile_plan_build_chunk(_lib.ptr(col), _lib.ptr(elen), self.V, self.steps,
+                                              self.chunk, _lib.ptr(img), plan.bytes,
+                                              ctypes.byref(plan)))
         blob = torch.from_numpy(img).to(self.device)
         if blob.data_ptr() % 256:
             raise RuntimeError("tile plan copy is not 256-byte aligned")
@@ -203,10 +218,12 @@ class ErosionState:
         if graph.V != self.V:
             raise ValueError("graph and state sizes differ")
         plan = ctypes.byref(graph.plan) if graph.plan is not None else None
-        _lib.check(_lib.lib().nx_erosion_run_rule(self.ctx.handle, _lib.ptr(self.buf), self.V,
-                                                  _lib.ptr(graph.col), _lib.ptr(graph.elen),
-                                                  plan, int(iters), float(rain), int(rule),
-                                                  _lib.stream_ptr(stream)))
+        with self.ctx.options(graph_chunk=getattr(graph, "graph_chunk", None),
+                              pair_kernel=getattr(graph, "variant", None)):
+            _lib.check(_lib.lib().nx_erosion_run_rule(
+                self.ctx.handle, _lib.ptr(self.buf), self.V, _lib.ptr(graph.col),
+                _lib.ptr(graph.elen), plan, int(iters), float(rain), int(rule),
+                _lib.stream_ptr(stream)))
 
     def store(self, h=None, w=None, s=None, stream=None):
         _lib.check(_lib.lib().nx_erosion_store(self.ctx.handle, _lib.ptr(self.buf), self.V,
@@ -246,7 +263,10 @@ def graph_for(verts, neighbors, order="sfc", tiled=True):
            _digest(np.asarray(neighbors, dtype=np.int32)), order, bool(tiled), dev)
     g = _GRAPHS.pop(key, None)
     if g is None:
-        g = ErosionGraph(verts, neighbors, order=order, tiled=tiled)
+        from . import autotune
+        t = autotune.lookup_erosion(np.shape(neighbors)[0]) if tiled else None
+        g = ErosionGraph(verts, neighbors, order=order, tiled=tiled,
+                         **(t.graph_kwargs() if t is not None else {}))
         while len(_GRAPHS) >= _GRAPHS_MAX:
             _GRAPHS.pop(next(iter(_GRAPHS)))
     _GRAPHS[key] = g          # most recently used last

@@ -424,3 +424,42 @@ def test_erode_terrain3_snapshots(oracle, tmp_path):
         assert got.shape == (H, W) and got.dtype == np.uint8
         assert (got == want).all(), f"pass {i + 1}"
     assert (href == hs).all()
+
+
+def test_erosion_autotune_picks_persists_and_keeps_results(oracle, tmp_path):
+    """autotune.tune_erosion times kernel variants, tile chunks and graph
+    chunks on a mesh, stores the best for the mesh's size class, and
+    lookup_erosion returns it.  Erosion on the tuned graph, and on other
+    candidate settings, gives the oracle's heights bit for bit; the tuned
+    options hold only for the run and the context's own come back after it."""
+    from nixis_amd import autotune, batch
+    pts, nb, h = _planet(48)
+    order = util.sfc_order(pts)
+    path = str(tmp_path / "tune.json")
+    best, table = autotune.tune_erosion(
+        lambda c: erosion.ErosionGraph(pts, nb, order=order, chunk=c), steps=6, reps=1,
+        path=path, variants=(0, 1, 5), chunks=(256, 512), graph_chunks=(8, 32))
+    assert len(table) == 3 + 2 + 2
+    assert best.variant in (0, 1, 5) and best.chunk in (0, 256, 512) and best.graph_chunk in (8, 32)
+    assert autotune.lookup_erosion(pts.shape[0], path=path) == best
+    href = h.copy()
+    oracle.erode_terrain3(pts, nb, href, 21, 1.0)
+    ctx = _lib.Context.get()
+    tiles = set()
+    for kw in (best.graph_kwargs(), {"chunk": 256, "variant": 5, "graph_chunk": 8},
+               {"chunk": 512, "variant": 1, "graph_chunk": 0}):
+        g = erosion.ErosionGraph(pts, nb, order=order, **kw)
+        tiles.add(g.plan.ntiles)
+        st = erosion.ErosionState(g.V)
+        st.load(torch.from_numpy(g.permute(h)).cuda())
+        st.run(g, 21, 1.0)
+        out = torch.empty(g.V, dtype=torch.float64, device="cuda")
+        st.store(out)
+        st.ctx.sync_check()
+        assert (g.unpermute(out.cpu().numpy()) == href).all(), kw
+        assert ctx.graph_chunk == 32 and ctx.pair_kernel == 0
+    assert len(tiles) >= 2      # the chunks cut different plans
+    # MeshCache cuts its plans with a given tuning
+    mc = batch.MeshCache()
+    g = mc.graph(48, tuning=autotune.ErosionTuning(256, 5, 8))
+    assert (g.chunk, g.variant, g.graph_chunk) == (256, 5, 8)
