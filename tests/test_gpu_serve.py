@@ -54,10 +54,14 @@ def test_plain_server_matches_served():
 def test_budgeted_ticks_match_plain():
     """Erosion ticks sized by a vertex-step budget (per-planet step caps, a
     budget small enough to split every job over many ticks, and the default
-    one) give each job the plain single-planet result."""
+    one), with younger jobs sharing the oldest job's ticks or not and with
+    noise passes held to a share of the ticks, give each job the plain
+    single-planet result."""
     jobs = _trace()
     a = serve.PlainServer().serve(jobs, keep_results=True)
-    for budget in (60_000, 400_000_000):
-        b = serve.PlanetServer(tick_vertex_steps=budget).serve(jobs, keep_results=True)
+    for budget, young, noise in ((60_000, 1.0, None), (400_000_000, 1.0, None),
+                                 (400_000_000, 0.0, None), (200_000, 0.3, 0.05)):
+        b = serve.PlanetServer(tick_vertex_steps=budget, young_share=young,
+                               noise_share=noise).serve(jobs, keep_results=True)
         for x, y in zip(a, b):
             assert (x.heights.cpu().numpy() == y.heights.cpu().numpy()).all(), (budget, x.job)

@@ -78,7 +78,7 @@ def test_erosion_ticks_hand_out_whole_pairs(budget, per_tick):
     per tick (rounding down to whole pairs never reaches zero steps)."""
     from nixis_amd.serve import PlanetServer
     srv = PlanetServer.__new__(PlanetServer)
-    srv.tick_vertex_steps, srv.steps_per_tick = budget, per_tick
+    srv.tick_vertex_steps, srv.steps_per_tick, srv.young_share = budget, per_tick, 1.0
     runs = [_Run(1000, 31), _Run(400, 7), _Run(3000, 10)]
     eroding, ticks = list(runs), 0
     while eroding:
@@ -92,3 +92,59 @@ def test_erosion_ticks_hand_out_whole_pairs(budget, per_tick):
         ticks += 1
         assert ticks < 100
     assert [r.steps_done for r in runs] == [31, 7, 10]
+
+
+def test_noise_share_admits_first_fit_within_the_tick_budget():
+    """With jobs eroding and a noise share set, a tick starts the waiting
+    jobs whose noise fits noise_share times the tick's estimated erosion
+    time, first fit in arrival order, and always at least the first; the
+    skipped jobs stay waiting in their order.  Without eroding jobs, or
+    without a share, every job that fits the HBM budget starts."""
+    from collections import deque
+    from nixis_amd import serve
+    from nixis_amd.serve import PlanetServer
+    srv = PlanetServer.__new__(PlanetServer)
+    srv.tick_vertex_steps, srv.steps_per_tick, srv.young_share = 10**12, 0, 1.0
+    eroding = [_Run(1_000_000, 40)]          # tick: 40 steps x 1e6 vertices
+    tick_s = 40 * 1_000_000 / serve.EROSION_RATE
+
+    def job(vo_s, name):
+        # a job whose noise takes vo_s seconds at NOISE_RATE
+        d = 2
+        octs = max(1, int(round(vo_s * serve.NOISE_RATE / (10 * d * d + 2))))
+        return Job(d=d, seed=0, n_octaves=octs, name=name)
+
+    share = 0.5
+    a, b, c, dd = (job(0.6 * share * tick_s, "a"), job(0.6 * share * tick_s, "b"),
+                   job(0.3 * share * tick_s, "c"), job(0.05 * share * tick_s, "d"))
+    for noise_share, erode, want, left in ((share, eroding, "acd", "b"), (None, eroding, "abcd", ""),
+                                           (share, [], "abcd", "")):
+        srv.noise_share = noise_share
+        waiting = deque([a, b, c, dd])
+        adm = Admission(budget=10**12, job_cost=cost)
+        got = srv._admit(waiting, adm, erode)
+        assert "".join(j.name for j in got) == want
+        assert "".join(j.name for j in waiting) == left
+    # the first job always starts, even when its noise alone exceeds the budget
+    srv.noise_share = share
+    waiting = deque([job(10 * tick_s, "x"), a])
+    got = srv._admit(waiting, Admission(budget=10**12, job_cost=cost), eroding)
+    assert [j.name for j in got] == ["x"] and [j.name for j in waiting] == ["a"]
+
+
+def test_young_share_keeps_the_oldest_job_alone():
+    """young_share 0 gives the oldest eroding job the whole tick; a share
+    lets younger jobs use at most that share of the tick's budget."""
+    from nixis_amd.serve import PlanetServer
+    srv = PlanetServer.__new__(PlanetServer)
+    srv.tick_vertex_steps, srv.steps_per_tick = 100_000, 0
+    runs = [_Run(1000, 50), _Run(100, 40), _Run(100, 40)]   # the oldest leaves 50,000
+    srv.young_share = 0.0
+    tick, caps = srv._tick(runs)
+    assert tick == runs[:1] and caps == [50]
+    srv.young_share = 0.05            # at most 5,000 vertex-steps for the younger jobs
+    tick, caps = srv._tick(runs)
+    assert tick == runs and caps == [50, 40, 10]
+    srv.young_share = 1.0
+    tick, caps = srv._tick(runs)
+    assert tick == runs and caps == [50, 40, 40]
