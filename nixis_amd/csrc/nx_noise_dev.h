@@ -438,8 +438,8 @@ NX_ND float noise3_sum(double xs, double ys, double zs, const NoiseTables &T, No
     const float d0x = (float)dx0d * R, d0y = (float)dy0d * R, d0z = (float)dz0d * R;
     // every constant offset below is scaled by R as well (S is SQUISH * R)
     constexpr float S = (float)(1.0 / 3.0 * 0.70710678118654752);
+    // (111's offset is 1 + 3 * SQUISH = 2 in every axis)
     constexpr float ONE = 0.70710678118654752f, TWO = (float)(2.0 * 0.70710678118654752);
-    // (1 + 3S = 2: 111's offset in every axis)
     constexpr float ONE_S = (float)((1.0 + 1.0 / 3.0) * 0.70710678118654752);
     constexpr float ONE_2S = (float)((1.0 + 2.0 / 3.0) * 0.70710678118654752);
     constexpr float S2 = (float)(2.0 / 3.0 * 0.70710678118654752);
@@ -459,23 +459,25 @@ NX_ND float noise3_sum(double xs, double ys, double zs, const NoiseTables &T, No
     const int g101 = T.grad[2 + p10 + zm];
     const int g011 = T.grad[2 + p01 + zm];
 
-    const float oneBits = r1 ? 0.0f : 1.0f;    // 010, 001
-    const float twoBits = inA ? 0.0f : 1.0f;   // 101, 011
+    const float r1f = r1 ? 1.0f : 0.0f, inAf = inA ? 1.0f : 0.0f;
+    const float oneBits = 1.0f - r1f;    // 010, 001
+    const float twoBits = 1.0f - inAf;   // 101, 011
 
     // Corner c: delta = d0 - c - S * popcount(c).  Slot A holds 100, or 111
     // in region 1 (where the one-bit corners are out); slot B holds 110, or
     // 000 in region 0 (where the two-bit corners are out): eight slots cover
-    // every region.
-    const float cAx = r1 ? TWO : ONE_S, cAyz = r1 ? TWO : S;
-    const int gA = T.grad[1 + (r1 ? p11 + 1 : p10) + zm];
-    float v = contrib(0.0f, d0x - cAx, d0y - cAyz, d0z - cAyz, gA, T, 1.0f);
+    // every region.  The two slots' deltas move from the first corner to the
+    // second by an FMA with the region flag (0 or 1).
     const float ax = d0x - S, ay = d0y - S, az = d0z - S;
+    const int gA = T.grad[1 + (r1 ? p11 + 1 : p10) + zm];
+    float v = contrib(0.0f, fmaf(-r1f, TWO - ONE_S, ax - ONE), fmaf(-r1f, TWO - S, ay),
+                      fmaf(-r1f, TWO - S, az), gA, T, 1.0f);
     v = contrib(v, ax, ay - ONE, az, g010, T, oneBits);
     v = contrib(v, ax, ay, az - ONE, g001, T, oneBits);
-    const float cBxy = inA ? 0.0f : ONE_2S, cBz = inA ? 0.0f : S2;
-    const int gB = T.grad[1 + (inA ? p00 : p11) + zm];
-    v = contrib(v, d0x - cBxy, d0y - cBxy, d0z - cBz, gB, T, 1.0f);
     const float bx = d0x - S2, by = d0y - S2, bz = d0z - S2;
+    const int gB = T.grad[1 + (inA ? p00 : p11) + zm];
+    v = contrib(v, fmaf(inAf, ONE_2S, bx - ONE), fmaf(inAf, ONE_2S, by - ONE),
+                fmaf(inAf, S2, bz), gB, T, 1.0f);
     v = contrib(v, bx - ONE, by, bz - ONE, g101, T, twoBits);
     v = contrib(v, bx, by - ONE, bz - ONE, g011, T, twoBits);
     // Extra vertices from the table (a 32-byte entry, two 16-byte LDS reads).
