@@ -306,7 +306,9 @@ int nx_erosion_run(nx_ctx *ctx, void *state, int64_t V, const int32_t *col,
  * outside neighbours (ring 1, ring 2, ...) bring the total to at most
  * max_load; each tile carries its rings' global indices and, for the tile
  * and all rings but the last, the neighbour slots renumbered into that local
- * set plus their edge lengths.  The plan is a byte image: nx_tile_plan_build
+ * set plus their edge lengths, each undirected edge's length stored once
+ * (a slot names the column, and whether its own vertex or the neighbour
+ * holds it).  The plan is a byte image: nx_tile_plan_build
  * fills it on the host, the caller copies it to device memory (256-byte
  * aligned) and stores that address in `blob`.  The same results as
  * nx_erosion_run (erosion.py:197-279 per step), with about half the HBM
@@ -323,6 +325,7 @@ typedef struct nx_tile_plan {
     int32_t steps;       /* steps per launch the plan serves: 2 or 4 */
     int32_t max_step1;   /* largest tile + ring 1 (first-step vertices) */
     int64_t off_pref;    /* int32 [ntiles][steps + 1]: tile, tile + ring 1, ... */
+    int64_t off_ebase;   /* int64 [ntiles + 1]: each tile's edge lengths in lel */
 } nx_tile_plan;
 
 /* Build the plan from the host copies of col [6,V] and elen [6,V] (as

@@ -55,7 +55,8 @@ class NxTilePlan(ctypes.Structure):
                 ("max_load", ctypes.c_int32)] + [
         (f"off_{n}", ctypes.c_int64)
         for n in ("start", "cs", "load", "rbase", "ring", "cbase", "lcol", "lel")] + [
-        ("steps", ctypes.c_int32), ("max_step1", ctypes.c_int32), ("off_pref", ctypes.c_int64)]
+        ("steps", ctypes.c_int32), ("max_step1", ctypes.c_int32), ("off_pref", ctypes.c_int64),
+        ("off_ebase", ctypes.c_int64)]
 
 
 class NxRescaleOpts(ctypes.Structure):

@@ -288,7 +288,7 @@ __global__ __launch_bounds__(1024) void k_erode_batch_pair_one(const ErosionMeta
                                                                int64_t total_iters, double rain)
 {
     __shared__ double2 A[kLoadMax];
-    __shared__ double2 B[kLoadMax];
+    __shared__ double2 B[kStepOneMax];
     const int64_t tile = xcd_tile(blockIdx.x, plan.ntiles);
     if (tile >= plan.ntiles) return;
     const ErosionMeta m = *meta;
@@ -324,7 +324,7 @@ __global__ __launch_bounds__(1024) void k_erode_batch_pair_args(const PairLaunch
                                                                 int32_t nt_total, int64_t k)
 {
     __shared__ double2 A[kLoadMax];
-    __shared__ double2 B[kLoadMax];
+    __shared__ double2 B[kStepOneMax];
     const int b = (int)xcd_tile(blockIdx.x, nt_total);
     if (b >= nt_total) return;
     // b and the prefix entries lie in [0, 2^31): b - prefix[q] is negative,
@@ -352,7 +352,7 @@ __global__ __launch_bounds__(1024) void k_erode_batch_pair(const BatchPlanetDev 
                                                            int64_t k)
 {
     __shared__ double2 A[kLoadMax];
-    __shared__ double2 B[kLoadMax];
+    __shared__ double2 B[kStepOneMax];
     const int64_t b = xcd_tile(blockIdx.x, nt_total);
     if (b >= nt_total) return;
     // The planet is the same for the whole block: keep its descriptor in
@@ -448,7 +448,7 @@ static int fill_desc(const nx_planet &q, BatchPlanetDev &d)
             const nx_tile_plan &pl = *q.plan;
             if (!pl.blob || ((uintptr_t)pl.blob & 255) || pl.V != q.V || pl.ntiles <= 0 ||
                 pl.ntiles > INT32_MAX || pl.steps != 2 || pl.max_tile > kTileMax ||
-                pl.max_load > kLoadMax)
+                pl.max_load > kLoadMax || pl.max_step1 > kStepOneMax)
                 return set_error(NX_EINVAL, "planet tile plan is not a 2-step plan for V=%lld within the kernel limits",
                                  (long long)q.V);
             d.plan = plan_dev(pl);

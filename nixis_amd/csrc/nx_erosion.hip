@@ -75,6 +75,7 @@ __global__ __launch_bounds__(256) void k_erode_step(const ErosionMeta *__restric
 }
 
 // Two steps per launch, one block per tile of the plan (nx_pair_dev.h).
+// B holds step one's results, at most kStepOneMax (tile + ring 1) vertices.
 template <int THREADS, bool KEEP, int RULE = 3>
 __global__ __launch_bounds__(THREADS) void k_erode_pair(const ErosionMeta *__restrict__ meta,
                                                        double2 *__restrict__ hw0,
@@ -84,7 +85,7 @@ __global__ __launch_bounds__(THREADS) void k_erode_pair(const ErosionMeta *__res
                                                        int j, double rain_mid, double rain_next)
 {
     __shared__ double2 A[kLoadMax];
-    __shared__ double2 B[THREADS == 1024 ? kLoadMax : 1];
+    __shared__ double2 B[THREADS == 1024 ? kStepOneMax : 1];
     const int64_t tile = xcd_tile(blockIdx.x, P.ntiles);
     if (tile >= P.ntiles) return;
     erode_tile_pair<THREADS, KEEP, RULE>(A, THREADS == 1024 ? B : nullptr, hw0, hw1, s0, s1,
@@ -342,7 +343,8 @@ static uint64_t plan_tag(const nx_tile_plan *plan, int variant, int rule)
         const int64_t f[] = {plan->bytes,   plan->V,        plan->ntiles,    plan->max_tile,
                              plan->max_load, plan->off_start, plan->off_cs,   plan->off_load,
                              plan->off_rbase, plan->off_ring, plan->off_cbase, plan->off_lcol,
-                             plan->off_lel,  plan->steps,    plan->max_step1, plan->off_pref};
+                             plan->off_lel,  plan->steps,    plan->max_step1, plan->off_pref,
+                             plan->off_ebase};
         for (int64_t v : f) mix((uint64_t)v);
     }
     return h;
@@ -401,6 +403,9 @@ static int check_plan(const nx_tile_plan *plan, int64_t V)
     if (plan->max_tile > kTileMax || plan->max_load > lmax)
         return set_error(NX_EINVAL, "tile plan tiles exceed %d owned / %d loaded vertices",
                          kTileMax, lmax);
+    if (plan->steps == 2 && plan->max_step1 > kStepOneMax)
+        return set_error(NX_EINVAL, "tile plan tiles hold %d first-step vertices (at most %d)",
+                         plan->max_step1, kStepOneMax);
     return NX_OK;
 }
 

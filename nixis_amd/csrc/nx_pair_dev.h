@@ -37,9 +37,10 @@ struct TilePlanDev {
     const int64_t *rbase;    // [ntiles + 1] into ring
     const int32_t *ring;     // global indices: ring 1 then ring 2, per tile
     const int64_t *cbase;    // [ntiles + 1] prefix of cs
-    const uint16_t *lcol;    // local neighbour index, [6][cs] per tile
-    const double *lel;       // edge length, [6][cs] per tile
+    const uint16_t *lcol;    // slot codes (slot_nbr, slot_len), [6][cs] per tile
+    const double *lel;       // edge lengths, [columns][cs] per tile from ebase
     const int32_t *pref;     // [ntiles][steps + 1]: tile, tile + ring 1, ...
+    const int64_t *ebase;    // [ntiles + 1] into lel
     int64_t ntiles;
     int32_t steps;
 };
@@ -57,9 +58,23 @@ inline TilePlanDev plan_dev(const nx_tile_plan &q)
     P.lcol = (const uint16_t *)(b + q.off_lcol);
     P.lel = (const double *)(b + q.off_lel);
     P.pref = (const int32_t *)(b + q.off_pref);
+    P.ebase = (const int64_t *)(b + q.off_ebase);
     P.ntiles = q.ntiles;
     P.steps = q.steps;
     return P;
+}
+
+// Slot codes of a tile plan (nx_tiles.cpp, tile_edges): bits 0-11 are the
+// neighbour's local index, bits 12-14 the column of the edge's length and
+// bit 15 tells whether vertex li itself (set) or the neighbour holds it; the
+// tile's lengths are a [columns][cs] block, so each undirected edge is read
+// from one place by both of its ends.
+NX_EHD int slot_nbr(int code) { return code & 4095; }
+template <class P>
+NX_EHD double slot_len(int code, int li, int cs, P len)
+{
+    const int owner = (code & 0x8000) ? li : (code & 4095);
+    return len[((code >> 12) & 7) * cs + owner];
 }
 
 #if defined(__HIPCC__)
@@ -89,7 +104,7 @@ __device__ __forceinline__ void erode_tile_pair(double2 *A, double2 *B, double2 
     const auto ring = gmem(P.ring) + gmem(P.rbase)[tile];
     const int64_t cb = gmem(P.cbase)[tile];
     const auto lcol = gmem(P.lcol) + 6 * cb;
-    const auto lel = gmem(P.lel) + 6 * cb;
+    const auto lel = gmem(P.lel) + gmem(P.ebase)[tile];
 
     for (int li = threadIdx.x; li < L; li += THREADS)
         A[li] = hw_src[li < T ? a + li : ring[li - T]];
@@ -110,8 +125,9 @@ __device__ __forceinline__ void erode_tile_pair(double2 *A, double2 *B, double2 
             if (li < cs && m < kOwnPer) {
 #pragma unroll
                 for (int q = 0; q < 6; ++q) {
-                    nbr[m][q] = lcol[q * cs + li];
-                    len[m][q] = lel[q * cs + li];
+                    const int code = lcol[q * cs + li];
+                    nbr[m][q] = slot_nbr(code);
+                    len[m][q] = slot_len(code, li, cs, lel);
                 }
             }
         }
@@ -129,8 +145,9 @@ __device__ __forceinline__ void erode_tile_pair(double2 *A, double2 *B, double2 
 #pragma unroll
             for (int q = 0; q < 6; ++q) {
                 const bool kept = KEEP && m < kOwnPer;
-                const int n = kept ? nbr[m < kKeep ? m : 0][q] : lcol[q * cs + li];
-                const double d = kept ? len[m < kKeep ? m : 0][q] : lel[q * cs + li];
+                const int code = kept ? 0 : lcol[q * cs + li];
+                const int n = kept ? nbr[m < kKeep ? m : 0][q] : slot_nbr(code);
+                const double d = kept ? len[m < kKeep ? m : 0][q] : slot_len(code, li, cs, lel);
                 const double2 nb = A[n];
                 erode_accumulate<RULE>(me.x, nb.x, nb.y, d, S, W);
             }
@@ -162,8 +179,9 @@ __device__ __forceinline__ void erode_tile_pair(double2 *A, double2 *B, double2 
             double S = sed[m], W = me.y;
 #pragma unroll
             for (int q = 0; q < 6; ++q) {
-                const int n = KEEP ? nbr[m < kKeep ? m : 0][q] : lcol[q * cs + li];
-                const double d = KEEP ? len[m < kKeep ? m : 0][q] : lel[q * cs + li];
+                const int code = KEEP ? 0 : lcol[q * cs + li];
+                const int n = KEEP ? nbr[m < kKeep ? m : 0][q] : slot_nbr(code);
+                const double d = KEEP ? len[m < kKeep ? m : 0][q] : slot_len(code, li, cs, lel);
                 const double2 nb = A2[n];
                 erode_accumulate<RULE>(me.x, nb.x, nb.y, d, S, W);
             }
@@ -203,7 +221,7 @@ __device__ __forceinline__ void erode_tile_multi(double2 *A, double2 *hw0, doubl
     const auto ring = gmem(P.ring) + gmem(P.rbase)[tile];
     const int64_t cb = gmem(P.cbase)[tile];
     const auto lcol = gmem(P.lcol) + 6 * cb;
-    const auto lel = gmem(P.lel) + 6 * cb;
+    const auto lel = gmem(P.lel) + gmem(P.ebase)[tile];
 
     for (int li = threadIdx.x; li < L; li += THREADS)
         A[li] = hw_src[li < T ? a + li : ring[li - T]];
@@ -236,8 +254,9 @@ __device__ __forceinline__ void erode_tile_multi(double2 *A, double2 *hw0, doubl
                         nb_i = nbr[KEEP ? m : 0][q];
                         d = len[KEEP ? m : 0][q];
                     } else {
-                        nb_i = lcol[q * cs + li];
-                        d = lel[q * cs + li];
+                        const int code = lcol[q * cs + li];
+                        nb_i = slot_nbr(code);
+                        d = slot_len(code, li, cs, lel);
                         if (KEEP) {
                             nbr[KEEP ? m : 0][q] = nb_i;
                             len[KEEP ? m : 0][q] = d;

@@ -19,10 +19,14 @@
 //   rbase  int64 [ntiles + 1]   offset of each tile's rings in `ring`
 //   ring   int32 [...]          global index of ring 1, ring 2, ... vertices
 //   cbase  int64 [ntiles + 1]   prefix sum of cs
-//   lcol   uint16 [6 * sum cs]  per tile, slot-major [6][cs]: local index of
-//                               each neighbour (tile: 0..T-1, then the rings)
-//   lel    double [6 * sum cs]  per tile, slot-major: the slot's edge length
+//   lcol   uint16 [6 * sum cs]  per tile, slot-major [6][cs]: the slot code
+//                               (tile_edges): neighbour's local index (tile:
+//                               0..T-1, then the rings), the column of its edge
+//                               length and whether this vertex owns it
+//   lel    double [sum K_t cs]  per tile, [K_t][cs]: each undirected edge
+//                               length once, in its owner's column
 //   pref   int32 [ntiles][K+1]  tile, tile + ring 1, ..., tile + rings 1..K
+//   ebase  int64 [ntiles + 1]   offset of each tile's lengths in `lel`
 #include <algorithm>
 #include <atomic>
 #include <cstring>
@@ -106,6 +110,137 @@ bool cut(const int32_t *col, int64_t V, int64_t a, int64_t len, int K, int limit
            cut(col, V, a + h, len - h, K, limit, step1_limit, out);
 }
 
+// Edge lengths of one tile, stored once per undirected edge.  Every length
+// the tile's step-1 vertices (local 0..c-1) read belongs to one of them: an
+// edge to a vertex outside that set to the vertex inside it, an edge between
+// two of them to one of the two, chosen so that no vertex owns more than
+// kt = 4 edges where the tile allows it (an orientation of bounded out-degree:
+// greedy, then augmenting paths move ownership away from vertices over kt;
+// kt grows where no path exists).  Owner o's j-th owned edge (in slot order)
+// has its length at [j][o] of the tile's [kt][c] block.  The slot code is the
+// neighbour's local index (bits 0-11), the length's column (bits 12-14) and
+// bit 15 set when the slot's own vertex is the owner.  A pad slot (the vertex
+// itself) is coded as owned, column 0: a tie adds nothing, whatever the
+// length.  lc [6][c]: local neighbour index of each slot; len [6][c]: its
+// length.  Returns kt; writes codes [6][c] and lens [kt][c] when given.
+int tile_edges(const int32_t *lc, const double *len, int c, uint16_t *codes, double *lens)
+{
+    struct Edge {
+        int32_t u, v;
+        int8_t ku, kv;
+        int32_t owner;
+    };
+    std::vector<Edge> E;
+    std::vector<int32_t> own((size_t)c, 0);
+    std::vector<std::vector<int32_t>> inc((size_t)c);   // internal edges at each vertex
+    // a slot whose edge its own vertex must hold: the neighbour is outside
+    // the step-1 set, or (a table that is not symmetric) has no slot back
+    std::vector<int8_t> forced((size_t)6 * c, 0);
+    auto back = [&](int32_t n, int32_t j) {
+        for (int q = 0; q < 6; ++q)
+            if (lc[(size_t)q * c + n] == j) return q;
+        return -1;
+    };
+    for (int32_t j = 0; j < c; ++j)
+        for (int k = 0; k < 6; ++k) {
+            const int32_t n = lc[(size_t)k * c + j];
+            if (n == j) continue;
+            if (n >= c || back(n, j) < 0) {
+                forced[(size_t)k * c + j] = 1;
+                ++own[(size_t)j];
+            }
+        }
+    for (int32_t j = 0; j < c; ++j)
+        for (int k = 0; k < 6; ++k) {
+            const int32_t n = lc[(size_t)k * c + j];
+            if (n == j || forced[(size_t)k * c + j] || j > n) continue;
+            const int kv = back(n, j);
+            if (forced[(size_t)kv * c + n]) continue;
+            const int32_t o = own[(size_t)j] <= own[(size_t)n] ? j : n;
+            ++own[(size_t)o];
+            inc[(size_t)j].push_back((int32_t)E.size());
+            inc[(size_t)n].push_back((int32_t)E.size());
+            E.push_back(Edge{j, n, (int8_t)k, (int8_t)kv, o});
+        }
+    int kt = 4;
+    std::vector<int32_t> prev((size_t)c);
+    std::vector<int32_t> queue;
+    for (int32_t s = 0; s < c; ++s) {
+        while (own[(size_t)s] > kt) {
+            // breadth-first search over owned edges (x owns e: e can move to the
+            // other end) for a vertex owning fewer than kt
+            std::fill(prev.begin(), prev.end(), -2);
+            prev[(size_t)s] = -1;
+            queue.assign(1, s);
+            int32_t found = -1;
+            for (size_t h = 0; h < queue.size() && found < 0; ++h) {
+                const int32_t x = queue[h];
+                for (int32_t e : inc[(size_t)x]) {
+                    if (E[(size_t)e].owner != x) continue;
+                    const int32_t y = E[(size_t)e].u == x ? E[(size_t)e].v : E[(size_t)e].u;
+                    if (prev[(size_t)y] != -2) continue;
+                    prev[(size_t)y] = e;
+                    if (own[(size_t)y] < kt) {
+                        found = y;
+                        break;
+                    }
+                    queue.push_back(y);
+                }
+            }
+            if (found < 0) {
+                ++kt;
+                continue;
+            }
+            for (int32_t y = found; y != s;) {
+                Edge &e = E[(size_t)prev[(size_t)y]];
+                const int32_t x = e.u == y ? e.v : e.u;
+                e.owner = y;
+                y = x;
+            }
+            --own[(size_t)s];
+            ++own[(size_t)found];
+        }
+    }
+    kt = 1;
+    for (int32_t j = 0; j < c; ++j) kt = std::max(kt, own[(size_t)j]);
+    if (!codes) return kt;
+    // columns in slot order of each owner
+    std::vector<int8_t> owns(forced);                  // slot (k, j) holds j's own edge
+    for (const Edge &e : E) {
+        if (e.owner == e.u)
+            owns[(size_t)e.ku * c + e.u] = 1;
+        else
+            owns[(size_t)e.kv * c + e.v] = 1;
+    }
+    std::vector<int8_t> colof((size_t)6 * c, 0);
+    std::fill(lens, lens + (size_t)kt * c, 0.0);
+    for (int32_t j = 0; j < c; ++j) {
+        int m = 0;
+        for (int k = 0; k < 6; ++k)
+            if (owns[(size_t)k * c + j]) {
+                colof[(size_t)k * c + j] = (int8_t)m;
+                lens[(size_t)m * c + j] = len[(size_t)k * c + j];
+                ++m;
+            }
+    }
+    for (int32_t j = 0; j < c; ++j)
+        for (int k = 0; k < 6; ++k) {
+            const size_t sl = (size_t)k * c + j;
+            const int32_t n = lc[sl];
+            uint32_t code;
+            if (n == j) {
+                code = (uint32_t)j | 0x8000u;
+            } else if (owns[sl]) {
+                code = (uint32_t)n | (uint32_t)colof[sl] << 12 | 0x8000u;
+            } else {
+                const int q = back(n, j);   // n owns the edge, in its slot back to j
+                code = (uint32_t)n | (uint32_t)colof[(size_t)q * c + n] << 12;
+            }
+            codes[sl] = (uint16_t)code;
+        }
+    return kt;
+}
+
 inline int64_t a256(int64_t x) { return (x + 255) & ~(int64_t)255; }
 
 }  // namespace
@@ -162,6 +297,57 @@ int tile_plan_build(const int32_t *col, const double *elen, int64_t V, int K, vo
     if (nring > (int64_t)(K / 2) * V)
         return set_error(NX_ERANGE, "tile rings hold %lld vertices for %lld owned: no locality",
                          (long long)nring, (long long)V);
+
+    // Local neighbour table [6][c] and the slots' lengths of tile i's step-1
+    // vertices (the tile, then rings 1..K-1).
+    std::atomic<int> bad{0};
+    auto local_table = [&](const Tile &t, std::vector<int32_t> &flat, std::vector<int32_t> &lc,
+                           std::vector<double> &ln) {
+        const int64_t a = t.a, b = t.a + t.T;
+        flat.clear();
+        for (auto &x : t.r) flat.insert(flat.end(), x.begin(), x.end());
+        auto local = [&](int32_t n) -> int32_t {
+            if (n >= a && n < b) return (int32_t)(n - a);
+            int32_t base = t.T;
+            for (auto &x : t.r) {
+                auto it = std::lower_bound(x.begin(), x.end(), n);
+                if (it != x.end() && *it == n) return base + (int32_t)(it - x.begin());
+                base += (int32_t)x.size();
+            }
+            return -1;
+        };
+        const int32_t c = t.T + t.ring_total() - (int32_t)t.r.back().size();
+        lc.resize((size_t)6 * c);
+        ln.resize((size_t)6 * c);
+        for (int32_t j = 0; j < c; ++j) {
+            const int64_t v = j < t.T ? a + j : flat[(size_t)(j - t.T)];
+            for (int k = 0; k < 6; ++k) {
+                const int32_t l = local(col[(int64_t)k * V + v]);
+                if (l < 0) bad.store(1);
+                lc[(size_t)k * c + j] = l < 0 ? j : l;
+                ln[(size_t)k * c + j] = elen[(int64_t)k * V + v];
+            }
+        }
+        return c;
+    };
+    // Sizing pass: each tile's length columns (tile_edges is deterministic,
+    // so the writing pass below recomputes the same orientation).
+    std::vector<int32_t> kts((size_t)nt, 0);
+    parallel_for(nt, [&](int64_t t0, int64_t t1) {
+        std::vector<int32_t> flat, lc;
+        std::vector<double> ln;
+        for (int64_t i = t0; i < t1; ++i) {
+            const int32_t c = local_table(*tiles[(size_t)i], flat, lc, ln);
+            kts[(size_t)i] = tile_edges(lc.data(), ln.data(), c, nullptr, nullptr);
+        }
+    }, 64);
+    if (bad.load()) return set_error(NX_EINVAL, "tile plan: a neighbour fell outside its tile's rings");
+    int64_t nlen = 0;
+    for (int64_t i = 0; i < nt; ++i) {
+        const Tile &t = *tiles[(size_t)i];
+        nlen += (int64_t)kts[(size_t)i] * (t.T + t.ring_total() - (int64_t)t.r.back().size());
+    }
+
     nx_tile_plan q;
     std::memset(&q, 0, sizeof(q));
     q.V = V;
@@ -177,8 +363,9 @@ int tile_plan_build(const int32_t *col, const double *elen, int64_t V, int K, vo
     q.off_ring = o;  o = a256(o + 4 * nring);
     q.off_cbase = o; o = a256(o + 8 * (nt + 1));
     q.off_lcol = o;  o = a256(o + 2 * 6 * ncs);
-    q.off_lel = o;   o = a256(o + 8 * 6 * ncs);
+    q.off_lel = o;   o = a256(o + 8 * nlen);
     q.off_pref = o;  o = a256(o + 4 * (K + 1) * nt);
+    q.off_ebase = o; o = a256(o + 8 * (nt + 1));
     q.steps = K;
     q.bytes = o;
     q.blob = plan->blob;
@@ -195,8 +382,10 @@ int tile_plan_build(const int32_t *col, const double *elen, int64_t V, int K, vo
     auto *lcol = (uint16_t *)(img + q.off_lcol);
     auto *lel = (double *)(img + q.off_lel);
     auto *pref = (int32_t *)(img + q.off_pref);
+    auto *ebase = (int64_t *)(img + q.off_ebase);
     rbase[0] = 0;
     cbase[0] = 0;
+    ebase[0] = 0;
     for (int64_t i = 0; i < nt; ++i) {
         const Tile &t = *tiles[(size_t)i];
         start[i] = (int32_t)t.a;
@@ -210,38 +399,17 @@ int tile_plan_build(const int32_t *col, const double *elen, int64_t V, int K, vo
         cs[i] = n - (int32_t)t.r.back().size();
         rbase[i + 1] = rbase[i] + t.ring_total();
         cbase[i + 1] = cbase[i] + cs[i];
+        ebase[i + 1] = ebase[i] + (int64_t)kts[(size_t)i] * cs[i];
     }
     start[nt] = (int32_t)V;
-    std::atomic<int> bad{0};
     parallel_for(nt, [&](int64_t t0, int64_t t1) {
+        std::vector<int32_t> flat, lc;
+        std::vector<double> ln;
         for (int64_t i = t0; i < t1; ++i) {
             const Tile &t = *tiles[(size_t)i];
-            const int64_t a = t.a, b = t.a + t.T;
-            std::vector<int32_t> flat;  // rings 1..K in order: local index T + position
-            flat.reserve((size_t)t.ring_total());
-            for (auto &x : t.r) flat.insert(flat.end(), x.begin(), x.end());
+            const int32_t c = local_table(t, flat, lc, ln);
             std::copy(flat.begin(), flat.end(), ring + rbase[i]);
-            auto local = [&](int32_t n) -> int32_t {
-                if (n >= a && n < b) return (int32_t)(n - a);
-                int32_t base = t.T;
-                for (auto &x : t.r) {
-                    auto it = std::lower_bound(x.begin(), x.end(), n);
-                    if (it != x.end() && *it == n) return base + (int32_t)(it - x.begin());
-                    base += (int32_t)x.size();
-                }
-                return -1;
-            };
-            const int32_t c = cs[i];
-            for (int32_t j = 0; j < c; ++j) {
-                const int64_t v = j < t.T ? a + j : flat[(size_t)(j - t.T)];
-                for (int k = 0; k < 6; ++k) {
-                    const int64_t e = 6 * cbase[i] + (int64_t)k * c + j;
-                    const int32_t l = local(col[(int64_t)k * V + v]);
-                    if (l < 0) bad.store(1);
-                    lcol[e] = (uint16_t)(l < 0 ? 0 : l);
-                    lel[e] = elen[(int64_t)k * V + v];
-                }
-            }
+            tile_edges(lc.data(), ln.data(), c, lcol + 6 * cbase[i], lel + ebase[i]);
         }
     }, 64);
     if (bad.load()) return set_error(NX_EINVAL, "tile plan: a neighbour fell outside its tile's rings");

@@ -52,21 +52,37 @@ def _plan(col, elen, steps=2):
     ring = arr(q.off_ring, np.int32, int(rbase[-1]))
     cbase = arr(q.off_cbase, np.int64, nt + 1)
     lcol = arr(q.off_lcol, np.uint16, 6 * int(cbase[-1]))
-    lel = arr(q.off_lel, np.float64, 6 * int(cbase[-1]))
+    ebase = arr(q.off_ebase, np.int64, nt + 1)
+    lel = arr(q.off_lel, np.float64, int(ebase[-1]))
     pref = arr(q.off_pref, np.int32, (steps + 1) * nt).reshape(nt, steps + 1)
-    return q, start, cs, load, rbase, ring, cbase, lcol, lel, pref
+    return q, start, cs, load, rbase, ring, cbase, lcol, (lel, ebase), pref
+
+
+def _decode(codes, lens, c):
+    """Slot codes [6][c] of one tile -> (local neighbour [6][c], the length
+    each slot reads [6][c]): bits 0-11 neighbour, 12-14 column, bit 15 set
+    when the slot's own vertex holds the length, else the neighbour does."""
+    codes = codes.astype(np.int64)
+    nbr = codes & 4095
+    colm = (codes >> 12) & 7
+    mine = (codes >> 15) & 1
+    li = np.broadcast_to(np.arange(c), codes.shape)
+    owner = np.where(mine == 1, li, nbr)
+    assert (owner < c).all()
+    return nbr, lens.reshape(-1, c)[colm, owner]
 
 
 def _tiles(col, elen, steps=2):
-    q, start, cs, load, rbase, ring, cbase, lcol, lel, pref = _plan(col, elen, steps)
+    q, start, cs, load, rbase, ring, cbase, lcol, (lel, ebase), pref = _plan(col, elen, steps)
     for t in range(q.ntiles):
         a, b = int(start[t]), int(start[t + 1])
         T = b - a
         rg = ring[rbase[t]:rbase[t + 1]]
         glob = np.concatenate([np.arange(a, b), rg])          # local -> global
         c = int(cs[t])
-        lc = lcol[6 * cbase[t]:6 * cbase[t + 1]].reshape(6, c)
-        le = lel[6 * cbase[t]:6 * cbase[t + 1]].reshape(6, c)
+        lens = lel[ebase[t]:ebase[t + 1]]
+        assert lens.size % c == 0 and 1 <= lens.size // c <= 6
+        lc, le = _decode(lcol[6 * cbase[t]:6 * cbase[t + 1]].reshape(6, c), lens, c)
         assert pref[t, 0] == T and pref[t, -2] == c and pref[t, -1] == int(load[t])
         yield q, a, b, T, c, int(load[t]), glob, lc, le, pref[t]
 
@@ -87,10 +103,48 @@ def test_plan_covers_and_rings_are_exact(d):
         r1 = sorted(set(col[:, a:b].ravel()) - inside)
         r2 = sorted(set(col[:, r1].ravel()) - inside - set(r1)) if r1 else []
         assert list(glob[T:c]) == r1 and list(glob[c:L]) == r2
-        # every local neighbour of tile + ring 1 names the global neighbour
+        # every local neighbour of tile + ring 1 names the global neighbour,
+        # and every slot but a pad (a pentagon's sixth slot, the vertex
+        # itself: a tie, whatever the length) reads its edge's length
         assert (glob[lc] == col[:, glob[:c]]).all()
-        assert (le == elen[:, glob[:c]]).all()
+        pad = lc == np.arange(c)[None, :]
+        assert (le == elen[:, glob[:c]])[~pad].all()
     assert nxt == V and q.max_step1 == step1
+
+
+@pytest.mark.parametrize("d,steps", [(40, 2), (100, 2), (40, 4)])
+def test_edge_lengths_are_stored_once(d, steps):
+    """Each undirected edge between two step-1 vertices of a tile has its
+    length in one place, read by both ends, and each vertex holds at most a
+    handful of columns: most tiles need four ([4][cs] doubles, 32 bytes per
+    vertex against 48 for six per-slot lengths)."""
+    pts, nb, o, col, elen = _ell(d)
+    q, start, cs, load, rbase, ring, cbase, lcol, (lel, ebase), pref = _plan(col, elen, steps)
+    ncols = []
+    for t in range(q.ntiles):
+        c = int(cs[t])
+        codes = lcol[6 * cbase[t]:6 * cbase[t + 1]].reshape(6, c).astype(np.int64)
+        k = (int(ebase[t + 1]) - int(ebase[t])) // c
+        ncols.append(k)
+        nbr, colm, mine = codes & 4095, (codes >> 12) & 7, (codes >> 15) & 1
+        assert (colm < k).all()
+        li = np.broadcast_to(np.arange(c), codes.shape)
+        owner = np.where(mine == 1, li, nbr)
+        where = colm * c + owner
+        for j in range(c):
+            for s in range(6):
+                n = int(nbr[s, j])
+                if n == j or n >= c:
+                    assert mine[s, j] == 1      # pads and edges leaving the set
+                    continue
+                back = [r for r in range(6) if nbr[r, n] == j]
+                assert len(back) == 1 and where[s, j] == where[back[0], n]
+        # a column entry is read by at most two slots (the edge's two ends)
+        used = np.bincount(where[(nbr != li)].ravel(), minlength=k * c)
+        assert used.max() <= 2
+    ncols = np.array(ncols)
+    print(f"\nd={d} steps={steps}: {q.ntiles} tiles, columns {np.bincount(ncols)}")
+    assert (ncols <= 6).all() and (ncols <= 4).mean() >= 0.9
 
 
 def _step_rows(n_rows, Hs, Ws, Ss, lc, d):
@@ -181,7 +235,8 @@ def test_four_step_rings_are_exact_layers(d):
             assert list(glob[pref[m]:pref[m + 1]]) == nxt_layer
             seen |= set(nxt_layer)
             layer = nxt_layer
-        assert (glob[lc] == col[:, glob[:c]]).all() and (le == elen[:, glob[:c]]).all()
+        pad = lc == np.arange(c)[None, :]
+        assert (glob[lc] == col[:, glob[:c]]).all() and (le == elen[:, glob[:c]])[~pad].all()
     assert nxt == V
 
 
