@@ -270,7 +270,9 @@ def bench_erosion(args):
     }
 
 
-CHAIN_BYTES_PER_VERTEX = 75  # five passes: reads 8+8+9+9+8, writes 9+8+8+8
+# min/max, summary and writing passes: reads 8+8+8, writes 8+1
+CHAIN_BYTES_PER_VERTEX = 33
+CHAIN5_BYTES_PER_VERTEX = 75  # the five-pass form: reads 8+8+9+9+8, writes 9+8+8+8
 
 
 def bench_post_chain(args, nz):
@@ -293,10 +295,26 @@ def bench_post_chain(args, nz):
         h.copy_(src)
         total += _events_ms(lambda: util.height_chain_device(h, -4000.0, 8850.0, 55.0,
                                                              workspace=ws))
-    _lib.Context.get(h.device).sync_check()
+    ctx = _lib.Context.get(h.device)
+    ctx.sync_check()
     ms = total / args.steps
     gbs = CHAIN_BYTES_PER_VERTEX * n / (ms * 1e-3) / 1e9
     log(f"[chain] {ms:.3f} ms per chain over {n} heights, {gbs:.0f} GB/s")
+    # the five-pass form for comparison (same results)
+    ctx.set_chain_passes(5)
+    try:
+        t5 = 0.0
+        for i in range(args.warmup + args.steps):
+            h.copy_(src)
+            t = _events_ms(lambda: util.height_chain_device(h, -4000.0, 8850.0, 55.0,
+                                                            workspace=ws))
+            t5 += t if i >= args.warmup else 0.0
+        ctx.sync_check()
+    finally:
+        ctx.set_chain_passes(2)
+    ms5 = t5 / args.steps
+    log(f"[chain] five-pass form: {ms5:.3f} ms, "
+        f"{CHAIN5_BYTES_PER_VERTEX * n / (ms5 * 1e-3) / 1e9:.0f} GB/s")
     # noise and chain as one call (min/max reduced in the noise kernel)
     from nixis_amd import terrain
     rough, strength = terrain.octave_scalars(nz["n_oct"], nz["f0"], 0.4, nz["lac"], nz["pers"],
@@ -316,6 +334,7 @@ def bench_post_chain(args, nz):
                         f"({n} vertices): rescale [-4000, 8850], ocean 55%, power 0.5 / 2.0, "
                         "shift, split rescale",
             "ms": round(ms, 4), "Gvertex_per_s": round(n / (ms * 1e-3) / 1e9, 2),
+            "five_pass_ms": round(ms5, 4),
             "fused_noise_chain_ms": round(fused_ms, 4), "noise_ms": round(nz["ms"], 4),
             "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),

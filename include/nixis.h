@@ -60,6 +60,9 @@ int nx_ctx_destroy(nx_ctx *ctx);
                                 default); a tuning knob, results identical   */
 #define NX_OPT_MULTI_KERNEL 3 /* four-step erosion kernel variant, 0..3 (0 =
                                * 1024 threads, neighbour data read per step) */
+#define NX_OPT_CHAIN_PASSES 4 /* height chain form: 2 (default: one summary
+                               * pass, one writing pass) or 5 (the pass per
+                               * step form); results identical               */
 int nx_ctx_set_option(nx_ctx *ctx, int option, int64_t value);
 /* Synchronise `stream`, then return and clear the context's sticky kernel
  * error (NX_ERANGE: a noise coordinate outside |x| < 2^30 after the octave

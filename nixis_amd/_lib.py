@@ -23,6 +23,7 @@ NX_MAX_OCTAVES = 32
 NX_OPT_GRAPH_CHUNK = 1
 NX_OPT_PAIR_KERNEL = 2
 NX_OPT_MULTI_KERNEL = 3
+NX_OPT_CHAIN_PASSES = 4
 
 
 class NixisError(RuntimeError):
@@ -232,6 +233,7 @@ class Context:
         self.graph_chunk = 32
         self.pair_kernel = 0
         self.multi_kernel = 0
+        self.chain_passes = 2
 
     @classmethod
     def get(cls, device=None):
@@ -265,6 +267,11 @@ class Context:
         """Two-step erosion kernel variant 0..6 (0: default); same results."""
         check(lib().nx_ctx_set_option(self.handle, NX_OPT_PAIR_KERNEL, int(variant)))
         self.pair_kernel = int(variant)
+
+    def set_chain_passes(self, passes):
+        """Height chain form: 2 (default) or 5 (a pass per step); same results."""
+        check(lib().nx_ctx_set_option(self.handle, NX_OPT_CHAIN_PASSES, int(passes)))
+        self.chain_passes = int(passes)
 
     @contextlib.contextmanager
     def options(self, graph_chunk=None, pair_kernel=None):

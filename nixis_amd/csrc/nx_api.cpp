@@ -160,6 +160,10 @@ int nx_ctx_set_option(nx_ctx *ctx, int option, int64_t value)
         if (value < 0 || value > 6) return set_error(NX_EINVAL, "pair kernel variant %lld outside 0..6", (long long)value);
         ctx->pair_variant = (int)value;
         return NX_OK;
+    case NX_OPT_CHAIN_PASSES:
+        if (value != 2 && value != 5) return set_error(NX_EINVAL, "height chain passes %lld: 2 or 5", (long long)value);
+        ctx->chain_passes = (int)value;
+        return NX_OK;
     default:
         return set_error(NX_EINVAL, "unknown option %d", option);
     }

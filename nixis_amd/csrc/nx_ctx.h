@@ -56,6 +56,7 @@ struct nx_ctx {
     int graph_chunk = 32;                  // erosion steps per captured graph; 0 = eager
     int pair_variant = 0;                  // two-step kernel variant (0 = default)
     int multi_variant = 0;                 // four-step kernel variant (0 = default)
+    int chain_passes = 2;                  // height chain: 2 (default) or the five-pass form
     void *chain_scratch = nullptr;         // this device's address of the static chain scratch
     std::map<GraphKey, hipGraphExec_t> graphs;
     std::mutex mu;

@@ -58,6 +58,8 @@ __device__ __forceinline__ void stream8(int64_t n, Load load, Body body)
 // (2j, 2j+1) stream with stream4 and block 0 thread 0 takes an odd last value.
 __device__ __forceinline__ bool tail_thread() { return blockIdx.x == 0 && threadIdx.x == 0; }
 
+constexpr int kMinmaxThreads = 256;
+
 __global__ void k_minmax_init(unsigned long long *keys)
 {
     if (threadIdx.x == 0) {
@@ -66,7 +68,7 @@ __global__ void k_minmax_init(unsigned long long *keys)
     }
 }
 
-__global__ __launch_bounds__(256) void k_minmax(const double *__restrict__ x, int64_t n,
+__global__ __launch_bounds__(kMinmaxThreads) void k_minmax(const double *__restrict__ x, int64_t n,
                                                 unsigned long long *__restrict__ keys, int vec)
 {
     unsigned long long lo = ~0ull, hi = 0ull;
@@ -133,9 +135,11 @@ static int vec2(const void *p) { return (uintptr_t)p % 2 == 0 ? 1 : 0; }
 // lane: every block ends in two atomics on the same keys, and with eight
 // blocks per CU those atomics at the end of the pass cost a large part of it
 // (40 M values: 80 against 54 us alone, 89 against 79 us in the height chain).
+// Blocks of 1024 threads were no faster.
 static unsigned mm_grid(nx_ctx *ctx, int64_t n)
 {
-    return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, (int64_t)ctx->num_cus));
+    return (unsigned)std::max<int64_t>(
+        1, std::min<int64_t>((n + kMinmaxThreads - 1) / kMinmaxThreads, (int64_t)ctx->num_cus));
 }
 
 int rescale(nx_ctx *ctx, const double *x, int64_t n, double lower, double upper, double *out,
@@ -150,7 +154,7 @@ int rescale(nx_ctx *ctx, const double *x, int64_t n, double lower, double upper,
     unsigned long long *keys = ctx->d_keys + 2 * (ctx->key_slot.fetch_add(1) % kKeySlots);
     const unsigned grid = (unsigned)std::min<int64_t>((n + 255) / 256, (int64_t)ctx->num_cus * 8);
     hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(64), 0, stream, keys);
-    hipLaunchKernelGGL(k_minmax, dim3(mm_grid(ctx, n)), dim3(256), 0, stream, x, n, keys, vec16(x));
+    hipLaunchKernelGGL(k_minmax, dim3(mm_grid(ctx, n)), dim3(kMinmaxThreads), 0, stream, x, n, keys, vec16(x));
     hipLaunchKernelGGL(k_rescale_map, dim3(grid), dim3(256), 0, stream, x, n, keys, lower, upper,
                        out, minmax, ctx->d_err, vec16(x) & vec16(out));
     NX_HIP(hipGetLastError());
@@ -169,8 +173,10 @@ __global__ __launch_bounds__(256) void k_rescale_ex(const double *x, int64_t n,
                                                     const unsigned long long *__restrict__ keys,
                                                     RescaleArgs a, double *out,
                                                     double *__restrict__ minmax,
-                                                    int *__restrict__ err, int vec)
+                                                    int *__restrict__ err, int vec,
+                                                    const int32_t *__restrict__ skip)
 {
+    if (skip && *skip) return;
     double x_min = dunkey(keys[0]), x_max = dunkey(keys[1]);
     if (blockIdx.x == 0 && threadIdx.x == 0 && minmax) {
         minmax[0] = x_min;
@@ -236,9 +242,9 @@ static int rescale_ex_launch(nx_ctx *ctx, const double *x, int64_t n, const Resc
     unsigned long long *keys = key_slot(ctx);
     const unsigned grid = red_grid(ctx, n);
     hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(64), 0, stream, keys);
-    hipLaunchKernelGGL(k_minmax, dim3(mm_grid(ctx, n)), dim3(256), 0, stream, x, n, keys, vec16(x));
+    hipLaunchKernelGGL(k_minmax, dim3(mm_grid(ctx, n)), dim3(kMinmaxThreads), 0, stream, x, n, keys, vec16(x));
     hipLaunchKernelGGL(k_rescale_ex, dim3(grid), dim3(256), 0, stream, x, n, keys, a, out,
-                       minmax, ctx->d_err, vec16(x) & vec16(out));
+                       minmax, ctx->d_err, vec16(x) & vec16(out), (const int32_t *)nullptr);
     NX_HIP(hipGetLastError());
     return NX_OK;
 }
@@ -326,11 +332,21 @@ struct PrHead {
     double level;       // ocean level (height chain)
     double pad;
     PSum s[2];          // whole-array summaries: [0] mask unset, [1] mask set
+    // Height chain in two passes (chain_derive, k_chain_final): the ocean and
+    // land power_rescale bounds (lower, range) and the min/max of the shifted
+    // heights, and whether they were derived (else the five-pass kernels run).
+    double lo1, r1, lo0, r0, h4min, h4max;
+    int32_t fast, pad2;
 };
 static_assert(sizeof(PrHead) <= 256, "PrHead must fit the 256-byte workspace head");
+// The flag's byte offset in the workspace is part of the tested behaviour
+// (tests read it to see which form ran).
+static_assert(offsetof(PrHead, fast) == 224, "PrHead::fast moved");
 
-constexpr int kPrThreads = 256;
-constexpr int kPrPer = 8;                       // consecutive values per lane
+constexpr int kPrThreads = 512;
+constexpr int kFinalThreads = 1024;
+constexpr int kPrPer = 8;                          // values per lane per wave chunk
+constexpr int kPrSlices = kPrPer / 2;              // slices of 128 values per chunk
 constexpr int64_t kPrUnit = (int64_t)64 * kPrPer;  // values per wave chunk
 
 __device__ __forceinline__ PSum psum_empty()
@@ -446,17 +462,243 @@ struct ChainArgs {
     double *level_out;             // device f64[1]
 };
 
+// util.py:224-248 on one selected value r, with the selected values' lower
+// bound and range.  x ** 2 and x ** 0.5 as the correctly rounded t * t and
+// sqrt(t), the values an accurate pow returns (the height chain's powers).
+__device__ __forceinline__ double pr_power_map(double r, double mask_lower, double mask_range,
+                                               double power)
+{
+    const double temp_upper = 1.0, temp_lower = 0.0, temp_range = temp_upper - temp_lower;
+    double t = ((r - mask_lower) / mask_range) * temp_range + temp_lower;
+    if (power == 2.0)
+        t = t * t;
+    else if (power == 0.5)
+        t = t == 0.0 ? 0.0 : sqrt(t);  // pow(-0, 0.5) is +0
+    else
+        t = pow(t, power);
+    return ((t - temp_lower) / temp_range) * mask_range + mask_lower;
+}
+
+// nixis.py:364 on one value: rescale(h, lower, upper, mid=0) (util.py:141-160
+// with mid), x_min / x_max the array's; zero reports a zero range used.
+__device__ __forceinline__ double split_rescale(double v, double x_min, double x_max,
+                                                double lower, double upper, bool &zero)
+{
+    const double mid = 0.0;
+    if (v <= mid) {
+        const double xr = mid - x_min;
+        zero = zero | (xr == 0.0);
+        return ((v - x_min) / xr) * (mid - lower) + lower;
+    }
+    const double xr = x_max - mid;
+    zero = zero | (xr == 0.0);
+    return ((v - mid) / xr) * (upper - mid) + mid;
+}
+
+// The height chain's constants after its summary pass (nixis.py:348-364).
+struct ChainConsts {
+    double lo1, r1, lo0, r0, h4min, h4max;  // ocean / land bounds; shifted min/max
+    int fast;                               // all derived (else the five-pass form)
+};
+
+// The constants from the whole-array summaries so (mask set, the ocean) and
+// sl (mask unset, the land) and the rescaled heights' min/max and level, in
+// one thread: the ocean bounds from the ocean summary and the min/max of the
+// rescaled heights; the min/max after the ocean map as the images of the
+// ocean values' extremes beside the land values' own; the land bounds from
+// those; the min/max after the land map and the shift likewise.  Each map
+// is monotone non-decreasing on its selected values when its range is
+// positive (x - lower >= 0, a positive divisor, sqrt or t * t of t >= 0, a
+// positive factor), so the image of an extreme is the extreme of the images,
+// bit for bit.  fast is set only when every range is positive and every
+// extreme finite and nonzero (the reductions order -0 below +0, which a
+// comparison does not).
+__device__ ChainConsts chain_derive(const PSum &so, const PSum &sl, double t_min, double t_max,
+                                    double level)
+{
+    bool ok = isfinite(t_min) && isfinite(t_max) && isfinite(level) && (so.ne || sl.ne);
+    bool any = false;
+    double mn = 0.0, mx = 0.0;
+    auto take = [&](double a, double b) {
+        ok = ok && isfinite(a) && isfinite(b) && a != 0.0 && b != 0.0;
+        mn = !any || a < mn ? a : mn;
+        mx = !any || b > mx ? b : mx;
+        any = true;
+    };
+    ChainConsts c;
+    double up1 = t_min;
+    c.lo1 = t_max;
+    psum_bounds(so, t_min, t_max, c.lo1, up1);  // as k_pr_map, mode 1
+    c.r1 = up1 - c.lo1;
+    double fo_mn = 0.0, fo_mx = 0.0;
+    if (so.ne) {
+        ok = ok && c.r1 > 0.0 && isfinite(c.r1);
+        fo_mn = pr_power_map(so.mn, c.lo1, c.r1, 0.5);
+        fo_mx = pr_power_map(so.mx, c.lo1, c.r1, 0.5);
+        take(fo_mn, fo_mx);
+    }
+    if (sl.ne) take(sl.mn, sl.mx);
+    double up0 = mn;
+    c.lo0 = mx;
+    psum_bounds(sl, mn, mx, c.lo0, up0);  // as k_pr_map, mode 0
+    c.r0 = up0 - c.lo0;
+    any = false;
+    if (so.ne) take(fo_mn, fo_mx);
+    if (sl.ne) {
+        ok = ok && c.r0 > 0.0 && isfinite(c.r0);
+        take(pr_power_map(sl.mn, c.lo0, c.r0, 2.0), pr_power_map(sl.mx, c.lo0, c.r0, 2.0));
+    }
+    c.h4min = mn - level;
+    c.h4max = mx - level;
+    ok = ok && isfinite(c.h4min) && isfinite(c.h4max) && c.h4min != 0.0 && c.h4max != 0.0;
+    c.fast = ok ? 1 : 0;
+    return c;
+}
+
+// In-order reduction of the nsum pairs of block summaries sum[2c], sum[2c+1]
+// by one block of NT threads; the results are valid in thread 0.
+template <int NT>
+__device__ void pr_reduce_sums(const PSum *sum, int64_t nsum, PSum (*lds)[NT / 64], PSum &s0,
+                               PSum &s1)
+{
+    const int64_t per = (nsum + NT - 1) / NT;
+    const int64_t c0 = (int64_t)threadIdx.x * per;
+    const int64_t c1 = c0 + per < nsum ? c0 + per : nsum;
+    s0 = psum_empty();
+    s1 = psum_empty();
+    for (int64_t c = c0; c < c1; ++c) {
+        s0 = psum_join(s0, sum[2 * c]);
+        s1 = psum_join(s1, sum[2 * c + 1]);
+    }
+    s0 = psum_block<NT / 64>(s0, lds[0]);
+    s1 = psum_block<NT / 64>(s1, lds[1]);
+}
+
+__device__ __forceinline__ double wave_fmin(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+__device__ __forceinline__ double wave_fmax(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+__device__ __forceinline__ double lane_double(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// One half's run within a wave, in sequence order, the same in every lane:
+// before its first break the run is strictly decreasing, so its minimum is
+// `last` and its maximum `first`; the values from the break on (the
+// breaking value included) are taken by the lanes' (mn, mx).
+struct WaveRun {
+    double first, last;  // last: the last value before the break
+    int ne, brk;
+};
+
+// Append one slice of 128 values to an unbroken run: lane l holds the
+// values at positions 2l (a, if fa) and 2l + 1 (b, if fb) of the slice, the
+// values of this half only.  Finds each value's predecessor (the last value
+// before it in the slice, else the run's last), the first value not below its
+// predecessor (the break), and gives the values from the break on to the
+// lane's (mn, mx).
+__device__ __forceinline__ void wave_run_slice(WaveRun &R, double a, double b, bool fa, bool fb,
+                                               double &mn, double &mx)
+{
+    const int lane = threadIdx.x & 63;
+    const unsigned long long hb = __ballot(fa || fb);
+    if (hb == 0) return;
+    // Inclusive scan: the last value at or before this lane.
+    double pv = fb ? b : a;
+    int ph = (fa || fb) ? 1 : 0;
+    for (int off = 1; off < 64; off <<= 1) {
+        const double t = __shfl_up(pv, off, 64);
+        const int th = __shfl_up(ph, off, 64);
+        if (lane >= off && !ph) {
+            pv = t;
+            ph = th;
+        }
+    }
+    // The predecessor of this lane's first value.
+    double ep = __shfl_up(pv, 1, 64);
+    int eh = __shfl_up(ph, 1, 64);
+    if (lane == 0) eh = 0;
+    if (!eh) {
+        ep = R.last;
+        eh = R.ne;
+    }
+    const bool ba = fa && eh && !(a < ep);
+    const double pb = fa ? a : ep;  // the predecessor of b
+    const bool bb = fb && (fa || eh) && !(b < pb);
+    if (!R.ne) R.first = lane_double(fa ? a : b, __ffsll((long long)hb) - 1);
+    R.ne = 1;
+    const unsigned long long bm = __ballot(ba || bb);
+    if (bm == 0) {
+        R.last = lane_double(pv, 63 - __clzll((long long)hb));
+        return;
+    }
+    const int L = __ffsll((long long)bm) - 1;
+    const int at_b = __builtin_amdgcn_readlane(ba ? 0 : 1, L);
+    R.last = lane_double(at_b ? pb : ep, L);
+    R.brk = 1;
+    const int P = 2 * L + at_b;  // the break's position
+    if (fa && 2 * lane >= P) {
+        mn = a < mn ? a : mn;
+        mx = a > mx ? a : mx;
+    }
+    if (fb && 2 * lane + 1 >= P) {
+        mn = b < mn ? b : mn;
+        mx = b > mx ? b : mx;
+    }
+}
+
+// The wave's summary of one half (valid in every lane).
+__device__ __forceinline__ PSum wave_run_summary(const WaveRun &R, double mn, double mx)
+{
+    PSum r = psum_empty();
+    if (!R.ne) return r;
+    r.ne = 1;
+    r.brk = R.brk;
+    r.first = R.first;
+    r.last = R.last;
+    r.mn = R.last;
+    r.mx = R.first;
+    if (R.brk) {
+        mn = wave_fmin(mn);
+        mx = wave_fmax(mx);
+        r.mn = mn < r.mn ? mn : r.mn;
+        r.mx = mx > r.mx ? mx : r.mx;
+        r.suf = mx;
+    }
+    return r;
+}
+
 // Summaries of the values whose mask is unset (sum[2b]) and set (sum[2b + 1])
 // over block b's values.  Wave w of block b takes the contiguous run of
-// wave chunks [(4b + w) * cpw, (4b + w + 1) * cpw), kPrPer consecutive values
-// per lane, and keeps its running summary in lane 0 (no barrier in the
-// loop); the four waves join in order at the end.  CHAIN: x holds the noise
-// heights; each value is rewritten as the standard rescale (util.py:139)
-// with min/max from head->keys0, the rescaled min/max and the ocean level
-// are derived from the same keys (the map is monotone, so its extremes are
-// the images of x's), and mask = t <= level (terrain.py:61-72) is written.
-// vec: x is 16-byte and mask 8-byte aligned (vector accesses).
-template <bool CHAIN>
+// wave chunks [(8b + w) * cpw, (8b + w + 1) * cpw), each as four slices of
+// 128 values, two per lane.  Each half's run is followed in sequence order
+// (WaveRun, wave_run_slice) until its first break (a value not below the
+// one before it); after the break only the minimum and maximum of that
+// half's later values can change the summary (a later value that is a new
+// running minimum is below the maximum already taken), so they are kept
+// per lane and reduced once at the end.  A chunk in which no unbroken half
+// has a value takes the minimum/maximum path alone: waves whose run lies
+// inside one region see a single half for all of it, and every wave must
+// stream at the same rate.  The eight waves join in order.
+// CHAIN: x holds the noise heights; each value is the standard rescale
+// (util.py:139) with min/max from head->keys0, the rescaled min/max and the
+// ocean level are derived from the same keys (the map is monotone, so its
+// extremes are the images of x's), and mask = t <= level (terrain.py:61-72);
+// WRITE: t and the mask are written back.  vec: x is 16-byte and mask
+// 2-byte aligned (vector accesses).
+template <bool CHAIN, bool WRITE>
 __global__ __launch_bounds__(kPrThreads) void k_pr_summary(double *x, uint8_t *mask, int64_t n,
                                                            int64_t nunits, int64_t cpw, int vec,
                                                            PSum *__restrict__ sum,
@@ -484,96 +726,134 @@ __global__ __launch_bounds__(kPrThreads) void k_pr_summary(double *x, uint8_t *m
         }
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    PSum r0 = psum_empty(), r1 = psum_empty();  // the wave's running summaries (lane 0)
+    WaveRun R0 = {0.0, 0.0, 0, 0}, R1 = {0.0, 0.0, 0, 0};  // land, ocean
+    double mn0 = INFINITY, mx0 = -INFINITY, mn1 = INFINITY, mx1 = -INFINITY;  // after the break
     const int64_t u_begin = ((int64_t)blockIdx.x * (kPrThreads / 64) + wave) * cpw;
     const int64_t u_end = u_begin + cpw < nunits ? u_begin + cpw : nunits;
-    // Values of wave chunk u (and their mask bits unless CHAIN); lanes past
-    // the end get cnt = 0.
-    auto load = [&](int64_t u, double *v, bool *m, int &cnt) {
-        const int64_t base = u * kPrUnit + (int64_t)lane * kPrPer;
-        const int64_t left = n - base;
-        cnt = left <= 0 ? 0 : (left >= kPrPer ? kPrPer : (int)left);
-        if (cnt == kPrPer && vec) {
-            const double2 *x2 = (const double2 *)(x + base);
+    // Wave chunk u is kPrSlices slices of 128 values; lane l holds values
+    // 2l and 2l + 1 of each slice (v[2k], v[2k + 1] of slice k), so every
+    // access instruction covers one contiguous range.  Value j of the lane is
+    // at p0 + off(j) and exists when off(j) < left = n - p0.
+    auto off = [](int j) { return (int64_t)(j >> 1) * 128 + (j & 1); };
+    auto load = [&](int64_t u, double *v, unsigned &mb, int64_t &left) {
+        const int64_t p0 = u * kPrUnit + 2 * lane;
+        left = n - p0;
+        mb = 0;
+        if (vec && off(kPrPer - 1) < left) {  // the lane's whole chunk
 #pragma unroll
-            for (int k = 0; k < kPrPer / 2; ++k) {
-                const double2 t = x2[k];
+            for (int k = 0; k < kPrSlices; ++k) {
+                const double2 t = *(const double2 *)(x + p0 + k * 128);
                 v[2 * k] = t.x;
                 v[2 * k + 1] = t.y;
-            }
-            if (!CHAIN) {
-                const uint64_t w = *(const uint64_t *)(mask + base);
-#pragma unroll
-                for (int k = 0; k < kPrPer; ++k) m[k] = ((w >> (8 * k)) & 0xff) != 0;
+                if (!CHAIN) {
+                    const unsigned w = *(const uint16_t *)(mask + p0 + k * 128);
+                    mb |= ((w & 0xffu) != 0 ? 1u : 0u) << (2 * k);
+                    mb |= ((w >> 8) != 0 ? 1u : 0u) << (2 * k + 1);
+                }
             }
         } else {
 #pragma unroll
-            for (int k = 0; k < kPrPer; ++k) {
-                v[k] = k < cnt ? x[base + k] : 0.0;
-                if (!CHAIN) m[k] = k < cnt ? mask[base + k] != 0 : false;
+            for (int j = 0; j < kPrPer; ++j) {
+                const bool ok = off(j) < left;
+                v[j] = ok ? x[p0 + off(j)] : 0.0;
+                if (!CHAIN && ok && mask[p0 + off(j)] != 0) mb |= 1u << j;
             }
         }
     };
     double vn[kPrPer];
-    bool mn[kPrPer] = {};
-    int cn = 0;
-    if (u_begin < u_end) load(u_begin, vn, mn, cn);
+    unsigned mbn = 0;
+    int64_t leftn = 0;
+    if (u_begin < u_end) load(u_begin, vn, mbn, leftn);
     for (int64_t u = u_begin; u < u_end; ++u) {
-        const int64_t base = u * kPrUnit + (int64_t)lane * kPrPer;
+        const int64_t p0 = u * kPrUnit + 2 * lane;
         double v[kPrPer];
-        bool m[kPrPer];
-        const int cnt = cn;
+        unsigned mb = mbn;  // bit j: value j's mask is set
+        const int64_t left = leftn;
 #pragma unroll
-        for (int k = 0; k < kPrPer; ++k) {
-            v[k] = vn[k];
-            m[k] = mn[k];
-        }
-        if (u + 1 < u_end) load(u + 1, vn, mn, cn);  // in flight while u is processed
+        for (int k = 0; k < kPrPer; ++k) v[k] = vn[k];
+        if (u + 1 < u_end) load(u + 1, vn, mbn, leftn);  // in flight while u is processed
         if (CHAIN) {
+            mb = 0;
 #pragma unroll
             for (int k = 0; k < kPrPer; ++k) {
                 v[k] = ((v[k] - x_min) / xr) * nr + ca.lower;
-                m[k] = v[k] <= level;
+                mb |= (v[k] <= level ? 1u : 0u) << k;
             }
-            if (cnt == kPrPer && vec) {
-                double2 *x2 = (double2 *)(x + base);
-                uint64_t w = 0;
+        }
+        if (CHAIN && WRITE) {
+            if (vec && off(kPrPer - 1) < left) {
 #pragma unroll
-                for (int k = 0; k < kPrPer / 2; ++k) x2[k] = make_double2(v[2 * k], v[2 * k + 1]);
-#pragma unroll
-                for (int k = 0; k < kPrPer; ++k) w |= (uint64_t)(m[k] ? 1 : 0) << (8 * k);
-                *(uint64_t *)(mask + base) = w;
+                for (int k = 0; k < kPrSlices; ++k) {
+                    *(double2 *)(x + p0 + k * 128) = make_double2(v[2 * k], v[2 * k + 1]);
+                    *(uint16_t *)(mask + p0 + k * 128) =
+                        (uint16_t)(((mb >> (2 * k)) & 1u) | (((mb >> (2 * k + 1)) & 1u) << 8));
+                }
             } else {
 #pragma unroll
-                for (int k = 0; k < kPrPer; ++k)
-                    if (k < cnt) {
-                        x[base + k] = v[k];
-                        mask[base + k] = m[k] ? 1 : 0;
+                for (int j = 0; j < kPrPer; ++j)
+                    if (off(j) < left) {
+                        x[p0 + off(j)] = v[j];
+                        mask[p0 + off(j)] = (uint8_t)((mb >> j) & 1u);
                     }
             }
         }
-        PSum s0 = psum_empty(), s1 = psum_empty();
+        // bit j of okb: value j exists and is not NaN; ob / lb: of the
+        // ocean (mask set) / land half.
+        unsigned okb = 0;
 #pragma unroll
-        for (int k = 0; k < kPrPer; ++k)
-            if (k < cnt && v[k] == v[k]) {
-                if (m[k])
-                    psum_push(s1, v[k]);
-                else
-                    psum_push(s0, v[k]);
+        for (int k = 0; k < kPrPer; ++k) okb |= (off(k) < left && v[k] == v[k] ? 1u : 0u) << k;
+        const unsigned ob = okb & mb, lb = okb & ~mb;
+        // A half that has broken, or has no value in this chunk, takes its
+        // values' minimum and maximum only.
+        if ((R0.brk || !__any(lb != 0)) && (R1.brk || !__any(ob != 0))) {
+#pragma unroll
+            for (int k = 0; k < kPrPer; ++k) {
+                const bool o = (ob >> k) & 1u, l = (lb >> k) & 1u;
+                mn1 = o && v[k] < mn1 ? v[k] : mn1;
+                mx1 = o && v[k] > mx1 ? v[k] : mx1;
+                mn0 = l && v[k] < mn0 ? v[k] : mn0;
+                mx0 = l && v[k] > mx0 ? v[k] : mx0;
             }
-        s0 = psum_wave(s0);
-        s1 = psum_wave(s1);
-        if (lane == 0) {
-            r0 = psum_join(r0, s0);
-            r1 = psum_join(r1, s1);
+            continue;
+        }
+        // Otherwise slice by slice, in order (a rare path, kept small: the
+        // slice's two values are selected from v).
+#pragma unroll 1
+        for (int k = 0; k < kPrSlices; ++k) {
+            double a = v[0], b = v[1];
+#pragma unroll
+            for (int t = 1; t < kPrSlices; ++t) {
+                a = k == t ? v[2 * t] : a;
+                b = k == t ? v[2 * t + 1] : b;
+            }
+            const bool la = (lb >> (2 * k)) & 1u, lbb = (lb >> (2 * k + 1)) & 1u;
+            const bool oa = (ob >> (2 * k)) & 1u, obb = (ob >> (2 * k + 1)) & 1u;
+            if (R0.brk) {
+                mn0 = la && a < mn0 ? a : mn0;
+                mx0 = la && a > mx0 ? a : mx0;
+                mn0 = lbb && b < mn0 ? b : mn0;
+                mx0 = lbb && b > mx0 ? b : mx0;
+            } else {
+                wave_run_slice(R0, a, b, la, lbb, mn0, mx0);
+            }
+            if (R1.brk) {
+                mn1 = oa && a < mn1 ? a : mn1;
+                mx1 = oa && a > mx1 ? a : mx1;
+                mn1 = obb && b < mn1 ? b : mn1;
+                mx1 = obb && b > mx1 ? b : mx1;
+            } else {
+                wave_run_slice(R1, a, b, oa, obb, mn1, mx1);
+            }
         }
     }
+    const PSum w0 = wave_run_summary(R0, mn0, mx0), w1 = wave_run_summary(R1, mn1, mx1);
     if (lane == 0) {
-        lds[0][wave] = r0;
-        lds[1][wave] = r1;
+        lds[0][wave] = w0;
+        lds[1][wave] = w1;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
+        PSum r0 = lds[0][0], r1 = lds[1][0];
         for (int w = 1; w < kPrThreads / 64; ++w) {
             r0 = psum_join(r0, lds[0][w]);
             r1 = psum_join(r1, lds[1][w]);
@@ -584,28 +864,19 @@ __global__ __launch_bounds__(kPrThreads) void k_pr_summary(double *x, uint8_t *m
 }
 
 // One block: in-order reduction of the per-block summaries into head->s;
-// with keys, head->xmin/xmax = (min, max) of x from them.
+// from_keys: head->xmin/xmax = (min, max) of x from head->keys0.
 __global__ __launch_bounds__(1024) void k_pr_finish(const PSum *__restrict__ sum, int64_t nsum,
-                                                    PrHead *__restrict__ head,
-                                                    const unsigned long long *__restrict__ keys)
+                                                    PrHead *__restrict__ head, int from_keys)
 {
     __shared__ PSum lds[2][16];
-    const int64_t per = (nsum + blockDim.x - 1) / blockDim.x;
-    const int64_t c0 = (int64_t)threadIdx.x * per;
-    const int64_t c1 = c0 + per < nsum ? c0 + per : nsum;
-    PSum s0 = psum_empty(), s1 = psum_empty();
-    for (int64_t c = c0; c < c1; ++c) {
-        s0 = psum_join(s0, sum[2 * c]);
-        s1 = psum_join(s1, sum[2 * c + 1]);
-    }
-    s0 = psum_block<16>(s0, lds[0]);
-    s1 = psum_block<16>(s1, lds[1]);
+    PSum s0, s1;
+    pr_reduce_sums<1024>(sum, nsum, lds, s0, s1);
     if (threadIdx.x == 0) {
         head->s[0] = s0;
         head->s[1] = s1;
-        if (keys) {
-            head->xmin = dunkey(keys[0]);
-            head->xmax = dunkey(keys[1]);
+        if (from_keys) {
+            head->xmin = dunkey(head->keys0[0]);
+            head->xmax = dunkey(head->keys0[1]);
         }
     }
 }
@@ -617,36 +888,29 @@ __device__ __forceinline__ bool pr_sel(int mode, uint8_t m) { return mode == 1 ?
 // head->xmin/xmax.  shift: subtract head->level from every value afterwards
 // (nixis.py:362).  okeys (nullable): min/max keys of the values written.
 // x and out may alias.  vec: x and out 16-byte, mask 2-byte aligned.
+// skip_if_fast: return at once when head->fast (the two-pass chain wrote
+// the results).
 __global__ __launch_bounds__(256) void k_pr_map(const double *x, double *out,
                                                 const uint8_t *__restrict__ mask, int64_t n,
                                                 int mode, double power,
                                                 const PrHead *__restrict__ head,
                                                 const unsigned long long *__restrict__ keys,
                                                 int shift, unsigned long long *__restrict__ okeys,
-                                                int *__restrict__ err, int vec)
+                                                int *__restrict__ err, int vec, int skip_if_fast)
 {
+    if (skip_if_fast && head->fast) return;
     const double x_min = keys ? dunkey(keys[0]) : head->xmin;
     const double x_max = keys ? dunkey(keys[1]) : head->xmax;
     double mask_lower = x_max, mask_upper = x_min;
     if (mode == 0 || mode == 1) psum_bounds(head->s[mode], x_min, x_max, mask_lower, mask_upper);
     const double level = shift ? head->level : 0.0;
-    const double temp_upper = 1.0, temp_lower = 0.0, temp_range = temp_upper - temp_lower;
     const double mask_range = mask_upper - mask_lower;
     bool zero = false;
     unsigned long long lo = ~0ull, hi = 0ull;
     auto f = [&](double r, uint8_t m) {
         if (pr_sel(mode, m)) {
             zero = zero | (mask_range == 0.0);
-            double t = ((r - mask_lower) / mask_range) * temp_range + temp_lower;
-            // x ** 2 and x ** 0.5 as the correctly rounded t * t and sqrt(t),
-            // the values an accurate pow returns (the height chain's powers).
-            if (power == 2.0)
-                t = t * t;
-            else if (power == 0.5)
-                t = t == 0.0 ? 0.0 : sqrt(t);  // pow(-0, 0.5) is +0
-            else
-                t = pow(t, power);
-            r = ((t - temp_lower) / temp_range) * mask_range + mask_lower;
+            r = pr_power_map(r, mask_lower, mask_range, power);
         }
         if (shift) r = r - level;
         if (okeys) {
@@ -689,6 +953,87 @@ __global__ __launch_bounds__(256) void k_pr_map(const double *x, double *out,
     }
 }
 
+// The height chain's second pass (nixis.py:332-364 per value).  Each block
+// first reduces the summary pass's nsum pairs of block summaries and derives
+// the chain's constants (chain_derive) itself; block 0 also stores them in
+// head, with the summaries, for the five-pass kernels that follow.  Then per
+// value: the first rescale with min/max from head->keys0 and the mask
+// t <= level; when the constants hold, the ocean or land power_rescale map,
+// the shift by the level and the split rescale around 0 with the shifted
+// heights' min/max.  Heights and mask are written once.  Every operation is
+// the one the five-pass kernels apply to that value, in the same order.
+// vec: h 16-byte and mask 2-byte aligned.
+__global__ __launch_bounds__(kFinalThreads) void k_chain_final(double *h, uint8_t *__restrict__ mask,
+                                                               int64_t n, const PSum *__restrict__ sum,
+                                                               int64_t nsum, PrHead *__restrict__ head,
+                                                               double min_alt, double max_alt,
+                                                               int *__restrict__ err, int vec)
+{
+    __shared__ PSum lds[2][kFinalThreads / 64];
+    __shared__ ChainConsts cc;
+    {
+        PSum s0, s1;
+        pr_reduce_sums<kFinalThreads>(sum, nsum, lds, s0, s1);
+        if (threadIdx.x == 0) {
+            cc = chain_derive(s1, s0, head->xmin, head->xmax, head->level);
+            if (blockIdx.x == 0) {
+                head->s[0] = s0;
+                head->s[1] = s1;
+                head->lo1 = cc.lo1;
+                head->r1 = cc.r1;
+                head->lo0 = cc.lo0;
+                head->r0 = cc.r0;
+                head->h4min = cc.h4min;
+                head->h4max = cc.h4max;
+                head->fast = cc.fast;
+                // the five-pass kernels' reductions, should they run
+                head->keys1[0] = head->keys2[0] = ~0ull;
+                head->keys1[1] = head->keys2[1] = 0ull;
+            }
+        }
+        __syncthreads();
+    }
+    const double x_min = dunkey(head->keys0[0]), x_max = dunkey(head->keys0[1]);
+    const double xr = x_max - x_min, nr = max_alt - min_alt;
+    const double level = head->level;
+    const double lo1 = cc.lo1, r1 = cc.r1, lo0 = cc.lo0, r0 = cc.r0;
+    const double h4min = cc.h4min, h4max = cc.h4max;
+    const bool fast = cc.fast != 0;
+    bool zero = false;
+    auto f = [&](double v, uint8_t &m) {
+        const double t = ((v - x_min) / xr) * nr + min_alt;
+        m = t <= level ? 1 : 0;
+        if (!fast) return t;  // the five-pass kernels continue from here
+        double r = m ? pr_power_map(t, lo1, r1, 0.5) : pr_power_map(t, lo0, r0, 2.0);
+        r = r - level;
+        return split_rescale(r, h4min, h4max, min_alt, max_alt, zero);
+    };
+    if (vec) {
+        double2 *h2 = (double2 *)h;
+        uint16_t *m2 = (uint16_t *)mask;
+        stream4<double2>(n >> 1, [&](int64_t j) { return h2[j]; }, [&](int64_t j, double2 d) {
+            uint8_t ma, mb;
+            const double a = f(d.x, ma), b = f(d.y, mb);
+            h2[j] = make_double2(a, b);
+            m2[j] = (uint16_t)(ma | (mb << 8));
+        });
+        if ((n & 1) && tail_thread()) {
+            uint8_t m;
+            h[n - 1] = f(h[n - 1], m);
+            mask[n - 1] = m;
+        }
+    } else {
+        stream4<double>(n, [&](int64_t i) { return h[i]; }, [&](int64_t i, double v) {
+            uint8_t m;
+            h[i] = f(v, m);
+            mask[i] = m;
+        });
+    }
+    // chain_derive sets fast only when no range is zero; kept as a check.
+    if (__any(zero) && (threadIdx.x & 63) == 0)
+        __hip_atomic_fetch_or(err, NX_ERR_DIV0_BIT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 static size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // Summary blocks: at most kPrMaxBlocks; each wave takes a contiguous run of
@@ -699,12 +1044,32 @@ struct PrGrid {
     int64_t nunits, cpw, blocks;
 };
 
-static PrGrid pr_grid(nx_ctx *ctx, int64_t n)
+// Resident blocks per CU of one summary kernel (its register use sets it):
+// the grid is one round of resident blocks, every wave a contiguous run.
+template <bool CHAIN, bool WRITE>
+static int pr_blocks_per_cu()
+{
+    static std::atomic<int> cached{0};
+    int v = cached.load(std::memory_order_relaxed);
+    if (v == 0) {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &nb, reinterpret_cast<const void *>(&k_pr_summary<CHAIN, WRITE>), kPrThreads, 0) !=
+                hipSuccess ||
+            nb < 1)
+            nb = 1;
+        v = nb;
+        cached.store(v, std::memory_order_relaxed);
+    }
+    return v;
+}
+
+static PrGrid pr_grid(nx_ctx *ctx, int64_t n, int blocks_per_cu)
 {
     constexpr int64_t wpb = kPrThreads / 64;
     PrGrid g;
     g.nunits = (n + kPrUnit - 1) / kPrUnit;
-    const int64_t want = wpb * std::max<int64_t>(1, std::min<int64_t>(kPrMaxBlocks, (int64_t)ctx->num_cus * 8));
+    const int64_t want = wpb * std::max<int64_t>(1, std::min<int64_t>(kPrMaxBlocks, (int64_t)ctx->num_cus * blocks_per_cu));
     g.cpw = (g.nunits + want - 1) / want;
     const int64_t waves = (g.nunits + g.cpw - 1) / g.cpw;
     g.blocks = (waves + wpb - 1) / wpb;
@@ -718,7 +1083,7 @@ size_t power_rescale_workspace_bytes(int64_t n)
 
 static int pr_vec(const void *x, const void *mask)
 {
-    return ((uintptr_t)x % 16 == 0 && (uintptr_t)mask % 8 == 0) ? 1 : 0;
+    return ((uintptr_t)x % 16 == 0 && (uintptr_t)mask % 2 == 0) ? 1 : 0;
 }
 
 int power_rescale(nx_ctx *ctx, const double *x, const uint8_t *mask, int64_t n, int mode,
@@ -730,31 +1095,40 @@ int power_rescale(nx_ctx *ctx, const double *x, const uint8_t *mask, int64_t n, 
     NX_HIP(hipSetDevice(ctx->device));
     auto *head = (PrHead *)ws;
     auto *sum = (PSum *)((char *)ws + 256);
-    const PrGrid g = pr_grid(ctx, n);
+    const PrGrid g = pr_grid(ctx, n, pr_blocks_per_cu<false, false>());
     const unsigned grid = red_grid(ctx, n);
     hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(64), 0, stream, head->keys0);
-    hipLaunchKernelGGL(k_minmax, dim3(mm_grid(ctx, n)), dim3(256), 0, stream, x, n, head->keys0,
+    hipLaunchKernelGGL(k_minmax, dim3(mm_grid(ctx, n)), dim3(kMinmaxThreads), 0, stream, x, n, head->keys0,
                        vec16(x));
-    hipLaunchKernelGGL(k_pr_summary<false>, dim3((unsigned)g.blocks), dim3(kPrThreads), 0, stream,
-                       const_cast<double *>(x), const_cast<uint8_t *>(mask), n, g.nunits, g.cpw,
-                       pr_vec(x, mask), sum, head, ChainArgs{0.0, 0.0, 0.0, nullptr}, ctx->d_err);
-    hipLaunchKernelGGL(k_pr_finish, dim3(1), dim3(1024), 0, stream, sum, g.blocks, head,
-                       (const unsigned long long *)head->keys0);
+    hipLaunchKernelGGL((k_pr_summary<false, false>), dim3((unsigned)g.blocks), dim3(kPrThreads), 0,
+                       stream, const_cast<double *>(x), const_cast<uint8_t *>(mask), n, g.nunits,
+                       g.cpw, pr_vec(x, mask), sum, head, ChainArgs{0.0, 0.0, 0.0, nullptr},
+                       ctx->d_err);
+    hipLaunchKernelGGL(k_pr_finish, dim3(1), dim3(1024), 0, stream, sum, g.blocks, head, 1);
     hipLaunchKernelGGL(k_pr_map, dim3(grid), dim3(256), 0, stream, x, out, mask, n, mode, power,
                        head, (const unsigned long long *)nullptr, 0,
-                       (unsigned long long *)nullptr, ctx->d_err, vec16(x) & vec16(out) & vec2(mask));
+                       (unsigned long long *)nullptr, ctx->d_err, vec16(x) & vec16(out) & vec2(mask), 0);
     NX_HIP(hipGetLastError());
     return NX_OK;
 }
 
 // ---- nixis.py:330-364: the chain after noise ------------------------------------
 //
-// Five passes over the heights: min/max; rescale + mask + power_rescale
-// summaries of both halves (the land half's values are not changed by the
-// ocean half's power_rescale, so one pass summarises both); the ocean
-// power_rescale with the min/max of its result; the land power_rescale with
-// the shift by the ocean level and the min/max of its result; the final
-// split rescale around 0.
+// Two passes over the heights after their min/max: the first computes the
+// rescale and the mask of each value and the power_rescale summaries of both
+// halves without writing (the land half's values are not changed by the
+// ocean half's power_rescale, so one pass summarises both); the second pass
+// reduces the block summaries in every block, derives every later bound and
+// min/max from them (chain_derive), applies the whole chain to each value
+// and writes heights and mask.  When the derivation does not hold (a zero range,
+// a zero or non-finite extreme), the second pass writes the rescaled heights
+// and mask instead, and the kernels of the five-pass form, which otherwise
+// return at once, finish the chain: the ocean power_rescale with the min/max
+// of its result; the land power_rescale with the shift by the ocean level and
+// the min/max of its result; the final split rescale around 0.  The
+// five-pass form itself (NX_OPT_CHAIN_PASSES 5) writes in its summary pass
+// and reduces the summaries in a kernel of its own.  Both forms give the same
+// bits.
 
 size_t height_chain_workspace_bytes(int64_t n) { return power_rescale_workspace_bytes(n); }
 
@@ -766,28 +1140,43 @@ static int chain_after_keys(nx_ctx *ctx, double *h, int64_t n, double min_alt, d
     if (!(0.0 < ocean_percent && ocean_percent < 100.0)) ocean_percent = 50.0;  // util.py:561-564
     auto *head = (PrHead *)ws;
     auto *sum = (PSum *)((char *)ws + 256);
-    const PrGrid g = pr_grid(ctx, n);
+    const int two = ctx->chain_passes == 2 ? 1 : 0;
+    const PrGrid g = pr_grid(ctx, n, two ? pr_blocks_per_cu<true, false>() : pr_blocks_per_cu<true, true>());
     const unsigned grid = red_grid(ctx, n);
-    // nixis.py:332-346 in one pass
-    hipLaunchKernelGGL(k_pr_summary<true>, dim3((unsigned)g.blocks), dim3(kPrThreads), 0, stream,
-                       h, mask, n, g.nunits, g.cpw, pr_vec(h, mask), sum, head,
-                       ChainArgs{min_alt, max_alt, ocean_percent, level}, ctx->d_err);
-    hipLaunchKernelGGL(k_pr_finish, dim3(1), dim3(1024), 0, stream, sum, g.blocks, head,
-                       (const unsigned long long *)nullptr);
+    const ChainArgs ca{min_alt, max_alt, ocean_percent, level};
+    const int vsum = pr_vec(h, mask), vmap = vec16(h) & vec2(mask);
+    if (ctx->chain_passes == 2) {
+        // The summary pass, then the writing pass; the five-pass kernels
+        // after it return at once when head->fast, else continue from the
+        // rescaled heights and mask the writing pass left (its block 0
+        // stores head->s and head->fast and resets keys1 and keys2).
+        hipLaunchKernelGGL((k_pr_summary<true, false>), dim3((unsigned)g.blocks), dim3(kPrThreads),
+                           0, stream, h, mask, n, g.nunits, g.cpw, vsum, sum, head, ca, ctx->d_err);
+        const unsigned fgrid = (unsigned)std::max<int64_t>(
+            1, std::min<int64_t>((n + kFinalThreads - 1) / kFinalThreads, (int64_t)ctx->num_cus * 2));
+        hipLaunchKernelGGL(k_chain_final, dim3(fgrid), dim3(kFinalThreads), 0, stream, h, mask, n,
+                           (const PSum *)sum, g.blocks, head, min_alt, max_alt, ctx->d_err, vmap);
+    } else {
+        // nixis.py:332-346 in one pass
+        hipLaunchKernelGGL((k_pr_summary<true, true>), dim3((unsigned)g.blocks),
+                           dim3(kPrThreads), 0, stream, h, mask, n, g.nunits, g.cpw, vsum, sum, head,
+                           ca, ctx->d_err);
+        hipLaunchKernelGGL(k_pr_finish, dim3(1), dim3(1024), 0, stream, sum, g.blocks, head, 0);
+        hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(64), 0, stream, head->keys1);
+        hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(64), 0, stream, head->keys2);
+    }
     // nixis.py:352: the ocean floor, power 0.5
-    const int vmap = vec16(h) & vec2(mask);
-    hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(64), 0, stream, head->keys1);
     hipLaunchKernelGGL(k_pr_map, dim3(grid), dim3(256), 0, stream, h, h, mask, n, 1, 0.5, head,
-                       (const unsigned long long *)nullptr, 0, head->keys1, ctx->d_err, vmap);
+                       (const unsigned long long *)nullptr, 0, head->keys1, ctx->d_err, vmap, two);
     // nixis.py:355 and 362: the land, power 2.0, then the shift
-    hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(64), 0, stream, head->keys2);
     hipLaunchKernelGGL(k_pr_map, dim3(grid), dim3(256), 0, stream, h, h, mask, n, 0, 2.0, head,
-                       (const unsigned long long *)head->keys1, 1, head->keys2, ctx->d_err, vmap);
+                       (const unsigned long long *)head->keys1, 1, head->keys2, ctx->d_err, vmap,
+                       two);
     // nixis.py:364
     RescaleArgs mid_args{min_alt, max_alt, 0.0, 0.0, 0.0, 1, 0, 0, 0};
     hipLaunchKernelGGL(k_rescale_ex, dim3(grid), dim3(256), 0, stream, h, n,
                        (const unsigned long long *)head->keys2, mid_args, h, (double *)nullptr,
-                       ctx->d_err, vec16(h));
+                       ctx->d_err, vec16(h), two ? (const int32_t *)&head->fast : (const int32_t *)nullptr);
     NX_HIP(hipGetLastError());
     return NX_OK;
 }
@@ -809,7 +1198,7 @@ int height_chain(nx_ctx *ctx, double *h, int64_t n, double min_alt, double max_a
     NX_HIP(hipSetDevice(ctx->device));
     auto *head = (PrHead *)ws;
     hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(64), 0, stream, head->keys0);
-    hipLaunchKernelGGL(k_minmax, dim3(mm_grid(ctx, n)), dim3(256), 0, stream, h, n, head->keys0,
+    hipLaunchKernelGGL(k_minmax, dim3(mm_grid(ctx, n)), dim3(kMinmaxThreads), 0, stream, h, n, head->keys0,
                        vec16(h));
     return chain_after_keys(ctx, h, n, min_alt, max_alt, ocean_percent, mask, level, ws, stream);
 }

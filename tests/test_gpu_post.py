@@ -172,6 +172,71 @@ def test_height_chain_matches_oracle(oracle):
     assert got.min() == -4000.0 and got.max() == 8850.0
 
 
+# byte offset of the two-pass form's flag in the chain workspace (PrHead::fast)
+_FAST_AT = 224
+
+
+def _chain_both_forms(x, percent=55.0, offset=0):
+    """Run the chain in the two-pass and the five-pass form on copies of x
+    placed `offset` elements into their buffers; returns per form the heights,
+    mask, level, raised error (or None) and the two-pass flag."""
+    out = []
+    ctx = _lib.Context.get(0)
+    try:
+        for passes in (2, 5):
+            ctx.set_chain_passes(passes)
+            buf = torch.zeros(x.size + offset, dtype=torch.float64, device="cuda")
+            h = buf[offset:]
+            h.copy_(_dev(x))
+            need = _lib.lib().nx_height_chain_workspace_bytes(x.size)
+            ws = torch.zeros(need, dtype=torch.uint8, device="cuda")
+            hh, m, lv = util.height_chain_device(h, -4000.0, 8850.0, percent, workspace=ws)
+            err = None
+            try:
+                ctx.sync_check()
+            except ZeroDivisionError as e:
+                err = type(e)
+            fast = int(ws[_FAST_AT:_FAST_AT + 4].cpu().numpy().view(np.int32)[0])
+            out.append((hh.cpu().numpy(), m.cpu().numpy(), lv.item(), err, fast))
+    finally:
+        ctx.set_chain_passes(2)
+    return out
+
+
+def _same_bits(a, b):
+    return np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+@pytest.mark.parametrize("case", ["noise", "normal_odd_unaligned", "quantized", "few_values",
+                                  "two_values"])
+def test_height_chain_two_pass_equals_five_pass(case):
+    """The two-pass form (a summary pass, the bounds derived from the
+    summaries, one writing pass) gives the five-pass form's heights, mask,
+    level and errors bit for bit; it runs whenever its conditions hold, and
+    leaves degenerate inputs (a zero power_rescale range) to the five-pass
+    kernels."""
+    rng = np.random.default_rng(5)
+    offset, want_fast = 0, 1
+    if case == "noise":
+        x = _noise_heights(100)
+    elif case == "normal_odd_unaligned":
+        x, offset = rng.standard_normal(1_000_001), 1
+    elif case == "quantized":           # many equal values, some at the level
+        x = rng.integers(0, 101, 300_000).astype(np.float64)
+    elif case == "few_values":
+        x = rng.integers(0, 4, 100_000).astype(np.float64)
+    else:                               # the ocean is one value: range 0
+        x, want_fast = np.tile([0.0, 1.0], 50_000), 0
+    print("chain forms", case, x.size, offset, flush=True)
+    (h2, m2, l2, e2, f2), (h5, m5, l5, e5, _) = _chain_both_forms(x, offset=offset)
+    assert f2 == want_fast
+    assert _same_bits(h2, h5) and np.array_equal(m2, m5) and l2 == l5 and e2 is e5
+    if want_fast:
+        assert e2 is None and h2.min() == -4000.0 and h2.max() == 8850.0
+    else:
+        assert e2 is ZeroDivisionError
+
+
 def test_height_chain_full_size_properties():
     """At 10,000,002 vertices (d=1000): the final range is exactly
     [min_alt, max_alt], the ocean is exactly the vertices at or below 0,
