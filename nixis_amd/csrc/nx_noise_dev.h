@@ -50,11 +50,18 @@ namespace nx {
 //   case 4: octahedron, both closest on the (1,1,1) side (mask = a & b)
 //   case 5: octahedron, both on the (0,0,0) side (mask = a | b)
 //   case 6: octahedron, one on each side (mask = far * 8 + near)
+// An entry is 64 bytes: the kernels read the first three 16-byte words
+// (delta constants and lattice offsets, each offset a whole word so it adds
+// to a coordinate without being unpacked); the packed code is read only for
+// selection records.
 struct alignas(16) ExtEntry {
-    int32_t code;
     float c0x, c0y, c0z;
-    int32_t pad;
+    int32_t ox0;
     float c1x, c1y, c1z;
+    int32_t ox1;
+    int32_t oy0, oz0, oy1, oz1;
+    int32_t code;
+    int32_t pad[3];
 };
 // The case and mask are functions of five comparison outcomes per region,
 // so the table is indexed by those bits (6-bit index):
@@ -129,6 +136,12 @@ constexpr ExtEntry ext_entry(int k, int m)
     ExtEntry r{};
     r.code = (e0[0] + 1) | (e0[1] + 1) << 2 | (e0[2] + 1) << 4 | (e1[0] + 1) << 6 |
              (e1[1] + 1) << 8 | (e1[2] + 1) << 10;
+    r.ox0 = e0[0];
+    r.oy0 = e0[1];
+    r.oz0 = e0[2];
+    r.ox1 = e1[0];
+    r.oy1 = e1[1];
+    r.oz1 = e1[2];
     // o + (ox + oy + oz) / 3, scaled by 1 / sqrt(2) like the deltas
     const double R = 0.70710678118654752;
     const double t0 = (e0[0] + e0[1] + e0[2]) / 3.0;
@@ -339,9 +352,28 @@ typedef uint64_t LaneMask;
 #if defined(__HIP_DEVICE_COMPILE__)
 __device__ __forceinline__ LaneMask lanes(bool b) { return __builtin_amdgcn_ballot_w64(b); }
 __device__ __forceinline__ bool lane_bit(LaneMask m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
+// 2 v + (this lane's bit of m), as one add with m as the carry-in (a
+// full-rate integer add, where a select per bit and the ORs joining the bits
+// issue at about half rate).
+__device__ __forceinline__ int shift_in(int v, LaneMask m)
+{
+    int r;
+    LaneMask c;
+    asm("v_addc_co_u32 %0, %1, %2, %2, %3" : "=v"(r), "=s"(c) : "v"(v), "s"(m));
+    return r;
+}
+__device__ __forceinline__ int first_bit(LaneMask m)
+{
+    int r;
+    LaneMask c;
+    asm("v_addc_co_u32 %0, %1, 0, 0, %2" : "=v"(r), "=s"(c) : "s"(m));
+    return r;
+}
 #else
 inline LaneMask lanes(bool b) { return b ? ~0ull : 0ull; }
 inline bool lane_bit(LaneMask m) { return m != 0; }
+inline int shift_in(int v, LaneMask m) { return 2 * v + (m != 0 ? 1 : 0); }
+inline int first_bit(LaneMask m) { return m != 0 ? 1 : 0; }
 #endif
 
 // Region of the in-cell coordinates (in_sum <= 1: tetrahedron at 000, >= 2:
@@ -397,8 +429,7 @@ NX_ND RegionSel select_region(double xins, double yins, double zins, double in_s
     const LaneMask i3 = (octa & repA) | (r1 & ltB) | (r0 & gtA);
     const LaneMask i4 = (octa & repB) | r1;
     RegionSel r;
-    r.idx = (lane_bit(i0) ? 1 : 0) + (lane_bit(i1) ? 2 : 0) + (lane_bit(i2) ? 4 : 0) +
-            (lane_bit(i3) ? 8 : 0) + (lane_bit(i4) ? 16 : 0) + (lane_bit(octa) ? 32 : 0);
+    r.idx = shift_in(shift_in(shift_in(shift_in(shift_in(first_bit(octa), i4), i3), i2), i1), i0);
     r.inA = lane_bit(inA);
     r.r1 = lane_bit(r1);
     r.octa = lane_bit(octa);
@@ -413,7 +444,6 @@ NX_ND RegionSel select_region(double xins, double yins, double zins, double in_s
 template <bool SEL>
 NX_ND float noise3_sum(double xs, double ys, double zs, const NoiseTables &T, NoiseSel *sel)
 {
-    const double SQUISH = 1.0 / 3;
     // floor(xs) is the integer the reference's int(floor(xs)) holds, as a
     // double (exact inside the checked domain |xs| < 2^30), so it serves the
     // FP64 arithmetic directly and the int only the hashing.
@@ -424,10 +454,8 @@ NX_ND float noise3_sum(double xs, double ys, double zs, const NoiseTables &T, No
     // The deltas to the cell origin, x - (xsb + sq) with sq = (xsb + ysb +
     // zsb) / 3 (opensimplex.py:283-299), only feed the FP32 section.  Since
     // x = xs - so and so = -(x + y + z) / 6, in_sum = -3 (so + sq), so the
-    // delta is xins + in_sum / 3: the same value up to FP64 rounding, from
-    // one shared product.
-    const double u = in_sum * SQUISH;
-    const double dx0d = xins + u, dy0d = yins + u, dz0d = zins + u;
+    // delta is xins + in_sum / 3: the same value up to rounding, computed
+    // below in FP32 from the FP64 in-cell coordinates.
 
     const RegionSel rs = select_region(xins, yins, zins, in_sum);
     const bool inA = rs.inA, r1 = rs.r1;
@@ -435,9 +463,11 @@ NX_ND float noise3_sum(double xs, double ys, double zs, const NoiseTables &T, No
 
     // ---- FP32 section -----------------------------------------------------
     constexpr float R = 0.70710678118654752f;   // 1 / sqrt(2)
-    const float d0x = (float)dx0d * R, d0y = (float)dy0d * R, d0z = (float)dz0d * R;
     // every constant offset below is scaled by R as well (S is SQUISH * R)
     constexpr float S = (float)(1.0 / 3.0 * 0.70710678118654752);
+    const float sr = (float)in_sum * S;
+    const float d0x = fmaf((float)xins, R, sr), d0y = fmaf((float)yins, R, sr),
+                d0z = fmaf((float)zins, R, sr);
     // (111's offset is 1 + 3 * SQUISH = 2 in every axis)
     constexpr float ONE = 0.70710678118654752f, TWO = (float)(2.0 * 0.70710678118654752);
     constexpr float ONE_S = (float)((1.0 + 1.0 / 3.0) * 0.70710678118654752);
@@ -480,13 +510,10 @@ NX_ND float noise3_sum(double xs, double ys, double zs, const NoiseTables &T, No
                 fmaf(inAf, S2, bz), gB, T, 1.0f);
     v = contrib(v, bx - ONE, by, bz - ONE, g101, T, twoBits);
     v = contrib(v, bx, by - ONE, bz - ONE, g011, T, twoBits);
-    // Extra vertices from the table (a 32-byte entry, two 16-byte LDS reads).
+    // Extra vertices from the table (three 16-byte LDS reads of the entry).
     const ExtEntry &E = T.ext[rs.idx];
-    const int code = E.code;
-    const int ge0 = hash_full(T, xm + (code & 3) - 1, ym + ((code >> 2) & 3) - 1,
-                              zm + ((code >> 4) & 3) - 1);
-    const int ge1 = hash_full(T, xm + ((code >> 6) & 3) - 1, ym + ((code >> 8) & 3) - 1,
-                              zm + ((code >> 10) & 3) - 1);
+    const int ge0 = hash_full(T, xm + E.ox0, ym + E.oy0, zm + E.oz0);
+    const int ge1 = hash_full(T, xm + E.ox1, ym + E.oy1, zm + E.oz1);
     v = contrib(v, d0x - E.c0x, d0y - E.c0y, d0z - E.c0z, ge0, T, 1.0f);
     v = contrib(v, d0x - E.c1x, d0y - E.c1y, d0z - E.c1z, ge1, T, 1.0f);
 
@@ -518,6 +545,7 @@ NX_ND float noise3_sum(double xs, double ys, double zs, const NoiseTables &T, No
         sel->v[0] = xsb;
         sel->v[1] = ysb;
         sel->v[2] = zsb;
+        const int code = E.code;
         sel->v[3] = region | (code & 63) << 2 | ((code >> 6) & 63) << 8;
         sel->v[4] = (int)cs;
     }
