@@ -330,6 +330,43 @@ def side_stream(device=None):
     return s
 
 
+_masked = {}
+
+
+def masked_stream(n_cus, device=None, stride=1):
+    """A stream on `device` whose kernels run on `n_cus` of its CUs (CU mask
+    bits 0, stride, 2 stride, ...), non-blocking, default priority, created
+    once per (device, n_cus, stride) and kept for the life of the process."""
+    import torch
+    if device is None:
+        device = torch.cuda.current_device()
+    if isinstance(device, torch.device):
+        device = device.index if device.index is not None else torch.cuda.current_device()
+    lib()
+    key = (device, int(n_cus), int(stride))
+    with _ctx_lock:
+        s = _masked.get(key)
+        if s is None:
+            total = torch.cuda.get_device_properties(device).multi_processor_count
+            bits = [i * stride for i in range(int(n_cus)) if i * stride < total]
+            words = (total + 31) // 32
+            mask = (ctypes.c_uint32 * words)()
+            for b in bits:
+                mask[b // 32] |= 1 << (b % 32)
+            hip = ctypes.CDLL("libamdhip64.so.7")
+            fn = hip.hipExtStreamCreateWithCUMask
+            fn.restype = _c_int
+            fn.argtypes = [ctypes.POINTER(_c_void_p), ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
+            h = _c_void_p()
+            with torch.cuda.device(device):
+                rc = fn(ctypes.byref(h), words, mask)
+            if rc != 0 or not h.value:
+                raise NixisError(NX_EHIP, f"hipExtStreamCreateWithCUMask failed ({rc})")
+            s = torch.cuda.ExternalStream(h.value, device=torch.device("cuda", device))
+            _masked[key] = s
+    return s
+
+
 def ptr(t):
     """Device (or host) address of a tensor / ndarray, None for None."""
     if t is None:

@@ -12,7 +12,8 @@ served, until enough of the budget is released by finished jobs.
 batched noise pass (on the noise stream) for the jobs admitted since the
 last tick and one batched chunk of erosion steps (on the erosion stream),
 so jobs that arrive while others are on the GPU join the next tick
-(continuous batching) and noise-only jobs do not wait behind erosion.  An
+(continuous batching) and noise-only jobs do not wait behind erosion; while
+jobs erode, noise passes may use only part of the CUs (noise_cu_share).  An
 erosion tick hands out a budget of vertex-steps to the eroding jobs oldest
 first, each job taking as many of its remaining steps as the budget left
 covers (at least two): a small planet finishes all its steps in one tick
@@ -145,13 +146,29 @@ EROSION_RATE = 8.0e10
 
 class PlanetServer:
     def __init__(self, hbm_budget=64 << 30, steps_per_tick=None, tuning=None, device=None,
-                 tick_vertex_steps=400_000_000, noise_share=None, young_share=0.0):
+                 tick_vertex_steps=400_000_000, noise_share=None, young_share=0.0,
+                 max_queued=3, noise_cu_share=0.5):
         import torch
         self.device = device or torch.device("cuda", torch.cuda.current_device())
         self.stream = torch.cuda.current_stream(self.device)
         self.erode_stream = _lib.side_stream(self.device)
+        # While jobs erode, noise passes run on a stream whose kernels may use
+        # only noise_cu_share of the CUs (a CU mask; None or 1 for all): a
+        # kernel trace of the 120-job trace showed the erosion launches that
+        # overlap a noise pass running up to twice as long when the noise
+        # blocks may take every CU (the 64 KB erosion blocks then find no
+        # room), and the large jobs at the tail finishing about 5 ms later.
+        self.noise_masked = None
+        if noise_cu_share is not None and 0.0 < noise_cu_share < 1.0:
+            ncu = torch.cuda.get_device_properties(self.device).multi_processor_count
+            self.noise_masked = _lib.masked_stream(max(1, int(ncu * noise_cu_share)), self.device)
+        # at most this many ticks (noise passes and erosion chunks) queued on
+        # the GPU before the host waits for the oldest
+        self.max_queued = max(1, int(max_queued))
         self.meshes = _batch.MeshCache(self.device)
         self.noise_runner = _batch.BatchRunner(self.device, self.stream)
+        self.masked_runner = (_batch.BatchRunner(self.device, self.noise_masked)
+                              if self.noise_masked is not None else None)
         self.erode_runner = _batch.BatchRunner(self.device, self.erode_stream)
         self.budget = int(hbm_budget)
         # steps_per_tick: a fixed per-job cap per tick instead of the budget
@@ -244,16 +261,27 @@ class PlanetServer:
                 outcomes[id(j)].admitted = now
                 admitted.append(_batch.PlanetRun(j, self.meshes))
             if admitted:
-                self.noise_runner.noise(admitted, self.tuning)
-                ev = clock.event(self.stream)
+                if eroding and self.masked_runner is not None:
+                    # the fields were allocated in the current stream's order
+                    ns = self.noise_masked
+                    ns.wait_stream(self.stream)
+                    for r in admitted:
+                        r.elev.record_stream(ns)
+                        if r.state is not None:
+                            r.state.buf.record_stream(ns)
+                    self.masked_runner.noise(admitted, self.tuning)
+                else:
+                    ns = self.stream
+                    self.noise_runner.noise(admitted, self.tuning)
+                ev = clock.event(ns)
                 done_now = [r for r in admitted if r.state is None]
                 inflight.append((ev, done_now))
                 new_erode = [r for r in admitted if r.state is not None]
                 if new_erode:
                     self.erode_stream.wait_event(ev)
                     for r in new_erode:
-                        # allocated on the noise stream, used on the erosion
-                        # stream: keep the allocator from reusing them early
+                        # allocated in the current stream's order, used on the
+                        # erosion stream: keep the allocator from reusing them early
                         r.elev.record_stream(self.erode_stream)
                         r.state.buf.record_stream(self.erode_stream)
                     eroding += new_erode
@@ -269,8 +297,8 @@ class PlanetServer:
                 print(f"[serve] t={now:.4f} pending={len(pending)} waiting={len(waiting)} "
                       f"admitted={len(admitted)} eroding={len(eroding)} "
                       f"inflight={len(inflight)} used={adm.used}", file=sys.stderr, flush=True)
-            # Keep at most two erosion ticks queued; retire what has finished.
-            while inflight and (len(inflight) > 3 or inflight[0][0].query()):
+            # Keep at most max_queued ticks queued; retire what has finished.
+            while inflight and (len(inflight) > self.max_queued or inflight[0][0].query()):
                 ev, done_runs = inflight.popleft()
                 ev.synchronize()
                 t = clock.gpu_time(ev)

@@ -43,10 +43,16 @@ def test_served_tiny_budget_queues_and_rejects():
     assert all(o.done >= o.admitted >= o.job.arrival - 1e-3 for o in outs if not o.rejected)
 
 
-def test_plain_server_matches_served():
+@pytest.mark.parametrize("cu_share", [0.5, None])
+def test_plain_server_matches_served(cu_share):
+    """Served results equal the plain ones, with the noise passes of jobs
+    that arrive while others erode on the CU-masked stream (the default) or
+    on the whole GPU."""
     jobs = _trace()
     a = serve.PlainServer().serve(jobs, keep_results=True)
-    b = serve.PlanetServer(steps_per_tick=5).serve(jobs, keep_results=True)
+    srv = serve.PlanetServer(steps_per_tick=5, noise_cu_share=cu_share)
+    assert (srv.noise_masked is not None) == (cu_share is not None)
+    b = srv.serve(jobs, keep_results=True)
     for x, y in zip(a, b):
         assert (x.heights.cpu().numpy() == y.heights.cpu().numpy()).all()
 
