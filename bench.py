@@ -473,9 +473,17 @@ def bench_batched(args):
     ms_noise = _events_ms(lambda: br.noise(runs, tuning))
     ms_p = min(_events_ms(lambda: batch.run_plain(runs)) for _ in range(2))
     def plain_noise():
+        # the batched noise phase's work one planet per call: noise, the
+        # rescale and the erosion state load
+        from nixis_amd import util
         for r in runs:
+            j = r.job
             terrain.sample_octaves_device(r.verts, None, r.perm, r.pgi, r.rough, r.strength,
-                                          1.0, out=r.elev)
+                                          j.world_radius, out=r.elev)
+            if j.lower < j.upper:
+                util.rescale_device(r.elev, j.lower, j.upper, out=r.elev)
+            if r.state is not None:
+                r.state.load(r.elev)
     ms_pnoise = _events_ms(plain_noise)
     vo = sum(j.vertex_octaves for j in jobs)
     vi = sum(j.vertex_iters for j in jobs)

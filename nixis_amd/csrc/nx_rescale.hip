@@ -344,7 +344,7 @@ static_assert(sizeof(PrHead) <= 256, "PrHead must fit the 256-byte workspace hea
 static_assert(offsetof(PrHead, fast) == 224, "PrHead::fast moved");
 
 constexpr int kPrThreads = 512;
-constexpr int kFinalThreads = 1024;
+constexpr int kFinalThreads = 256;  // as red_grid assumes
 constexpr int kPrPer = 8;                          // values per lane per wave chunk
 constexpr int kPrSlices = kPrPer / 2;              // slices of 128 values per chunk
 constexpr int64_t kPrUnit = (int64_t)64 * kPrPer;  // values per wave chunk
@@ -398,44 +398,6 @@ __device__ __forceinline__ PSum psum_join(const PSum &a, const PSum &b)
         r.suf = b.suf;
     }
     return r;
-}
-
-__device__ __forceinline__ PSum psum_shfl_down(const PSum &s, int off)
-{
-    PSum r;
-    r.first = __shfl_down(s.first, off, 64);
-    r.last = __shfl_down(s.last, off, 64);
-    r.mn = __shfl_down(s.mn, off, 64);
-    r.mx = __shfl_down(s.mx, off, 64);
-    r.suf = __shfl_down(s.suf, off, 64);
-    r.ne = __shfl_down(s.ne, off, 64);
-    r.brk = __shfl_down(s.brk, off, 64);
-    return r;
-}
-
-// In-order reduction over one wave; the result is valid in lane 0.
-__device__ __forceinline__ PSum psum_wave(PSum s)
-{
-    const int lane = threadIdx.x & 63;
-    for (int off = 1; off < 64; off <<= 1) {
-        const PSum o = psum_shfl_down(s, off);
-        if ((lane & (2 * off - 1)) == 0) s = psum_join(s, o);
-    }
-    return s;
-}
-
-// In-order reduction over the block (lane order within waves, then wave
-// order); the result is valid in thread 0.  lds holds one entry per wave.
-template <int NW>
-__device__ PSum psum_block(PSum s, PSum *lds)
-{
-    s = psum_wave(s);
-    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0)
-        for (int w = 1; w < NW; ++w) s = psum_join(s, lds[w]);
-    __syncthreads();  // lds is reused by the next call
-    return s;
 }
 
 // Bounds of the selected values from the whole array's summary.
@@ -555,25 +517,6 @@ __device__ ChainConsts chain_derive(const PSum &so, const PSum &sl, double t_min
     return c;
 }
 
-// In-order reduction of the nsum pairs of block summaries sum[2c], sum[2c+1]
-// by one block of NT threads; the results are valid in thread 0.
-template <int NT>
-__device__ void pr_reduce_sums(const PSum *sum, int64_t nsum, PSum (*lds)[NT / 64], PSum &s0,
-                               PSum &s1)
-{
-    const int64_t per = (nsum + NT - 1) / NT;
-    const int64_t c0 = (int64_t)threadIdx.x * per;
-    const int64_t c1 = c0 + per < nsum ? c0 + per : nsum;
-    s0 = psum_empty();
-    s1 = psum_empty();
-    for (int64_t c = c0; c < c1; ++c) {
-        s0 = psum_join(s0, sum[2 * c]);
-        s1 = psum_join(s1, sum[2 * c + 1]);
-    }
-    s0 = psum_block<NT / 64>(s0, lds[0]);
-    s1 = psum_block<NT / 64>(s1, lds[1]);
-}
-
 __device__ __forceinline__ double wave_fmin(double v)
 {
     for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
@@ -678,6 +621,125 @@ __device__ __forceinline__ PSum wave_run_summary(const WaveRun &R, double mn, do
         r.suf = mx;
     }
     return r;
+}
+
+// In-order join of the runs held by the block's threads (thread t holds run
+// t; empty runs are skipped), in parallel; the result is valid in every
+// thread.  Joined in order, the runs stay one strictly decreasing run until
+// the first run K that breaks: one broken inside, or one whose first value
+// is not below the last value of the nearest non-empty run before it.  So
+// first is the first non-empty run's, min and max are over all runs, and
+// after a break suf is run K's (its max if the break is at its start, else
+// its own suf) joined with the max of every later run; without one, last is
+// the last non-empty run's.  A lane scan finds each run's predecessor,
+// ballots the first break and the first and last runs.
+template <int NT>
+__device__ PSum psum_join_block(const PSum &P)
+{
+    constexpr int NW = NT / 64;
+    __shared__ double s_last[NW], s_mn[NW], s_mx[NW], s_sf[NW];
+    __shared__ int s_has[NW], s_k[NW], s_f[NW], s_l[NW];
+    __shared__ double s_first, s_lastv;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int ne = P.ne;
+    // inclusive scan over the wave: the last value of the latest non-empty run
+    double pv = P.last;
+    int ph = ne;
+    for (int off = 1; off < 64; off <<= 1) {
+        const double tv = __shfl_up(pv, off, 64);
+        const int th = __shfl_up(ph, off, 64);
+        if (lane >= off && !ph) {
+            pv = tv;
+            ph = th;
+        }
+    }
+    if (lane == 63) {
+        s_last[w] = pv;
+        s_has[w] = ph;
+    }
+    const unsigned long long bn = __ballot(ne);
+    if (lane == 0) {
+        s_f[w] = bn ? w * 64 + __ffsll((long long)bn) - 1 : NT;
+        s_l[w] = bn ? w * 64 + 63 - __clzll((long long)bn) : -1;
+    }
+    double mn = ne ? P.mn : INFINITY, mx = ne ? P.mx : -INFINITY;
+    mn = wave_fmin(mn);
+    mx = wave_fmax(mx);
+    if (lane == 0) {
+        s_mn[w] = mn;
+        s_mx[w] = mx;
+    }
+    __syncthreads();
+    // the predecessor: within the wave, else the latest earlier wave's
+    double ep = __shfl_up(pv, 1, 64);
+    int eh = __shfl_up(ph, 1, 64);
+    if (lane == 0) eh = 0;
+    if (!eh) {
+        for (int j = w - 1; j >= 0; --j)
+            if (s_has[j]) {
+                ep = s_last[j];
+                eh = 1;
+                break;
+            }
+    }
+    const bool cross = ne && eh && !(P.first < ep);
+    const bool brk = ne && (P.brk || cross);
+    const unsigned long long bb = __ballot(brk);
+    if (lane == 0) s_k[w] = bb ? w * 64 + __ffsll((long long)bb) - 1 : NT;
+    int F = NT, L = -1;
+    double MN = INFINITY, MX = -INFINITY;
+    for (int j = 0; j < NW; ++j) {
+        F = s_f[j] < F ? s_f[j] : F;
+        L = s_l[j] > L ? s_l[j] : L;
+        MN = fmin(MN, s_mn[j]);
+        MX = fmax(MX, s_mx[j]);
+    }
+    if (t == F) s_first = P.first;
+    if (t == L) s_lastv = P.last;
+    __syncthreads();
+    int K = NT;
+    for (int j = 0; j < NW; ++j) K = s_k[j] < K ? s_k[j] : K;
+    double sf = (ne && t > K) ? P.mx : -INFINITY;
+    if (t == K) sf = cross ? P.mx : P.suf;
+    sf = wave_fmax(sf);
+    if (lane == 0) s_sf[w] = sf;
+    __syncthreads();
+    PSum r = psum_empty();
+    if (F < NT) {
+        r.ne = 1;
+        r.first = s_first;
+        r.last = s_lastv;
+        r.mn = MN;
+        r.mx = MX;
+        if (K < NT) {
+            double SF = -INFINITY;
+            for (int j = 0; j < NW; ++j) SF = fmax(SF, s_sf[j]);
+            r.brk = 1;
+            r.suf = SF;
+        }
+    }
+    __syncthreads();  // the shared scalars are reused by the next call
+    return r;
+}
+
+// In-order reduction of the nsum pairs of block summaries sum[2c], sum[2c+1]
+// by one block of NT threads: each thread joins a run of ceil(nsum / NT)
+// consecutive pairs, then the block joins the threads' results in parallel;
+// the results are valid in every thread.
+template <int NT>
+__device__ void pr_reduce_sums(const PSum *sum, int64_t nsum, PSum &s0, PSum &s1)
+{
+    const int64_t per = (nsum + NT - 1) / NT;
+    const int64_t c0 = (int64_t)threadIdx.x * per;
+    const int64_t c1 = c0 + per < nsum ? c0 + per : nsum;
+    s0 = psum_empty();
+    s1 = psum_empty();
+    for (int64_t c = c0; c < c1; ++c) {
+        s0 = psum_join(s0, sum[2 * c]);
+        s1 = psum_join(s1, sum[2 * c + 1]);
+    }
+    s0 = psum_join_block<NT>(s0);
+    s1 = psum_join_block<NT>(s1);
 }
 
 // Summaries of the values whose mask is unset (sum[2b]) and set (sum[2b + 1])
@@ -864,19 +926,33 @@ __global__ __launch_bounds__(kPrThreads) void k_pr_summary(double *x, uint8_t *m
 }
 
 // One block: in-order reduction of the per-block summaries into head->s;
-// from_keys: head->xmin/xmax = (min, max) of x from head->keys0.
+// from_keys: head->xmin/xmax = (min, max) of x from head->keys0; chain: the
+// two-pass height chain's constants (chain_derive) into head, and the
+// five-pass kernels' reduction keys reset for the case they run.
 __global__ __launch_bounds__(1024) void k_pr_finish(const PSum *__restrict__ sum, int64_t nsum,
-                                                    PrHead *__restrict__ head, int from_keys)
+                                                    PrHead *__restrict__ head, int from_keys,
+                                                    int chain)
 {
-    __shared__ PSum lds[2][16];
     PSum s0, s1;
-    pr_reduce_sums<1024>(sum, nsum, lds, s0, s1);
+    pr_reduce_sums<1024>(sum, nsum, s0, s1);
     if (threadIdx.x == 0) {
         head->s[0] = s0;
         head->s[1] = s1;
         if (from_keys) {
             head->xmin = dunkey(head->keys0[0]);
             head->xmax = dunkey(head->keys0[1]);
+        }
+        if (chain) {
+            const ChainConsts cc = chain_derive(s1, s0, head->xmin, head->xmax, head->level);
+            head->lo1 = cc.lo1;
+            head->r1 = cc.r1;
+            head->lo0 = cc.lo0;
+            head->r0 = cc.r0;
+            head->h4min = cc.h4min;
+            head->h4max = cc.h4max;
+            head->fast = cc.fast;
+            head->keys1[0] = head->keys2[0] = ~0ull;
+            head->keys1[1] = head->keys2[1] = 0ull;
         }
     }
 }
@@ -953,52 +1029,26 @@ __global__ __launch_bounds__(256) void k_pr_map(const double *x, double *out,
     }
 }
 
-// The height chain's second pass (nixis.py:332-364 per value).  Each block
-// first reduces the summary pass's nsum pairs of block summaries and derives
-// the chain's constants (chain_derive) itself; block 0 also stores them in
-// head, with the summaries, for the five-pass kernels that follow.  Then per
-// value: the first rescale with min/max from head->keys0 and the mask
-// t <= level; when the constants hold, the ocean or land power_rescale map,
-// the shift by the level and the split rescale around 0 with the shifted
+// The height chain's second pass (nixis.py:332-364 per value), after
+// k_pr_finish has derived the chain's constants into head: the first
+// rescale with min/max from head->keys0 and the mask t <= level; when the
+// constants hold (head->fast), the ocean or land power_rescale map, the
+// shift by the level and the split rescale around 0 with the shifted
 // heights' min/max.  Heights and mask are written once.  Every operation is
 // the one the five-pass kernels apply to that value, in the same order.
 // vec: h 16-byte and mask 2-byte aligned.
 __global__ __launch_bounds__(kFinalThreads) void k_chain_final(double *h, uint8_t *__restrict__ mask,
-                                                               int64_t n, const PSum *__restrict__ sum,
-                                                               int64_t nsum, PrHead *__restrict__ head,
+                                                               int64_t n,
+                                                               const PrHead *__restrict__ head,
                                                                double min_alt, double max_alt,
                                                                int *__restrict__ err, int vec)
 {
-    __shared__ PSum lds[2][kFinalThreads / 64];
-    __shared__ ChainConsts cc;
-    {
-        PSum s0, s1;
-        pr_reduce_sums<kFinalThreads>(sum, nsum, lds, s0, s1);
-        if (threadIdx.x == 0) {
-            cc = chain_derive(s1, s0, head->xmin, head->xmax, head->level);
-            if (blockIdx.x == 0) {
-                head->s[0] = s0;
-                head->s[1] = s1;
-                head->lo1 = cc.lo1;
-                head->r1 = cc.r1;
-                head->lo0 = cc.lo0;
-                head->r0 = cc.r0;
-                head->h4min = cc.h4min;
-                head->h4max = cc.h4max;
-                head->fast = cc.fast;
-                // the five-pass kernels' reductions, should they run
-                head->keys1[0] = head->keys2[0] = ~0ull;
-                head->keys1[1] = head->keys2[1] = 0ull;
-            }
-        }
-        __syncthreads();
-    }
     const double x_min = dunkey(head->keys0[0]), x_max = dunkey(head->keys0[1]);
     const double xr = x_max - x_min, nr = max_alt - min_alt;
     const double level = head->level;
-    const double lo1 = cc.lo1, r1 = cc.r1, lo0 = cc.lo0, r0 = cc.r0;
-    const double h4min = cc.h4min, h4max = cc.h4max;
-    const bool fast = cc.fast != 0;
+    const double lo1 = head->lo1, r1 = head->r1, lo0 = head->lo0, r0 = head->r0;
+    const double h4min = head->h4min, h4max = head->h4max;
+    const bool fast = head->fast != 0;
     bool zero = false;
     auto f = [&](double v, uint8_t &m) {
         const double t = ((v - x_min) / xr) * nr + min_alt;
@@ -1104,7 +1154,7 @@ int power_rescale(nx_ctx *ctx, const double *x, const uint8_t *mask, int64_t n, 
                        stream, const_cast<double *>(x), const_cast<uint8_t *>(mask), n, g.nunits,
                        g.cpw, pr_vec(x, mask), sum, head, ChainArgs{0.0, 0.0, 0.0, nullptr},
                        ctx->d_err);
-    hipLaunchKernelGGL(k_pr_finish, dim3(1), dim3(1024), 0, stream, sum, g.blocks, head, 1);
+    hipLaunchKernelGGL(k_pr_finish, dim3(1), dim3(1024), 0, stream, sum, g.blocks, head, 1, 0);
     hipLaunchKernelGGL(k_pr_map, dim3(grid), dim3(256), 0, stream, x, out, mask, n, mode, power,
                        head, (const unsigned long long *)nullptr, 0,
                        (unsigned long long *)nullptr, ctx->d_err, vec16(x) & vec16(out) & vec2(mask), 0);
@@ -1117,10 +1167,10 @@ int power_rescale(nx_ctx *ctx, const double *x, const uint8_t *mask, int64_t n, 
 // Two passes over the heights after their min/max: the first computes the
 // rescale and the mask of each value and the power_rescale summaries of both
 // halves without writing (the land half's values are not changed by the
-// ocean half's power_rescale, so one pass summarises both); the second pass
-// reduces the block summaries in every block, derives every later bound and
-// min/max from them (chain_derive), applies the whole chain to each value
-// and writes heights and mask.  When the derivation does not hold (a zero range,
+// ocean half's power_rescale, so one pass summarises both); one block joins
+// the block summaries and derives every later bound and min/max from them
+// (k_pr_finish, chain_derive); the second pass applies the whole chain to
+// each value and writes heights and mask.  When the derivation does not hold (a zero range,
 // a zero or non-finite extreme), the second pass writes the rescaled heights
 // and mask instead, and the kernels of the five-pass form, which otherwise
 // return at once, finish the chain: the ocean power_rescale with the min/max
@@ -1146,22 +1196,21 @@ static int chain_after_keys(nx_ctx *ctx, double *h, int64_t n, double min_alt, d
     const ChainArgs ca{min_alt, max_alt, ocean_percent, level};
     const int vsum = pr_vec(h, mask), vmap = vec16(h) & vec2(mask);
     if (ctx->chain_passes == 2) {
-        // The summary pass, then the writing pass; the five-pass kernels
-        // after it return at once when head->fast, else continue from the
-        // rescaled heights and mask the writing pass left (its block 0
-        // stores head->s and head->fast and resets keys1 and keys2).
+        // The summary pass, the join and derivation, then the writing pass;
+        // the five-pass kernels after it return at once when head->fast,
+        // else continue from the rescaled heights and mask the writing pass
+        // left (k_pr_finish resets keys1 and keys2).
         hipLaunchKernelGGL((k_pr_summary<true, false>), dim3((unsigned)g.blocks), dim3(kPrThreads),
                            0, stream, h, mask, n, g.nunits, g.cpw, vsum, sum, head, ca, ctx->d_err);
-        const unsigned fgrid = (unsigned)std::max<int64_t>(
-            1, std::min<int64_t>((n + kFinalThreads - 1) / kFinalThreads, (int64_t)ctx->num_cus * 2));
-        hipLaunchKernelGGL(k_chain_final, dim3(fgrid), dim3(kFinalThreads), 0, stream, h, mask, n,
-                           (const PSum *)sum, g.blocks, head, min_alt, max_alt, ctx->d_err, vmap);
+        hipLaunchKernelGGL(k_pr_finish, dim3(1), dim3(1024), 0, stream, sum, g.blocks, head, 0, 1);
+        hipLaunchKernelGGL(k_chain_final, dim3(grid), dim3(kFinalThreads), 0, stream, h, mask, n,
+                           (const PrHead *)head, min_alt, max_alt, ctx->d_err, vmap);
     } else {
         // nixis.py:332-346 in one pass
         hipLaunchKernelGGL((k_pr_summary<true, true>), dim3((unsigned)g.blocks),
                            dim3(kPrThreads), 0, stream, h, mask, n, g.nunits, g.cpw, vsum, sum, head,
                            ca, ctx->d_err);
-        hipLaunchKernelGGL(k_pr_finish, dim3(1), dim3(1024), 0, stream, sum, g.blocks, head, 0);
+        hipLaunchKernelGGL(k_pr_finish, dim3(1), dim3(1024), 0, stream, sum, g.blocks, head, 0, 0);
         hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(64), 0, stream, head->keys1);
         hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(64), 0, stream, head->keys2);
     }
