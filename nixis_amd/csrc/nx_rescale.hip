@@ -38,27 +38,11 @@ __device__ __forceinline__ void stream4(int64_t n, Load load, Body body)
     for (; i < n; i += stride) body(i, load(i));
 }
 
-// The same with eight loads in flight per lane, for the read-only min/max.
-template <typename T, typename Load, typename Body>
-__device__ __forceinline__ void stream8(int64_t n, Load load, Body body)
-{
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (; i + 7 * stride < n; i += 8 * stride) {
-        T v[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = load(i + u * stride);
-#pragma unroll
-        for (int u = 0; u < 8; ++u) body(i + u * stride, v[u]);
-    }
-    for (; i < n; i += stride) body(i, load(i));
-}
-
 // vec: the arrays are aligned for 16-byte accesses of value pairs; pairs
 // (2j, 2j+1) stream with stream4 and block 0 thread 0 takes an odd last value.
 __device__ __forceinline__ bool tail_thread() { return blockIdx.x == 0 && threadIdx.x == 0; }
 
-constexpr int kMinmaxThreads = 256;
+constexpr int kMinmaxThreads = 1024;
 
 __global__ void k_minmax_init(unsigned long long *keys)
 {
@@ -68,6 +52,10 @@ __global__ void k_minmax_init(unsigned long long *keys)
     }
 }
 
+// Min/max keys of x.  Each wave takes a contiguous run of 512-value chunks
+// (four slices of 128, lane l holding values 2l and 2l + 1 of each, so every
+// load instruction covers one contiguous range) with the next chunk's loads
+// in flight while it reduces the current one.  vec: x is 16-byte aligned.
 __global__ __launch_bounds__(kMinmaxThreads) void k_minmax(const double *__restrict__ x, int64_t n,
                                                 unsigned long long *__restrict__ keys, int vec)
 {
@@ -77,16 +65,41 @@ __global__ __launch_bounds__(kMinmaxThreads) void k_minmax(const double *__restr
         lo = k < lo ? k : lo;
         hi = k > hi ? k : hi;
     };
-    if (vec) {
-        const double2 *x2 = (const double2 *)x;
-        stream8<double2>(n >> 1, [&](int64_t j) { return x2[j]; },
-                         [&](int64_t, double2 d) {
-                             upd(d.x);
-                             upd(d.y);
-                         });
-        if ((n & 1) && tail_thread()) upd(x[n - 1]);
-    } else {
-        stream8<double>(n, [&](int64_t i) { return x[i]; }, [&](int64_t, double v) { upd(v); });
+    const int lane = threadIdx.x & 63;
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    const int64_t wid = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int64_t nunits = (n + 511) / 512;
+    const int64_t cpw = (nunits + waves - 1) / waves;
+    const int64_t u0 = wid * cpw, u1 = u0 + cpw < nunits ? u0 + cpw : nunits;
+    // value j of the lane's share of chunk u sits at u * 512 + 2 lane + off(j)
+    auto off = [](int j) { return (int64_t)(j >> 1) * 128 + (j & 1); };
+    auto load = [&](int64_t u, double *v, int64_t &left) {
+        const int64_t p0 = u * 512 + 2 * lane;
+        left = n - p0;
+        if (vec && off(7) < left) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const double2 t = *(const double2 *)(x + p0 + k * 128);
+                v[2 * k] = t.x;
+                v[2 * k + 1] = t.y;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = off(j) < left ? x[p0 + off(j)] : 0.0;
+        }
+    };
+    double vn[8];
+    int64_t leftn = 0;
+    if (u0 < u1) load(u0, vn, leftn);
+    for (int64_t u = u0; u < u1; ++u) {
+        double v[8];
+        const int64_t left = leftn;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = vn[j];
+        if (u + 1 < u1) load(u + 1, vn, leftn);
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (off(j) < left) upd(v[j]);
     }
     block_minmax(lo, hi);
     if (threadIdx.x == 0) {
@@ -135,7 +148,8 @@ static int vec2(const void *p) { return (uintptr_t)p % 2 == 0 ? 1 : 0; }
 // lane: every block ends in two atomics on the same keys, and with eight
 // blocks per CU those atomics at the end of the pass cost a large part of it
 // (40 M values: 80 against 54 us alone, 89 against 79 us in the height chain).
-// Blocks of 1024 threads were no faster.
+// One block of kMinmaxThreads per CU: sixteen waves streaming, two atomics
+// per CU.
 static unsigned mm_grid(nx_ctx *ctx, int64_t n)
 {
     return (unsigned)std::max<int64_t>(
