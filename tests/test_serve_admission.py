@@ -79,6 +79,7 @@ def test_erosion_ticks_hand_out_whole_pairs(budget, per_tick):
     from nixis_amd.serve import PlanetServer
     srv = PlanetServer.__new__(PlanetServer)
     srv.tick_vertex_steps, srv.steps_per_tick, srv.young_share = budget, per_tick, 1.0
+    srv.fill_vertices = 0
     runs = [_Run(1000, 31), _Run(400, 7), _Run(3000, 10)]
     eroding, ticks = list(runs), 0
     while eroding:
@@ -105,6 +106,7 @@ def test_noise_share_admits_first_fit_within_the_tick_budget():
     from nixis_amd.serve import PlanetServer
     srv = PlanetServer.__new__(PlanetServer)
     srv.tick_vertex_steps, srv.steps_per_tick, srv.young_share = 10**12, 0, 1.0
+    srv.fill_vertices = 0
     eroding = [_Run(1_000_000, 40)]          # tick: 40 steps x 1e6 vertices
     tick_s = 40 * 1_000_000 / serve.EROSION_RATE
 
@@ -137,7 +139,7 @@ def test_young_share_keeps_the_oldest_job_alone():
     lets younger jobs use at most that share of the tick's budget."""
     from nixis_amd.serve import PlanetServer
     srv = PlanetServer.__new__(PlanetServer)
-    srv.tick_vertex_steps, srv.steps_per_tick = 100_000, 0
+    srv.tick_vertex_steps, srv.steps_per_tick, srv.fill_vertices = 100_000, 0, 0
     runs = [_Run(1000, 50), _Run(100, 40), _Run(100, 40)]   # the oldest leaves 50,000
     srv.young_share = 0.0
     tick, caps = srv._tick(runs)
@@ -148,3 +150,23 @@ def test_young_share_keeps_the_oldest_job_alone():
     srv.young_share = 1.0
     tick, caps = srv._tick(runs)
     assert tick == runs and caps == [50, 40, 40]
+
+
+def test_ticks_of_small_jobs_are_filled():
+    """With young_share 0, younger jobs still join a tick while its planets
+    together have fewer than fill_vertices vertices, each for at most the
+    oldest job's number of steps; an oldest job that fills the GPU keeps the
+    tick to itself."""
+    from nixis_amd.serve import PlanetServer
+    srv = PlanetServer.__new__(PlanetServer)
+    srv.tick_vertex_steps, srv.steps_per_tick, srv.young_share = 100_000, 0, 0.0
+    runs = [_Run(1000, 50), _Run(100, 40), _Run(300, 60), _Run(2000, 10)]
+    srv.fill_vertices = 1500
+    tick, caps = srv._tick(runs)
+    assert tick == runs and caps == [50, 40, 50, 10]
+    srv.fill_vertices = 1050
+    tick, caps = srv._tick(runs)
+    assert tick == runs[:2] and caps == [50, 40]
+    srv.fill_vertices = 500
+    tick, caps = srv._tick(runs)
+    assert tick == runs[:1] and caps == [50]
