@@ -159,9 +159,16 @@ class PlanetServer:
         # blocks may take every CU (the 64 KB erosion blocks then find no
         # room), and the large jobs at the tail finishing about 5 ms later.
         self.noise_masked = None
+        self.noise_mask_error = None
         if noise_cu_share is not None and 0.0 < noise_cu_share < 1.0:
             ncu = torch.cuda.get_device_properties(self.device).multi_processor_count
-            self.noise_masked = _lib.masked_stream(max(1, int(ncu * noise_cu_share)), self.device)
+            try:
+                self.noise_masked = _lib.masked_stream(max(1, int(ncu * noise_cu_share)),
+                                                       self.device)
+            except _lib.NixisError as e:
+                # a scheduling choice, not a result: without the mask the
+                # noise passes use every CU (recorded for the caller)
+                self.noise_mask_error = str(e)
         # at most this many ticks (noise passes and erosion chunks) queued on
         # the GPU before the host waits for the oldest
         self.max_queued = max(1, int(max_queued))
