@@ -237,6 +237,31 @@ def test_height_chain_two_pass_equals_five_pass(case):
         assert e2 is ZeroDivisionError
 
 
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 511, 512, 513, 1025, 4097])
+def test_small_and_ragged_sizes(oracle, n):
+    """The min/max pass (per-wave runs of 512-value chunks), rescale and both
+    forms of the height chain at sizes around the chunk and block edges:
+    rescale bit for bit against the oracle, the chain's two forms bit for bit
+    against each other, errors included (one value, or one per half, is a
+    zero range)."""
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n)
+    print("small sizes", n, flush=True)
+    if n > 1:
+        got = util.rescale(x, -4000.0, 8850.0)
+        assert np.array_equal(got, oracle.rescale_ex(x, -4000.0, 8850.0))
+    (h2, m2, l2, e2, _), (h5, m5, l5, e5, _) = _chain_both_forms(x)
+    assert _same_bits(h2, h5) and np.array_equal(m2, m5) and e2 is e5
+    assert _same_bits(np.array([l2]), np.array([l5]))      # NaN when x is constant
+    if e2 is None:
+        ref_h, ref_mask, ref_level = oracle.height_chain(x, -4000.0, 8850.0, 55.0)
+        assert l2 == ref_level and np.array_equal(m2.astype(bool), ref_mask)
+        assert np.abs(h2 - ref_h).max() <= 64 * EPS * (8850.0 + 4000.0)
+    else:
+        with pytest.raises(ZeroDivisionError):
+            oracle.height_chain(x, -4000.0, 8850.0, 55.0)
+
+
 def test_height_chain_full_size_properties():
     """At 10,000,002 vertices (d=1000): the final range is exactly
     [min_alt, max_alt], the ocean is exactly the vertices at or below 0,
