@@ -346,9 +346,10 @@ struct PrHead {
     double level;       // ocean level (height chain)
     double pad;
     PSum s[2];          // whole-array summaries: [0] mask unset, [1] mask set
-    // Height chain in two passes (chain_derive, k_chain_final): the ocean and
-    // land power_rescale bounds (lower, range) and the min/max of the shifted
-    // heights, and whether they were derived (else the five-pass kernels run).
+    // Height chain in two passes (derived by k_pr_finish, read by
+    // k_chain_final): the ocean and land power_rescale bounds (lower, range)
+    // and the min/max of the shifted heights, and whether they were derived
+    // (else the five-pass kernels run).
     double lo1, r1, lo0, r0, h4min, h4max;
     int32_t fast, pad2;
 };
@@ -358,7 +359,7 @@ static_assert(sizeof(PrHead) <= 256, "PrHead must fit the 256-byte workspace hea
 static_assert(offsetof(PrHead, fast) == 224, "PrHead::fast moved");
 
 constexpr int kPrThreads = 512;
-constexpr int kFinalThreads = 256;  // as red_grid assumes
+constexpr int kFinalThreads = 256;  // the block size red_grid sizes grids for
 constexpr int kPrPer = 8;                          // values per lane per wave chunk
 constexpr int kPrSlices = kPrPer / 2;              // slices of 128 values per chunk
 constexpr int64_t kPrUnit = (int64_t)64 * kPrPer;  // values per wave chunk
