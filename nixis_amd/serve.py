@@ -18,10 +18,11 @@ erosion tick hands out a budget of vertex-steps to the eroding jobs oldest
 first, each job taking as many of its remaining steps as the budget left
 covers (at least two): a small planet finishes all its steps in one tick
 while a large one advances a few steps per tick in arrival order.  By
-default the oldest eroding job has each tick to itself (young_share), and
-noise passes may be held to a share of the tick's time (noise_share).  At most
-a few ticks are in flight.  Completion times come
-from GPU events relative to an event recorded at the start of the trace.
+default the oldest eroding job has each tick to itself (young_share) unless
+its planet is too small to fill the GPU (fill_vertices), and noise passes may
+be held to a share of the tick's time (noise_share).  At most a few ticks are
+in flight.  Completion times come from GPU events relative to an event
+recorded at the start of the trace.
 Results stay on the device.  ``PlainServer`` runs the same trace one planet
 per call, in arrival order, for comparison.
 """
@@ -197,9 +198,10 @@ class PlanetServer:
         self.noise_share = None if noise_share is None else float(noise_share)
         # young_share: of a tick's vertex-steps, at most this share goes to
         # eroding jobs other than the oldest.  0 (the default) erodes the
-        # jobs strictly in arrival order, one per tick: measured on the
-        # 120-job trace, giving younger jobs the oldest one's leftover budget
-        # delayed the largest jobs (p99 163.6 ms at 1.0, 156.4 ms at 0)
+        # jobs in arrival order, the oldest alone in its tick unless it is
+        # too small to fill the GPU (fill_vertices): measured on the 120-job
+        # trace, giving younger jobs the oldest one's leftover budget delayed
+        # the largest jobs (p99 163.6 ms at 1.0, 156.4 ms at 0)
         self.young_share = float(young_share)
         self.trace = bool(os.environ.get("NX_SERVE_TRACE"))
 
