@@ -227,6 +227,23 @@ def test_pair_launches_match_single_steps(oracle, d, iters, chunk):
           f"pair vs single {_rel(a[0], b[0]):.1e}, vs fp64 {_rel(a[0], href):.1e}")
 
 
+@pytest.mark.parametrize("variant", range(7))
+def test_pair_kernel_variants_match_oracle(oracle, variant):
+    """Every two-step kernel variant (block size, neighbour data kept in
+    registers or read again) gives the FP64 oracle's fields bit for bit, from
+    one tile to over a thousand (odd step counts end with a single step)."""
+    for d, iters, chunk in ((2, 5, 0), (24, 12, 0), (60, 7, 256), (160, 4, 256), (200, 3, 0)):
+        pts, nb, h = _planet(d)
+        g = erosion.ErosionGraph(pts, nb, chunk=chunk, variant=variant)
+        assert g.plan is not None
+        print(f"variant {variant} d={d} iters={iters} tiles {g.plan.ntiles}", flush=True)
+        got = _run_graph(g, h, iters, 1.0)
+        href = h.copy()
+        w, s = oracle.erode_terrain3(pts, nb, href, iters, 1.0, return_fields=True)
+        for x, r in zip(got, (href, w, s)):
+            assert np.array_equal(x.view(np.int64), r.view(np.int64))
+
+
 @pytest.mark.parametrize("rule", [3, 4])
 def test_four_step_plans_match_oracle(oracle, rule):
     """Four steps per launch (tile + four rings in LDS) with the remainder
