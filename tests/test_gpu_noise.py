@@ -91,6 +91,21 @@ def test_noise_crafted_points(oracle):
                oracle.noise3d(0.3, -0.2, 0.9, perm, pgi)) <= NOISE_TOL
 
 
+def test_noise_selection_on_ties_and_crafted_points(oracle):
+    """Points whose in-cell coordinates tie (a 1/8 grid: equal scores,
+    in_sum exactly 1 or 2, pair sums exactly 1) and the crafted lattice-face
+    and region-boundary points: the device build selects the oracle's
+    lattice points octave by octave, on the doubling path (lacunarity 2,
+    frequencies 1, 2, 4, 8) and on the direct one (lacunarity 3); scaling by
+    2 or 3 keeps the in-cell parts on the 1/8 grid, so every octave has ties."""
+    from test_noise_host import _tie_points
+    from test_oracle_noise import _crafted
+    rng = np.random.default_rng(21)
+    pts = np.ascontiguousarray(np.concatenate([_crafted(), _tie_points(rng, 20_000)]))
+    _check(oracle, pts, 1234, 4, 1.0, 0.4, 2.0, 0.5, label="ties, doubling")
+    _check(oracle, pts, 99, 3, 1.0, 0.4, 3.0, 0.5, label="ties, lacunarity 3")
+
+
 def test_noise_accumulates_into_elevations(oracle):
     pts = util.create_points(16)
     perm, pgi = osx.init(5)
