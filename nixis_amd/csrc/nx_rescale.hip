@@ -192,6 +192,9 @@ __global__ __launch_bounds__(256) void k_rescale_ex(const double *x, int64_t n,
 {
     if (skip && *skip) return;
     double x_min = dunkey(keys[0]), x_max = dunkey(keys[1]);
+    // numba's np.min and np.max both return the array's first NaN, so one
+    // NaN makes both extremes NaN (the keys put a NaN at one end only).
+    if (isnan(x_min) || isnan(x_max)) x_min = x_max = isnan(x_min) ? x_min : x_max;
     if (blockIdx.x == 0 && threadIdx.x == 0 && minmax) {
         minmax[0] = x_min;
         minmax[1] = x_max;

@@ -31,7 +31,11 @@
  *                          per-vertex h+s conservation and the flat-field
  *                          closed form for water on small icospheres
  *                          (tests/test_oracle_erosion.py).
- *   ora_rescale         -- closed form on affine inputs.
+ *   ora_rescale         -- closed form on affine inputs; the min/max of an
+ *                          array with a NaN follow numba's np.min/np.max
+ *                          source (the first NaN, for both), checked by
+ *                          its consequences for every form
+ *                          (tests/test_oracle_post.py).
  */
 #define _GNU_SOURCE /* sincos */
 #include "nx_oracle.h"
@@ -664,15 +668,33 @@ void ora_sample_octaves(const double *verts, int64_t V, double *elevations,
 }
 
 /* util.py:110-140, standard mode */
+/* np.min(x) and np.max(x) as numba compiles them for a float array (numba
+ * 0.54.1 numba/np/arraymath.py:516-531 and the max counterpart at 599-614):
+ * the first NaN in array order is returned at once; otherwise the running
+ * extreme replaces on a strict comparison.  n >= 1. */
+static void np_min_max(const double *x, int64_t n, double *mn, double *mx)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        if (isnan(x[i])) {
+            *mn = *mx = x[i];
+            return;
+        }
+    }
+    double lo = x[0], hi = x[0];
+    for (int64_t i = 1; i < n; ++i) {
+        if (x[i] < lo) lo = x[i];
+        if (x[i] > hi) hi = x[i];
+    }
+    *mn = lo;
+    *mx = hi;
+}
+
 int ora_rescale(const double *x, int64_t n, double lower, double upper,
                 double *out)
 {
     if (n <= 0) return 0;
-    double x_min = x[0], x_max = x[0];
-    for (int64_t i = 1; i < n; ++i) {
-        if (x[i] < x_min) x_min = x[i];
-        if (x[i] > x_max) x_max = x[i];
-    }
+    double x_min, x_max;
+    np_min_max(x, n, &x_min, &x_max);
     double x_range = x_max - x_min;
     double new_range = upper - lower;
     if (x_range == 0) return -1;
@@ -922,11 +944,8 @@ int ora_rescale_ex(const double *x, int64_t n, double lower, double upper, int h
 {
     if (n <= 0) return 0;
     memcpy(out, x, sizeof(double) * (size_t)n);          /* new_array = np.copy(x) */
-    double x_min = x[0], x_max = x[0];
-    for (int64_t i = 1; i < n; ++i) {
-        if (x[i] < x_min) x_min = x[i];
-        if (x[i] > x_max) x_max = x[i];
-    }
+    double x_min, x_max;
+    np_min_max(x, n, &x_min, &x_max);
     if (has_umin && u_min < x_min) x_min = u_min;       /* 131-134 */
     if (has_umax && u_max > x_max) x_max = u_max;
     int zero = 0;
@@ -986,11 +1005,8 @@ int ora_power_rescale(const double *x, const uint8_t *mask, int64_t n, int mode,
     if (n <= 0) return 0;
     memcpy(out, x, sizeof(double) * (size_t)n);
     const double temp_upper = 1.0, temp_lower = 0.0;
-    double x_min = x[0], x_max = x[0];
-    for (int64_t i = 1; i < n; ++i) {
-        if (x[i] < x_min) x_min = x[i];
-        if (x[i] > x_max) x_max = x[i];
-    }
+    double x_min, x_max;
+    np_min_max(x, n, &x_min, &x_max);
     double mask_lower = x_max, mask_upper = x_min;
     const double temp_range = temp_upper - temp_lower;
     for (int64_t a = 0; a < n; ++a) {

@@ -52,6 +52,25 @@ def test_rescale_forms_bit_exact(oracle, kw):
     assert got.dtype == np.float64 and np.array_equal(got, ref)
 
 
+@pytest.mark.parametrize("special", ["nan", "-nan", "inf", "-inf"])
+def test_rescale_special_values_match_oracle(oracle, special):
+    """One NaN of either sign makes both extremes NaN, as numba's np.min and
+    np.max do (the device keys order a NaN at one end only); an infinity is
+    an extreme like any value.  Every form equals the oracle: NaN where the
+    oracle has NaN, bit for bit elsewhere."""
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(100_003)
+    x[4321] = {"nan": np.nan, "-nan": -np.nan, "inf": np.inf, "-inf": -np.inf}[special]
+    for kw in ({}, {"mid": 0.1}, {"mid": -0.2, "mode": "lower"}, {"mid": 0.3, "mode": "upper"},
+               {"u_min": -5.0, "u_max": 5.0}):
+        print(special, kw, flush=True)
+        got = util.rescale(x, -4000.0, 8850.0, **kw)
+        ref = oracle.rescale_ex(x, -4000.0, 8850.0, **kw)
+        assert np.array_equal(np.isnan(got), np.isnan(ref))
+        ok = ~np.isnan(ref)
+        assert np.array_equal(got[ok], ref[ok])
+
+
 def test_rescale_hand_worked_on_device():
     x = np.array([0.0, 1.0, 2.0, 3.0, 4.0])
     assert util.rescale(x, 0, 10, mid=2.0).tolist() == [0, 1, 2, 6, 10]

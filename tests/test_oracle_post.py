@@ -43,6 +43,27 @@ def test_rescale_zero_range_raises(oracle):
         oracle.rescale_ex(np.array([1.0, 2.0]), 0, 3, mid=1.0)   # mid - x_min == 0
 
 
+def test_rescale_nan_makes_both_extremes_nan(oracle):
+    """np.min and np.max of a float array return its first NaN under numba
+    (numba 0.54.1 numba/np/arraymath.py:516-531, 599-614), so one NaN of
+    either sign makes every value that a form rescales NaN, values outside a
+    one-sided mode's half are copied, and power_rescale's bounds start at
+    NaN and stay there (util.py:125-126, 191-195)."""
+    for nan in (np.nan, -np.nan):
+        x = np.array([1.0, nan, -2.0, 3.0])
+        assert np.isnan(oracle.rescale(x, 0, 1)).all()
+        assert np.isnan(oracle.rescale_ex(x, -1.0, 1.0)).all()
+        assert np.isnan(oracle.rescale_ex(x, -1.0, 1.0, mid=0.0)).all()
+        lo = oracle.rescale_ex(x, -1.0, 1.0, mid=0.0, mode="lower")
+        assert lo[0] == 1.0 and np.isnan(lo[1]) and np.isnan(lo[2]) and lo[3] == 3.0
+        up = oracle.rescale_ex(x, -1.0, 1.0, mid=0.0, mode="upper")
+        assert np.isnan(up[0]) and np.isnan(up[1]) and up[2] == -2.0 and np.isnan(up[3])
+        m = np.array([True, False, True, False])
+        pr = oracle.power_rescale(x, m, 1, 2.0)
+        assert np.isnan(pr[0]) and np.isnan(pr[2]) and pr[3] == 3.0
+    assert np.signbit(np.array([-np.nan]))[0]   # the second case is the negative NaN
+
+
 def test_mask_is_inclusive(oracle):
     m = oracle.make_bool_elevation_mask(np.array([-1.0, 0.0, 0.5, 1.0]), 0.5)
     assert m.tolist() == [True, True, True, False]
