@@ -257,6 +257,7 @@ def bench_erosion(args):
                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
                      "traffic": profiled_traffic("erosion", V) if spl == 2 else None,
                      "algorithmic_bytes_per_launch": spl * B, "steps_per_launch": spl,
+                     "plan_stream_bytes_per_launch": getattr(graph, "stream_bytes_per_launch", None),
                      "launch_us": round(spl * per_step_ms * 1e3, 2)},
         "tuning": {"chunk": tuning.chunk, "variant": tuning.variant,
                    "graph_chunk": tuning.graph_chunk,

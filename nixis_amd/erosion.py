@@ -26,6 +26,28 @@ def calc_distance(v0, v1):
     return math.sqrt((v0[0] - v1[0]) ** 2 + (v0[1] - v1[1]) ** 2 + (v0[2] - v1[2]) ** 2)
 
 
+def _plan_stream_bytes(plan, img, V):
+    """Bytes one launch over this plan reads as contiguous runs from HBM:
+    every vertex's h, w and s (24 B), the slot codes (6 x 2 B) and ring
+    indices (4 B) of each tile's first-step and loaded vertices, and its
+    edge lengths (8 B each); the gathers of ring fields come on top and may
+    hit in L2.  Summed from the plan image's per-tile tables."""
+    nt = int(plan.ntiles)
+    K = int(plan.steps)
+
+    def tab(off, dt, n):
+        return np.frombuffer(img, dtype=dt, count=n, offset=int(off)).astype(np.int64)
+
+    T = np.diff(tab(plan.off_start, np.int32, nt + 1))
+    if K == 2:
+        cs, load = tab(plan.off_cs, np.int32, nt), tab(plan.off_load, np.int32, nt)
+    else:
+        pref = tab(plan.off_pref, np.int32, nt * (K + 1)).reshape(nt, K + 1)
+        cs, load = pref[:, K - 1], pref[:, K]
+    lengths = int(tab(plan.off_ebase, np.int64, nt + 1)[-1])
+    return 24 * int(V) + 12 * int(cs.sum()) + 8 * lengths + 4 * int((load - T).sum())
+
+
 class ErosionGraph:
     """Device neighbour table: col int32 [6,V] and FP64 elen [6,V],
     plus the tile plan that lets erosion run several steps per launch.
@@ -73,6 +95,7 @@ class ErosionGraph:
         self.elen = torch.from_numpy(elen).to(device)
         self.device = self.col.device
         self.plan = None
+        self.stream_bytes_per_launch = None
         self._plan_blob = None
         self.steps = int(steps)
         self._tuning(chunk, variant, graph_chunk)
@@ -117,6 +140,7 @@ class ErosionGraph:
         self._odev = order
         self.order = order.cpu().numpy() if order is not None else None
         self.plan = None
+        self.stream_bytes_per_launch = None
         self._plan_blob = None
         self.steps = int(steps)
         self._tuning(chunk, variant, graph_chunk)
@@ -137,6 +161,7 @@ class ErosionGraph:
         _lib.check(L.nx_tile_plan_build_chunk(_lib.ptr(col), _lib.ptr(elen), self.V, self.steps,
                                               self.chunk, _lib.ptr(img), plan.bytes,
                                               ctypes.byref(plan)))
+        self.stream_bytes_per_launch = _plan_stream_bytes(plan, img, self.V)
         blob = torch.from_numpy(img).to(self.device)
         if blob.data_ptr() % 256:
             raise RuntimeError("tile plan copy is not 256-byte aligned")
