@@ -66,15 +66,31 @@ inline TilePlanDev plan_dev(const nx_tile_plan &q)
 
 // Slot codes of a tile plan (nx_tiles.cpp, tile_edges): bits 0-11 are the
 // neighbour's local index, bits 12-14 the column of the edge's length and
-// bit 15 tells whether vertex li itself (set) or the neighbour holds it; the
-// tile's lengths are a [columns][cs] block, so each undirected edge is read
-// from one place by both of its ends.
+// bit 15 tells whether vertex li itself (set) or the neighbour holds it, so
+// each undirected edge is read from one place by both of its ends.  A
+// tile's own vertices (local 0..T-1) hold at most kLenFull lengths: columns
+// 0..kLenFull-1 of the length block have an entry for each of the cs
+// first-step vertices, later columns only for the ring vertices T..cs-1.
+constexpr int kLenFull = 3;
+NX_EHD int mul_small(int a, int b)  // a, b < 2^23 (a column and a vertex count)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __mul24(a, b);
+#else
+    return a * b;
+#endif
+}
+NX_EHD int len_index(int col, int owner, int cs, int T)
+{
+    return col < kLenFull ? mul_small(col, cs) + owner
+                          : kLenFull * cs + mul_small(col - kLenFull, cs - T) + (owner - T);
+}
 NX_EHD int slot_nbr(int code) { return code & 4095; }
 template <class P>
-NX_EHD double slot_len(int code, int li, int cs, P len)
+NX_EHD double slot_len(int code, int li, int cs, int T, P len)
 {
     const int owner = (code & 0x8000) ? li : (code & 4095);
-    return len[((code >> 12) & 7) * cs + owner];
+    return len[len_index((code >> 12) & 7, owner, cs, T)];
 }
 
 #if defined(__HIPCC__)
@@ -123,11 +139,16 @@ __device__ __forceinline__ void erode_tile_pair(double2 *A, double2 *B, double2 
         for (int m = 0; m < kKeep; ++m) {
             const int li = threadIdx.x + m * THREADS;
             if (li < cs && m < kOwnPer) {
+                int code[6];
+#pragma unroll
+                for (int q = 0; q < 6; ++q) code[q] = lcol[q * cs + li];
+                // the six code loads go out together, before any length's
+                // address depends on one of them
+                asm volatile("" ::: "memory");
 #pragma unroll
                 for (int q = 0; q < 6; ++q) {
-                    const int code = lcol[q * cs + li];
-                    nbr[m][q] = slot_nbr(code);
-                    len[m][q] = slot_len(code, li, cs, lel);
+                    nbr[m][q] = slot_nbr(code[q]);
+                    len[m][q] = slot_len(code[q], li, cs, T, lel);
                 }
             }
         }
@@ -147,7 +168,7 @@ __device__ __forceinline__ void erode_tile_pair(double2 *A, double2 *B, double2 
                 const bool kept = KEEP && m < kOwnPer;
                 const int code = kept ? 0 : lcol[q * cs + li];
                 const int n = kept ? nbr[m < kKeep ? m : 0][q] : slot_nbr(code);
-                const double d = kept ? len[m < kKeep ? m : 0][q] : slot_len(code, li, cs, lel);
+                const double d = kept ? len[m < kKeep ? m : 0][q] : slot_len(code, li, cs, T, lel);
                 const double2 nb = A[n];
                 erode_accumulate<RULE>(me.x, nb.x, nb.y, d, S, W);
             }
@@ -181,7 +202,7 @@ __device__ __forceinline__ void erode_tile_pair(double2 *A, double2 *B, double2 
             for (int q = 0; q < 6; ++q) {
                 const int code = KEEP ? 0 : lcol[q * cs + li];
                 const int n = KEEP ? nbr[m < kKeep ? m : 0][q] : slot_nbr(code);
-                const double d = KEEP ? len[m < kKeep ? m : 0][q] : slot_len(code, li, cs, lel);
+                const double d = KEEP ? len[m < kKeep ? m : 0][q] : slot_len(code, li, cs, T, lel);
                 const double2 nb = A2[n];
                 erode_accumulate<RULE>(me.x, nb.x, nb.y, d, S, W);
             }
@@ -256,7 +277,7 @@ __device__ __forceinline__ void erode_tile_multi(double2 *A, double2 *hw0, doubl
                     } else {
                         const int code = lcol[q * cs + li];
                         nb_i = slot_nbr(code);
-                        d = slot_len(code, li, cs, lel);
+                        d = slot_len(code, li, cs, T, lel);
                         if (KEEP) {
                             nbr[KEEP ? m : 0][q] = nb_i;
                             len[KEEP ? m : 0][q] = d;

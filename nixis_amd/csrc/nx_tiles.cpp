@@ -23,8 +23,9 @@
 //                               (tile_edges): neighbour's local index (tile:
 //                               0..T-1, then the rings), the column of its edge
 //                               length and whether this vertex owns it
-//   lel    double [sum K_t cs]  per tile, [K_t][cs]: each undirected edge
-//                               length once, in its owner's column
+//   lel    double [...]         per tile, len_block(K_t, cs, T) doubles: each
+//                               undirected edge length once, in its owner's
+//                               column (tile_edges, len_index)
 //   pref   int32 [ntiles][K+1]  tile, tile + ring 1, ..., tile + rings 1..K
 //   ebase  int64 [ntiles + 1]   offset of each tile's lengths in `lel`
 #include <algorithm>
@@ -110,20 +111,35 @@ bool cut(const int32_t *col, int64_t V, int64_t a, int64_t len, int K, int limit
            cut(col, V, a + h, len - h, K, limit, step1_limit, out);
 }
 
+// Doubles of a tile's length block: kt columns over c step-1 vertices, the
+// columns from kLenFull on over the ring vertices T..c-1 only.
+inline int64_t len_block(int kt, int c, int T)
+{
+    return (int64_t)std::min(kt, kLenFull) * c + (int64_t)std::max(kt - kLenFull, 0) * (c - T);
+}
+
 // Edge lengths of one tile, stored once per undirected edge.  Every length
-// the tile's step-1 vertices (local 0..c-1) read belongs to one of them: an
-// edge to a vertex outside that set to the vertex inside it, an edge between
-// two of them to one of the two, chosen so that no vertex owns more than
-// kt = 4 edges where the tile allows it (an orientation of bounded out-degree:
-// greedy, then augmenting paths move ownership away from vertices over kt;
-// kt grows where no path exists).  Owner o's j-th owned edge (in slot order)
-// has its length at [j][o] of the tile's [kt][c] block.  The slot code is the
-// neighbour's local index (bits 0-11), the length's column (bits 12-14) and
-// bit 15 set when the slot's own vertex is the owner.  A pad slot (the vertex
-// itself) is coded as owned, column 0: a tie adds nothing, whatever the
-// length.  lc [6][c]: local neighbour index of each slot; len [6][c]: its
-// length.  Returns kt; writes codes [6][c] and lens [kt][c] when given.
-int tile_edges(const int32_t *lc, const double *len, int c, uint16_t *codes, double *lens)
+// the tile's step-1 vertices (local 0..c-1, the tile's own 0..T-1 first)
+// read belongs to one of them: an edge to a vertex outside that set to the
+// vertex inside it, an edge between two of them to one of the two.  The
+// owners are chosen so that each of the tile's own vertices holds at most
+// kLenFull edges and the ring vertices as few as possible: greedy (an edge
+// between an own and a ring vertex to the ring vertex, else to the end that
+// holds fewer), then augmenting paths move ownership away from vertices over
+// their cap (kLenFull for own vertices, kr for ring vertices, kr growing
+// where no path exists).  Owner o's j-th owned edge (in slot order) has its
+// length at len_index(j, o, c, T) (nx_pair_dev.h): columns below kLenFull
+// hold all c vertices, later ones the ring vertices only.  The slot code is
+// the neighbour's local index (bits 0-11), the length's column (bits 12-14)
+// and bit 15 set when the slot's own vertex is the owner.  A pad slot (the
+// vertex itself) is coded as owned, column 0: a tie adds nothing, whatever
+// the length.  lc [6][c]: local neighbour index of each slot; len [6][c]:
+// its length.  Returns the number of columns kt, or -1 when an own vertex
+// cannot be held to kLenFull edges (a neighbour table that is not symmetric
+// can force more on it); writes codes [6][c] and the len_block(kt, c, T)
+// lengths when given.
+int tile_edges(const int32_t *lc, const double *len, int c, int T, uint16_t *codes,
+               double *lens)
 {
     struct Edge {
         int32_t u, v;
@@ -156,19 +172,21 @@ int tile_edges(const int32_t *lc, const double *len, int c, uint16_t *codes, dou
             if (n == j || forced[(size_t)k * c + j] || j > n) continue;
             const int kv = back(n, j);
             if (forced[(size_t)kv * c + n]) continue;
-            const int32_t o = own[(size_t)j] <= own[(size_t)n] ? j : n;
+            const bool jr = j >= T, nr = n >= T;
+            const int32_t o = jr != nr ? (jr ? j : n) : (own[(size_t)j] <= own[(size_t)n] ? j : n);
             ++own[(size_t)o];
             inc[(size_t)j].push_back((int32_t)E.size());
             inc[(size_t)n].push_back((int32_t)E.size());
             E.push_back(Edge{j, n, (int8_t)k, (int8_t)kv, o});
         }
-    int kt = 4;
+    int kr = kLenFull;
+    auto cap = [&](int32_t v) { return v < T ? kLenFull : kr; };
     std::vector<int32_t> prev((size_t)c);
     std::vector<int32_t> queue;
     for (int32_t s = 0; s < c; ++s) {
-        while (own[(size_t)s] > kt) {
+        while (own[(size_t)s] > cap(s)) {
             // breadth-first search over owned edges (x owns e: e can move to the
-            // other end) for a vertex owning fewer than kt
+            // other end) for a vertex holding fewer than its cap
             std::fill(prev.begin(), prev.end(), -2);
             prev[(size_t)s] = -1;
             queue.assign(1, s);
@@ -180,7 +198,7 @@ int tile_edges(const int32_t *lc, const double *len, int c, uint16_t *codes, dou
                     const int32_t y = E[(size_t)e].u == x ? E[(size_t)e].v : E[(size_t)e].u;
                     if (prev[(size_t)y] != -2) continue;
                     prev[(size_t)y] = e;
-                    if (own[(size_t)y] < kt) {
+                    if (own[(size_t)y] < cap(y)) {
                         found = y;
                         break;
                     }
@@ -188,7 +206,8 @@ int tile_edges(const int32_t *lc, const double *len, int c, uint16_t *codes, dou
                 }
             }
             if (found < 0) {
-                ++kt;
+                if (s < T) return -1;
+                ++kr;
                 continue;
             }
             for (int32_t y = found; y != s;) {
@@ -201,7 +220,7 @@ int tile_edges(const int32_t *lc, const double *len, int c, uint16_t *codes, dou
             ++own[(size_t)found];
         }
     }
-    kt = 1;
+    int kt = 1;
     for (int32_t j = 0; j < c; ++j) kt = std::max(kt, own[(size_t)j]);
     if (!codes) return kt;
     // columns in slot order of each owner
@@ -213,13 +232,13 @@ int tile_edges(const int32_t *lc, const double *len, int c, uint16_t *codes, dou
             owns[(size_t)e.kv * c + e.v] = 1;
     }
     std::vector<int8_t> colof((size_t)6 * c, 0);
-    std::fill(lens, lens + (size_t)kt * c, 0.0);
+    std::fill(lens, lens + len_block(kt, c, T), 0.0);
     for (int32_t j = 0; j < c; ++j) {
         int m = 0;
         for (int k = 0; k < 6; ++k)
             if (owns[(size_t)k * c + j]) {
                 colof[(size_t)k * c + j] = (int8_t)m;
-                lens[(size_t)m * c + j] = len[(size_t)k * c + j];
+                lens[len_index(m, j, c, T)] = len[(size_t)k * c + j];
                 ++m;
             }
     }
@@ -338,14 +357,20 @@ int tile_plan_build(const int32_t *col, const double *elen, int64_t V, int K, vo
         std::vector<double> ln;
         for (int64_t i = t0; i < t1; ++i) {
             const int32_t c = local_table(*tiles[(size_t)i], flat, lc, ln);
-            kts[(size_t)i] = tile_edges(lc.data(), ln.data(), c, nullptr, nullptr);
+            kts[(size_t)i] = tile_edges(lc.data(), ln.data(), c, tiles[(size_t)i]->T, nullptr,
+                                        nullptr);
         }
     }, 64);
     if (bad.load()) return set_error(NX_EINVAL, "tile plan: a neighbour fell outside its tile's rings");
     int64_t nlen = 0;
     for (int64_t i = 0; i < nt; ++i) {
         const Tile &t = *tiles[(size_t)i];
-        nlen += (int64_t)kts[(size_t)i] * (t.T + t.ring_total() - (int64_t)t.r.back().size());
+        if (kts[(size_t)i] < 0)
+            return set_error(NX_ERANGE,
+                             "tile plan: a tile's own vertex must hold more than %d edge lengths "
+                             "(a neighbour table that is not symmetric)", kLenFull);
+        nlen += len_block(kts[(size_t)i],
+                          (int)(t.T + t.ring_total() - (int64_t)t.r.back().size()), t.T);
     }
 
     nx_tile_plan q;
@@ -399,7 +424,7 @@ int tile_plan_build(const int32_t *col, const double *elen, int64_t V, int K, vo
         cs[i] = n - (int32_t)t.r.back().size();
         rbase[i + 1] = rbase[i] + t.ring_total();
         cbase[i + 1] = cbase[i] + cs[i];
-        ebase[i + 1] = ebase[i] + (int64_t)kts[(size_t)i] * cs[i];
+        ebase[i + 1] = ebase[i] + len_block(kts[(size_t)i], cs[i], t.T);
     }
     start[nt] = (int32_t)V;
     parallel_for(nt, [&](int64_t t0, int64_t t1) {
@@ -409,7 +434,7 @@ int tile_plan_build(const int32_t *col, const double *elen, int64_t V, int K, vo
             const Tile &t = *tiles[(size_t)i];
             const int32_t c = local_table(t, flat, lc, ln);
             std::copy(flat.begin(), flat.end(), ring + rbase[i]);
-            tile_edges(lc.data(), ln.data(), c, lcol + 6 * cbase[i], lel + ebase[i]);
+            tile_edges(lc.data(), ln.data(), c, t.T, lcol + 6 * cbase[i], lel + ebase[i]);
         }
     }, 64);
     if (bad.load()) return set_error(NX_EINVAL, "tile plan: a neighbour fell outside its tile's rings");
