@@ -58,8 +58,10 @@ int nx_ctx_destroy(nx_ctx *ctx);
                                 launches (default 32)                        */
 #define NX_OPT_PAIR_KERNEL 2 /* two-step erosion kernel variant, 0..6 (0 =
                                 default); a tuning knob, results identical   */
-#define NX_OPT_MULTI_KERNEL 3 /* four-step erosion kernel variant, 0..3 (0 =
-                               * 1024 threads, neighbour data read per step) */
+#define NX_OPT_MULTI_KERNEL 3 /* four-step erosion kernel variant, 0..4 (0 =
+                               * 1024 threads, neighbour data read per step;
+                               * 4 = first-step data kept in registers, two
+                               * LDS buffers)                                */
 #define NX_OPT_CHAIN_PASSES 4 /* height chain form: 2 (default: one summary
                                * pass, one writing pass) or 5 (the pass per
                                * step form); results identical               */
@@ -326,7 +328,7 @@ typedef struct nx_tile_plan {
     int64_t off_start, off_cs, off_load, off_rbase, off_ring, off_cbase,
             off_lcol, off_lel;   /* byte offsets of the arrays in the image */
     int32_t steps;       /* steps per launch the plan serves: 2 or 4 */
-    int32_t max_step1;   /* largest tile + ring 1 (first-step vertices) */
+    int32_t max_step1;   /* largest first-step set: tile + rings 1..steps-1 */
     int64_t off_pref;    /* int32 [ntiles][steps + 1]: tile, tile + ring 1, ... */
     int64_t off_ebase;   /* int64 [ntiles + 1]: each tile's edge lengths in lel */
 } nx_tile_plan;

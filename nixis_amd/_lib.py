@@ -259,7 +259,7 @@ class Context:
         self.graph_chunk = int(steps)
 
     def set_multi_kernel(self, variant):
-        """Four-step erosion kernel variant 0..3 (0: default); same results."""
+        """Four-step erosion kernel variant 0..4 (0: default); same results."""
         check(lib().nx_ctx_set_option(self.handle, NX_OPT_MULTI_KERNEL, int(variant)))
         self.multi_kernel = int(variant)
 

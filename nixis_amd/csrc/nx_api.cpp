@@ -153,7 +153,7 @@ int nx_ctx_set_option(nx_ctx *ctx, int option, int64_t value)
         ctx->graph_chunk = (int)value;
         return NX_OK;
     case NX_OPT_MULTI_KERNEL:
-        if (value < 0 || value > 3) return set_error(NX_EINVAL, "multi-step kernel variant %lld outside 0..3", (long long)value);
+        if (value < 0 || value > 4) return set_error(NX_EINVAL, "multi-step kernel variant %lld outside 0..4", (long long)value);
         ctx->multi_variant = (int)value;
         return NX_OK;
     case NX_OPT_PAIR_KERNEL:

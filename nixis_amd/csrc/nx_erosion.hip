@@ -123,10 +123,31 @@ __global__ __launch_bounds__(THREADS) void k_erode_multi(const ErosionMeta *__re
                                                       tile, rain_mid, rain_next);
 }
 
+// Four steps per launch with each thread's first-step data in registers and
+// the steps alternating between two LDS buffers (nx_pair_dev.h); the plan
+// keeps tile + rings 1..3 within kQuadStepOne and tile + rings within
+// kQuadLoadMax.
+template <int RULE = 3>
+__global__ __launch_bounds__(1024) void k_erode_quad(const ErosionMeta *__restrict__ meta,
+                                                    double2 *__restrict__ hw0,
+                                                    double2 *__restrict__ hw1,
+                                                    double *__restrict__ s0,
+                                                    double *__restrict__ s1, TilePlanDev P, int j,
+                                                    double rain_mid, double rain_next)
+{
+    __shared__ double2 A[kQuadLoadMax];
+    __shared__ double2 B[kQuadStepOne];
+    const int64_t tile = xcd_tile(blockIdx.x, P.ntiles);
+    if (tile >= P.ntiles) return;
+    erode_tile_quad<RULE>(A, B, hw0, hw1, s0, s1, (meta->buf + j) & 1, P, tile, rain_mid,
+                          rain_next);
+}
+
 // NX_OPT_MULTI_KERNEL variants; erosion_iteration4 uses the first.
 static const PairVariant kMultiVariants[] = {
     {k_erode_multi<1024, false>, 1024}, {k_erode_multi<1024, true>, 1024},
     {k_erode_multi<512, false>, 512},   {k_erode_multi<512, true>, 512},
+    {k_erode_quad<3>, 1024},
 };
 static const PairVariant kMultiRule4 = {k_erode_multi<1024, false, 4>, 1024};
 
@@ -403,9 +424,11 @@ static int check_plan(const nx_tile_plan *plan, int64_t V)
     if (plan->max_tile > kTileMax || plan->max_load > lmax)
         return set_error(NX_EINVAL, "tile plan tiles exceed %d owned / %d loaded vertices",
                          kTileMax, lmax);
-    if (plan->steps == 2 && plan->max_step1 > kStepOneMax)
-        return set_error(NX_EINVAL, "tile plan tiles hold %d first-step vertices (at most %d)",
-                         plan->max_step1, kStepOneMax);
+    const int s1max = plan->steps == 2 ? kStepOneMax : kQuadStepOne;
+    if (plan->max_step1 > s1max || (plan->steps != 2 && plan->max_load > kQuadLoadMax))
+        return set_error(NX_EINVAL, "tile plan tiles hold %d first-step and %d loaded vertices "
+                         "(at most %d and %d)", plan->max_step1, plan->max_load, s1max,
+                         plan->steps == 2 ? kLoadMax : kQuadLoadMax);
     return NX_OK;
 }
 

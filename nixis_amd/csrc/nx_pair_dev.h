@@ -28,6 +28,13 @@ constexpr int kStepOneMax = 1024;
 constexpr int kTileMin = 32;        // smallest tile the planner cuts to
 constexpr int kMultiSteps = 4;      // steps per launch of the multi-step kernel
 constexpr int kLoadMaxMulti = 3072; // tile + its kMultiSteps rings
+// Four-step plans keep the first step's vertices (tile + rings 1..3) within
+// kQuadStepOne, two per thread of a 1024-thread block, and tile + rings
+// 1..4 within kQuadLoadMax, so that both LDS buffers of erode_tile_quad fit
+// two blocks per CU; they start from tiles of kQuadChunk vertices.
+constexpr int kQuadStepOne = 2048;
+constexpr int kQuadLoadMax = 2688;
+constexpr int kQuadChunk = 704;
 
 // Device view of a plan image (pointers into nx_tile_plan::blob).
 struct TilePlanDev {
@@ -307,6 +314,103 @@ __device__ __forceinline__ void erode_tile_multi(double2 *A, double2 *hw0, doubl
             __syncthreads();
         }
     }
+}
+
+// Four steps of tile `tile` (a four-step plan within kQuadStepOne and
+// kQuadLoadMax) by a 1024-thread block, reading buffer p and writing p ^ 1.
+// Each thread keeps up to two first-step vertices (li and li + 1024): their
+// sediment, neighbour indices (two to a word) and edge lengths are loaded
+// before the first barrier, while the field gathers are in flight, and stay
+// in registers for all four steps.  h and w of the tile and its four rings
+// are staged in A; step 1 (tile + rings 1..3) writes B, step 2 (tile + rings
+// 1..2) A, step 3 (tile + ring 1) B, and step 4 writes the tile: one barrier
+// between steps.  Every vertex is computed from the same inputs with the
+// same arithmetic as by one-step launches, so the results are bit for bit
+// theirs.
+template <int RULE = 3>
+__device__ __forceinline__ void erode_tile_quad(double2 *A, double2 *B, double2 *hw0, double2 *hw1,
+                                                double *s0, double *s1, int p,
+                                                const TilePlanDev &P, int64_t tile,
+                                                double rain_mid, double rain_next)
+{
+    constexpr int TH = 1024;
+    constexpr int kPer = kQuadStepOne / TH;
+    const auto hw_src = gmem(p ? hw1 : hw0);
+    const auto s_src = gmem(p ? s1 : s0);
+    const auto hw_dst = gmem(p ? hw0 : hw1);
+    const auto s_dst = gmem(p ? s0 : s1);
+    const int64_t a = gmem(P.start)[tile];
+    const int T = (int)(gmem(P.start)[tile + 1] - a);
+    const auto pref = gmem(P.pref) + tile * (kMultiSteps + 1);
+    const int c3 = pref[1], c2 = pref[2], c1 = pref[3], L = pref[4];
+    const auto ring = gmem(P.ring) + gmem(P.rbase)[tile];
+    const int64_t cb = gmem(P.cbase)[tile];
+    const auto lcol = gmem(P.lcol) + 6 * cb;
+    const auto lel = gmem(P.lel) + gmem(P.ebase)[tile];
+
+    for (int li = threadIdx.x; li < L; li += TH)
+        A[li] = hw_src[li < T ? a + li : ring[li - T]];
+    double sed[kPer];
+    int code[kPer][6];
+#pragma unroll
+    for (int m = 0; m < kPer; ++m) {
+        const int li = threadIdx.x + m * TH;
+        sed[m] = 0.0;
+        if (li < c1) {
+            sed[m] = s_src[li < T ? a + li : ring[li - T]];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) code[m][q] = lcol[q * c1 + li];
+        }
+    }
+    // the code loads go out together, before any length's address depends on one
+    asm volatile("" ::: "memory");
+    uint32_t nbp[kPer][3];
+    double len[kPer][6];
+#pragma unroll
+    for (int m = 0; m < kPer; ++m) {
+        const int li = threadIdx.x + m * TH;
+        if (li < c1) {
+#pragma unroll
+            for (int q = 0; q < 6; ++q) len[m][q] = slot_len(code[m][q], li, c1, T, lel);
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+                nbp[m][q] = (uint32_t)slot_nbr(code[m][2 * q]) |
+                            (uint32_t)slot_nbr(code[m][2 * q + 1]) << 16;
+        }
+    }
+    __syncthreads();
+
+    // One step over local vertices 0..n-1 from `src`; `emit` takes the results.
+    auto step = [&](const double2 *src, int n, double rain, auto &&emit) {
+#pragma unroll
+        for (int m = 0; m < kPer; ++m) {
+            const int li = threadIdx.x + m * TH;
+            if (li < n) {
+                const double2 me = src[li];
+                double S = sed[m], W = me.y;
+#pragma unroll
+                for (int q = 0; q < 6; ++q) {
+                    const int nb_i = (int)((nbp[m][q >> 1] >> (16 * (q & 1))) & 0xffffu);
+                    const double2 nb = src[nb_i];
+                    erode_accumulate<RULE>(me.x, nb.x, nb.y, len[m][q], S, W);
+                }
+                double ho, wo, so;
+                erode_finish<RULE>(me.x, me.y, sed[m], S, W, rain, ho, wo, so);
+                sed[m] = so;
+                emit(li, ho, wo, so);
+            }
+        }
+    };
+    step(A, c1, rain_mid, [&](int li, double h, double w, double) { B[li] = make_double2(h, w); });
+    __syncthreads();
+    step(B, c2, rain_mid, [&](int li, double h, double w, double) { A[li] = make_double2(h, w); });
+    __syncthreads();
+    step(A, c3, rain_mid, [&](int li, double h, double w, double) { B[li] = make_double2(h, w); });
+    __syncthreads();
+    step(B, T, rain_next, [&](int li, double h, double w, double s) {
+        hw_dst[a + li] = make_double2(h, w);
+        s_dst[a + li] = s;
+    });
 }
 #endif
 

@@ -5,10 +5,11 @@
 // outside the tile and the earlier rings.  K Jacobi steps of the tile need
 // the fields of the tile and K rings (erosion.py:197-279 reads only the
 // one-ring of each vertex): step j runs on the tile and rings 1..K-j.  Tiles
-// start at `chunk` vertices (by default kPairChunk for two steps, kTileMax
-// for more; an autotuner may pick another per mesh size) and are
-// halved until tile + rings fit the kernel's load limit and, for two steps,
-// tile + ring 1 fits kStepOneMax; a space-filling-curve order (compact tiles,
+// start at `chunk` vertices (by default kPairChunk for two steps,
+// kQuadChunk for four; an autotuner may pick another per mesh size) and are
+// halved until tile + rings fit the kernel's load limit and the first step's
+// vertices (tile + rings 1..K-1) its step-one limit (kStepOneMax, or
+// kQuadStepOne for four steps); a space-filling-curve order (compact tiles,
 // short perimeters) gives few, full tiles; a graph without locality fails
 // with NX_ERANGE.
 //
@@ -66,8 +67,8 @@ bool in_rings(const Tile &t, int m, int32_t n)
     return false;
 }
 
-// K rings of [a, a+T); false if tile + rings exceed `limit` or tile + ring 1
-// exceeds `step1_limit`.
+// K rings of [a, a+T); false if tile + rings exceed `limit` or the first
+// step's vertices (tile + rings 1..K-1) exceed `step1_limit`.
 bool rings(const int32_t *col, int64_t V, int64_t a, int32_t T, int K, int limit, int step1_limit,
            Tile &t)
 {
@@ -91,7 +92,7 @@ bool rings(const int32_t *col, int64_t V, int64_t a, int32_t T, int K, int limit
         }
         sort_unique(cur);
         total += (int64_t)cur.size();
-        if (total > limit || (m == 0 && total > step1_limit)) return false;
+        if (total > limit || (m == K - 2 && total > step1_limit)) return false;
     }
     return true;
 }
@@ -274,9 +275,9 @@ int tile_plan_build(const int32_t *col, const double *elen, int64_t V, int K, vo
     if (chunk_in != 0 && (chunk_in < kTileMin || chunk_in > kTileMax))
         return set_error(NX_EINVAL, "tile chunk %d outside %d..%d (0 for the default)", chunk_in,
                          kTileMin, kTileMax);
-    const int limit = K == 2 ? kLoadMax : kLoadMaxMulti;
-    const int step1_limit = K == 2 ? kStepOneMax : kLoadMaxMulti;
-    const int64_t chunk = chunk_in ? chunk_in : (K == 2 ? kPairChunk : kTileMax);
+    const int limit = K == 2 ? kLoadMax : kQuadLoadMax;
+    const int step1_limit = K == 2 ? kStepOneMax : kQuadStepOne;
+    const int64_t chunk = chunk_in ? chunk_in : (K == 2 ? kPairChunk : kQuadChunk);
     for (int64_t i = 0; i < 6 * V; ++i)
         if (col[i] < 0 || col[i] >= V) return set_error(NX_EINVAL, "col[%lld] = %d outside 0..V-1", (long long)i, col[i]);
 
@@ -309,7 +310,8 @@ int tile_plan_build(const int32_t *col, const double *elen, int64_t V, int K, vo
         ncs += t->T + t->ring_total() - (int64_t)t->r.back().size();
         max_tile = std::max(max_tile, t->T);
         max_load = std::max<int32_t>(max_load, t->T + t->ring_total());
-        max_step1 = std::max<int32_t>(max_step1, t->T + (int32_t)t->r[0].size());
+        max_step1 = std::max<int32_t>(
+            max_step1, (int32_t)(t->T + t->ring_total() - (int64_t)t->r.back().size()));
     }
     // Rings larger than the tiles themselves read more than K single steps
     // would: such a graph has no locality to exploit.

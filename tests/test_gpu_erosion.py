@@ -257,7 +257,7 @@ def test_four_step_plans_match_oracle(oracle, rule):
     g = erosion.ErosionGraph(pts, nb, steps=4)
     assert g.plan is not None and g.plan.steps == 4
     ref_step = oracle.erosion_iteration3 if rule == 3 else oracle.erosion_iteration4
-    for variant in range(4):
+    for variant in range(5):
         g_ctx = _lib.Context.get(g.device)
         g_ctx.set_multi_kernel(variant)
         for n in (1, 4, 5, 9):
@@ -277,24 +277,32 @@ def test_four_step_plans_match_oracle(oracle, rule):
 
 
 def test_four_step_launches_equal_two_step_and_single_at_scale():
-    """d=1000 (10,000,002 vertices), 23 steps with rain: four-step plan,
-    two-step plan and one step per launch give the same fields bit for bit."""
+    """d=1000 (10,000,002 vertices), 23 steps with rain: four-step plans (the
+    default four-step kernel and variant 4, first-step data in registers),
+    the two-step plan and one step per launch give the same fields bit for
+    bit."""
     pts, nb, h = _planet(1000, n_oct=8)
     outs = []
-    for kw in ({"steps": 4}, {"steps": 2}, {"tiled": False}):
-        g = erosion.ErosionGraph(pts, nb, **kw)
-        st = erosion.ErosionState(g.V)
-        st.load(torch.from_numpy(g.permute(h)).cuda())
-        st.run(g, 23, 0.3 / 320)
-        o = [torch.empty(g.V, dtype=torch.float64, device="cuda") for _ in range(3)]
-        st.store(*o)
-        st.ctx.sync_check()
-        outs.append([g.unpermute(t.cpu().numpy()) for t in o])
-        print("plan", kw, "done", flush=True)
-    for a, b in zip(outs[0], outs[1]):
-        assert np.array_equal(a, b)
-    for a, b in zip(outs[0], outs[2]):
-        assert np.array_equal(a, b)
+    ctx = None
+    try:
+        for kw, mv in (({"steps": 4}, 0), ({"steps": 4}, 4), ({"steps": 2}, 0), ({"tiled": False}, 0)):
+            g = erosion.ErosionGraph(pts, nb, **kw)
+            ctx = _lib.Context.get(g.device)
+            ctx.set_multi_kernel(mv)
+            st = erosion.ErosionState(g.V)
+            st.load(torch.from_numpy(g.permute(h)).cuda())
+            st.run(g, 23, 0.3 / 320)
+            o = [torch.empty(g.V, dtype=torch.float64, device="cuda") for _ in range(3)]
+            st.store(*o)
+            st.ctx.sync_check()
+            outs.append([g.unpermute(t.cpu().numpy()) for t in o])
+            print("plan", kw, "four-step variant", mv, "done", flush=True)
+    finally:
+        if ctx is not None:
+            ctx.set_multi_kernel(0)
+    for other in outs[1:]:
+        for a, b in zip(outs[0], other):
+            assert np.array_equal(a, b)
 
 
 def test_graph_cache_equal_v_meshes_back_to_back(oracle):
