@@ -65,16 +65,19 @@ def parse():
     return ap.parse_args()
 
 
-def profiled_traffic(kind, V):
+def profiled_traffic(kind, V, steps_per_launch=None):
     """HBM bytes per launch of the dominant kernel, from the committed
     rocprofv3 FETCH_SIZE/WRITE_SIZE summary (profiles/traffic.json, copy-
-    calibrated), when it was measured on this workload size; else None."""
+    calibrated), when it was measured on this workload size (and, for
+    erosion, the same steps per launch); else None."""
     try:
         with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
             e = json.load(f).get(kind)
     except (OSError, ValueError):
         return None
     if not e or e.get("V") != V:
+        return None
+    if steps_per_launch is not None and e.get("steps_per_launch") != steps_per_launch:
         return None
     return int(e["bytes_per_launch"])
 
@@ -192,21 +195,23 @@ def bench_erosion(args):
     from nixis_amd import autotune
     t0 = time.perf_counter()
     order = util.sfc_order(pts)
-    graph = erosion.ErosionGraph(pts, nb, order=order)      # SFC order + tile plan
+    graph = erosion.ErosionGraph(pts, nb, order=order)      # SFC order + 4-step tile plan
     hd = graph.permute_device(h)
     log(f"[erosion] SFC order, ELL table and tile plan in {time.perf_counter() - t0:.1f}s")
-    # tile chunk, kernel variant and graph chunk from the autotuner
+    # steps per launch, tile chunk, kernel variant and graph chunk from the autotuner
     tuning = autotune.lookup_erosion(V)
     tuned_now = tuning is None
     if tuned_now:
         t0 = time.perf_counter()
         tuning, table = autotune.tune_erosion(
-            lambda c: graph if c == 0 else erosion.ErosionGraph(pts, nb, order=order, chunk=c),
+            lambda c, k: graph if (c, k) == (0, graph.steps)
+            else erosion.ErosionGraph(pts, nb, order=order, chunk=c, steps=k),
             heights=hd, steps=20, reps=2)
-        log(f"[erosion] autotuned in {time.perf_counter() - t0:.1f}s: chunk {tuning.chunk} "
-            f"variant {tuning.variant} graph chunk {tuning.graph_chunk}; "
-            f"{len(table)} candidates, {min(table)[0]:.1f}..{max(table)[0]:.1f} us/step")
-    if tuning.chunk != graph.chunk:
+        log(f"[erosion] autotuned in {time.perf_counter() - t0:.1f}s: {tuning.steps} steps per "
+            f"launch, chunk {tuning.chunk} variant {tuning.variant} graph chunk "
+            f"{tuning.graph_chunk}; {len(table)} candidates, "
+            f"{min(table)[0]:.1f}..{max(table)[0]:.1f} us/step")
+    if (tuning.chunk, tuning.steps) != (graph.chunk, graph.steps):
         graph = erosion.ErosionGraph(pts, nb, order=order, **tuning.graph_kwargs())
     else:
         graph.variant, graph.graph_chunk = tuning.variant, tuning.graph_chunk
@@ -243,7 +248,7 @@ def bench_erosion(args):
     gvi = V * iters / (ms * 1e-3) / 1e9
     B = 28 * V + 8 * nnz
     per_step_ms = ms / iters
-    spl = 2 if plan is not None else 1
+    spl = plan.steps if plan is not None else 1
     gbs = B / (per_step_ms * 1e-3) / 1e9
     log(f"[erosion] {iters} iters in {ms:.2f} ms ({per_step_ms * 1e3:.1f} us/iter), "
         f"{gvi:.2f} Gvi/s, {gbs:.0f} GB/s algorithmic")
@@ -255,11 +260,11 @@ def bench_erosion(args):
         "us_per_iter": round(per_step_ms * 1e3, 2),
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
-                     "traffic": profiled_traffic("erosion", V) if spl == 2 else None,
+                     "traffic": profiled_traffic("erosion", V, spl),
                      "algorithmic_bytes_per_launch": spl * B, "steps_per_launch": spl,
                      "plan_stream_bytes_per_launch": getattr(graph, "stream_bytes_per_launch", None),
                      "launch_us": round(spl * per_step_ms * 1e3, 2)},
-        "tuning": {"chunk": tuning.chunk, "variant": tuning.variant,
+        "tuning": {"steps_per_launch": spl, "chunk": tuning.chunk, "variant": tuning.variant,
                    "graph_chunk": tuning.graph_chunk,
                    "tiles": plan.ntiles if plan is not None else 0,
                    "source": "autotuned now" if tuned_now else "autotune cache"},

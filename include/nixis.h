@@ -59,9 +59,9 @@ int nx_ctx_destroy(nx_ctx *ctx);
 #define NX_OPT_PAIR_KERNEL 2 /* two-step erosion kernel variant, 0..6 (0 =
                                 default); a tuning knob, results identical   */
 #define NX_OPT_MULTI_KERNEL 3 /* four-step erosion kernel variant, 0..4 (0 =
-                               * 1024 threads, neighbour data read per step;
-                               * 4 = first-step data kept in registers, two
-                               * LDS buffers)                                */
+                               * default: first-step data kept in registers,
+                               * two LDS buffers; 1..4 = slot tables in LDS,
+                               * 1024 or 512 threads)                       */
 #define NX_OPT_CHAIN_PASSES 4 /* height chain form: 2 (default: one summary
                                * pass, one writing pass) or 5 (the pass per
                                * step form); results identical               */
@@ -356,7 +356,8 @@ int nx_tile_plan_build_steps(const int32_t *col, const double *elen, int64_t V,
                              nx_tile_plan *plan);
 
 /* nx_tile_plan_build_steps with the starting tile size `chunk` (kTileMin =
- * 32 .. 1024 vertices; 0 for the default, 768 for 2 steps and 1024 for 4).
+ * 32 .. 1024 vertices for 2 steps, .. 2048 for 4; 0 for the default, 768 for
+ * 2 steps and 1152 for 4).
  * Tiles are halved from there until their rings fit, so the chunk trades
  * ring overhead against the number of blocks per launch; the erosion
  * autotuner picks it per mesh size.  The results of erosion do not depend on

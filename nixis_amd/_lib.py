@@ -274,21 +274,22 @@ class Context:
         self.chain_passes = int(passes)
 
     @contextlib.contextmanager
-    def options(self, graph_chunk=None, pair_kernel=None):
+    def options(self, graph_chunk=None, pair_kernel=None, multi_kernel=None):
         """Set the given erosion options for the body of a with-block and
         restore the previous values after it (None leaves one alone)."""
-        old = (self.graph_chunk, self.pair_kernel)
+        setters = (("graph_chunk", self.set_graph_chunk, graph_chunk),
+                   ("pair_kernel", self.set_pair_kernel, pair_kernel),
+                   ("multi_kernel", self.set_multi_kernel, multi_kernel))
+        old = {name: getattr(self, name) for name, _, _ in setters}
         try:
-            if graph_chunk is not None and graph_chunk != old[0]:
-                self.set_graph_chunk(graph_chunk)
-            if pair_kernel is not None and pair_kernel != old[1]:
-                self.set_pair_kernel(pair_kernel)
+            for name, setter, value in setters:
+                if value is not None and value != old[name]:
+                    setter(value)
             yield self
         finally:
-            if self.graph_chunk != old[0]:
-                self.set_graph_chunk(old[0])
-            if self.pair_kernel != old[1]:
-                self.set_pair_kernel(old[1])
+            for name, setter, _ in setters:
+                if getattr(self, name) != old[name]:
+                    setter(old[name])
 
 
 def stream_ptr(stream=None):

@@ -4,12 +4,12 @@ Noise (batched form): tile size x blocks per CU of the persistent launch,
 keyed by the batch's shape class (log2 of the total vertex count and the
 mean octave count).
 
-Erosion: the tile plan's starting chunk (tile size before halving), the
-two-step kernel variant (block size and whether neighbour data stays in
-registers) and the steps per captured hipGraph, keyed by the mesh's size
-class (log2 of its vertex count).  A coordinate search times the variants
-on the default plan, then the chunks with the best variant, then the graph
-chunks; none of the choices changes the results.
+Erosion: the tile plan's steps per launch (four or two), its starting
+chunk (tile size before halving), the kernel variant and the steps per
+captured hipGraph, keyed by the mesh's size class (log2 of its vertex
+count).  For each step count a coordinate search times the variants on the
+default plan, then the chunks with the best variant; the graph chunks are
+timed for the best of all.  None of the choices changes the results.
 
 Candidates are timed with HIP events on the launch stream; the best one is
 stored in a JSON file, so later work of the same class reuses it without
@@ -88,49 +88,66 @@ def tune(runner, runs, reps=3, path=CACHE_PATH, tiles=TILES, bpcs=BLOCKS_PER_CU,
 # ---- erosion ---------------------------------------------------------------
 
 PAIR_VARIANTS = (0, 1, 2, 4, 5, 6)     # nx_ctx option NX_OPT_PAIR_KERNEL (3 repeats 0)
-CHUNKS = (512, 640, 704, 768, 832, 896)
+MULTI_VARIANTS = (0,)                  # NX_OPT_MULTI_KERNEL (1..4 are slower everywhere measured)
+VARIANTS = {4: MULTI_VARIANTS, 2: PAIR_VARIANTS}
+CHUNKS = {4: (896, 1024, 1088, 1216, 1280), 2: (512, 640, 704, 768, 832, 896)}
+STEPS = (4, 2)
 GRAPH_CHUNKS = (8, 16, 32, 64)
 
 
 @dataclass
 class ErosionTuning:
     chunk: int = 0          # tile plan starting chunk (0: the library default)
-    variant: int = 0        # two-step kernel variant
+    variant: int = 0        # kernel variant for the plan's steps per launch
     graph_chunk: int = 32   # erosion steps per captured graph
+    steps: int = 4          # steps per launch of the tile plan (4 or 2)
 
     def graph_kwargs(self):
-        return {"chunk": self.chunk, "variant": self.variant, "graph_chunk": self.graph_chunk}
+        return {"chunk": self.chunk, "variant": self.variant, "graph_chunk": self.graph_chunk,
+                "steps": self.steps}
 
 
 def erosion_key(V):
     return f"erosion:V2^{int(math.log2(max(int(V), 1)))}"
 
 
-def lookup_erosion(V, path=CACHE_PATH):
-    """Cached erosion tuning for meshes of V vertices' size class, or None."""
+def _tuning(e):
+    return ErosionTuning(e["chunk"], e["variant"], e["graph_chunk"], e.get("steps", 2))
+
+
+def lookup_erosion(V, path=CACHE_PATH, steps=None):
+    """Cached erosion tuning for meshes of V vertices' size class, or None;
+    with steps=2, the best two-step tuning (what the multi-planet launches,
+    which run two steps per launch, can use)."""
     e = load_cache(path).get(erosion_key(V))
-    return ErosionTuning(e["chunk"], e["variant"], e["graph_chunk"]) if e else None
+    if not e:
+        return None
+    if steps is not None and e.get("steps", 2) != steps:
+        e = e.get(f"steps{steps}")
+    return _tuning(e) if e else None
 
 
 def tune_erosion(make_graph, heights=None, steps=20, reps=3, path=CACHE_PATH, persist=True,
-                 variants=PAIR_VARIANTS, chunks=CHUNKS, graph_chunks=GRAPH_CHUNKS, rain=1.0):
-    """Tune erosion for one mesh.  make_graph(chunk) returns the mesh's
-    ErosionGraph with a tile plan cut from that starting chunk (0: default);
-    heights (device float64 [V] in the graph's numbering; default: a fixed
-    pseudo-random field) start every timed run.  Returns the best
-    ErosionTuning and the table of (us per step, chunk, variant, graph
-    chunk) timings."""
+                 variants=None, chunks=None, graph_chunks=GRAPH_CHUNKS, rain=1.0,
+                 step_options=STEPS):
+    """Tune erosion for one mesh.  make_graph(chunk, steps_per_launch)
+    returns the mesh's ErosionGraph with a tile plan of that many steps per
+    launch cut from that starting chunk (0: default); heights (device
+    float64 [V] in the graph's numbering; default: a fixed pseudo-random
+    field) start every timed run.  For each step count in step_options, the
+    kernel variants (variants[steps], default VARIANTS) are timed on the
+    default plan, then the chunks (chunks[steps], default CHUNKS) with the
+    best variant; the graph chunks are timed for the best of all.  Returns
+    the best ErosionTuning and the table of (us per step, steps, chunk,
+    variant, graph chunk) timings; the cache keeps the best and, when it is
+    not two-step, the best two-step tuning beside it."""
     import torch
     from . import erosion as _er
-    g = make_graph(0)
-    V = g.V
-    dev = g.device
-    if heights is None:
-        gen = torch.Generator(device="cpu").manual_seed(20261204)
-        heights = (torch.rand(V, generator=gen, dtype=torch.float64) * 18000 - 9000).to(dev)
-    st = _er.ErosionState(V, dev)
-    stream = torch.cuda.current_stream(dev)
+    variants = {**VARIANTS, **(variants or {})}
+    chunks = {**CHUNKS, **(chunks or {})}
     table = []
+    st = None
+    stream = None
 
     def time_it(graph, variant, gchunk):
         graph.variant, graph.graph_chunk = variant, gchunk
@@ -146,29 +163,67 @@ def tune_erosion(make_graph, heights=None, steps=20, reps=3, path=CACHE_PATH, pe
             b.record(stream)
             b.synchronize()
             best = min(best, a.elapsed_time(b) * 1e3 / steps)
-        table.append((round(best, 3), graph.chunk, variant, gchunk))
+        table.append((round(best, 3), graph.steps if graph.plan is not None else 1,
+                      graph.chunk, variant, gchunk))
         return best
 
-    if g.plan is None:      # one step per launch: only the graph chunk matters
-        variants, chunks = (0,), ()
     gchunk0 = 32 if 32 in graph_chunks or not graph_chunks else graph_chunks[0]
-    v_best = min(variants, key=lambda v: time_it(g, v, gchunk0))
-    c_best, g_best = 0, g
-    t_best = min(t for t, c, v, _ in table if v == v_best and c == g.chunk)
-    for c in chunks:
-        gc = make_graph(c)
-        if gc.plan is None:
+    per_steps = {}          # steps -> (us, ErosionTuning, graph)
+    for K in step_options:
+        g = make_graph(0, K)
+        if st is None:
+            V, dev = g.V, g.device
+            if heights is None:
+                gen = torch.Generator(device="cpu").manual_seed(20261204)
+                heights = (torch.rand(V, generator=gen, dtype=torch.float64) * 18000
+                           - 9000).to(dev)
+            st = _er.ErosionState(V, dev)
+            stream = torch.cuda.current_stream(dev)
+        if g.plan is None:      # one step per launch: only the graph chunk matters
+            t = time_it(g, 0, gchunk0)
+            per_steps.setdefault(1, (t, ErosionTuning(0, 0, gchunk0, K), g))
+            break
+        if g.steps != K:        # no plan of K steps for this mesh
             continue
-        t = time_it(gc, v_best, gchunk0)
-        if t < t_best:
-            c_best, g_best, t_best = c, gc, t
-    gc_best = min(graph_chunks, key=lambda k: time_it(g_best, v_best, k)) if graph_chunks else gchunk0
-    best = ErosionTuning(c_best, v_best, gc_best)
+        v_best = min(variants[K], key=lambda v: time_it(g, v, gchunk0))
+        t_best = min(t for t, k, c, v, _ in table if k == K and v == v_best and c == g.chunk)
+        c_best, g_best = 0, g
+        for c in chunks[K]:
+            gc = make_graph(c, K)
+            if gc.plan is None or gc.steps != K:
+                continue
+            t = time_it(gc, v_best, gchunk0)
+            if t < t_best:
+                c_best, g_best, t_best = c, gc, t
+        per_steps[K] = (t_best, ErosionTuning(c_best, v_best, gchunk0, K), g_best)
+    if not per_steps:
+        raise ValueError(f"no tile plan of {step_options} steps per launch for this mesh")
+    _, best, g_best = min(per_steps.values(), key=lambda r: r[0])
+    if graph_chunks:
+        best.graph_chunk = min(graph_chunks, key=lambda k: time_it(g_best, best.variant, k))
     if persist:
+        entry = {"chunk": best.chunk, "variant": best.variant, "graph_chunk": best.graph_chunk,
+                 "steps": best.steps, "us_per_step": min(t for t, *_ in table)}
+        if best.steps != 2 and 2 in per_steps:
+            t2, b2, _ = per_steps[2]
+            entry["steps2"] = {"chunk": b2.chunk, "variant": b2.variant,
+                               "graph_chunk": best.graph_chunk, "steps": 2,
+                               "us_per_step": round(t2, 3)}
         with _lock:
             cache = load_cache(path)
-            cache[erosion_key(V)] = {"chunk": best.chunk, "variant": best.variant,
-                                     "graph_chunk": best.graph_chunk,
-                                     "us_per_step": min(t for t, *_ in table)}
+            key = erosion_key(V)
+            cache[key] = merge_erosion_entry(cache.get(key), entry, per_steps)
             save_cache(cache, path)
     return best, table
+
+
+def merge_erosion_entry(prev, entry, searched):
+    """The cache entry to store after a search of the step counts in
+    `searched` found `entry`: a cached tuning of a step count not searched
+    now that is at least as fast stays first, with the new one beside it
+    (a search of two-step plans alone, for the multi-planet launches, keeps
+    a faster four-step tuning of the single-planet form)."""
+    if prev and prev.get("steps", 2) not in searched \
+            and prev.get("us_per_step", math.inf) <= entry["us_per_step"]:
+        return dict(prev, **{f"steps{entry['steps']}": entry})
+    return entry

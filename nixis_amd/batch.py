@@ -149,20 +149,22 @@ class MeshCache:
         return v
 
     def graph(self, d, tuning=None):
-        """The erosion graph of subdivision d, its tile plan cut with the
-        autotuned settings for its size class when the tuning cache has them
-        (`tuning` overrides; the results are the same either way)."""
+        """The erosion graph of subdivision d, its two-step tile plan cut
+        with the autotuned two-step settings for its size class when the
+        tuning cache has them (`tuning` overrides; the results are the same
+        either way).  The multi-planet launches run two steps per launch."""
         g = self._graphs.get(d)
         if g is None:
             if tuning is None:
                 from . import autotune
-                tuning = autotune.lookup_erosion(10 * d * d + 2)
+                tuning = autotune.lookup_erosion(10 * d * d + 2, steps=2)
             kw = tuning.graph_kwargs() if tuning is not None else {}
             g = self._build_graph(d, **kw)
             self._graphs[d] = g
         return g
 
     def _build_graph(self, d, **kw):
+        kw["steps"] = 2
         if self.on_device:
             pts, cells = _util.create_mesh_device(d, self.device)
             adj = _util.sort_adjacency_device(_util.build_adjacency_device(cells, pts.shape[0]))
@@ -175,7 +177,8 @@ class MeshCache:
         """Autotune erosion for subdivision d's size class (autotune.tune_erosion)
         and keep the tuned graph; returns (tuning, table)."""
         from . import autotune
-        best, table = autotune.tune_erosion(lambda c: self._build_graph(d, chunk=c), **kw)
+        kw.setdefault("step_options", (2,))
+        best, table = autotune.tune_erosion(lambda c, k: self._build_graph(d, chunk=c), **kw)
         self._graphs[d] = self._build_graph(d, **best.graph_kwargs())
         return best, table
 

@@ -126,8 +126,8 @@ __global__ __launch_bounds__(THREADS) void k_erode_multi(const ErosionMeta *__re
 // Four steps per launch with each thread's first-step data in registers and
 // the steps alternating between two LDS buffers (nx_pair_dev.h); the plan
 // keeps tile + rings 1..3 within kQuadStepOne and tile + rings within
-// kQuadLoadMax.
-template <int RULE = 3>
+// kQuadLoadMax.  NS = 2 or 3: the last two or three of those steps.
+template <int RULE = 3, int NS = 4>
 __global__ __launch_bounds__(1024) void k_erode_quad(const ErosionMeta *__restrict__ meta,
                                                     double2 *__restrict__ hw0,
                                                     double2 *__restrict__ hw1,
@@ -139,17 +139,25 @@ __global__ __launch_bounds__(1024) void k_erode_quad(const ErosionMeta *__restri
     __shared__ double2 B[kQuadStepOne];
     const int64_t tile = xcd_tile(blockIdx.x, P.ntiles);
     if (tile >= P.ntiles) return;
-    erode_tile_quad<RULE>(A, B, hw0, hw1, s0, s1, (meta->buf + j) & 1, P, tile, rain_mid,
-                          rain_next);
+    erode_tile_quad<RULE, NS>(A, B, hw0, hw1, s0, s1, (meta->buf + j) & 1, P, tile, rain_mid,
+                              rain_next);
 }
 
-// NX_OPT_MULTI_KERNEL variants; erosion_iteration4 uses the first.
+// NX_OPT_MULTI_KERNEL variants: 0 keeps each thread's first-step data in
+// registers (the default); 1..4 stage the slot tables in LDS and keep
+// neighbour data in registers (KEEP) or not, in 1024- or 512-thread blocks.
+// erosion_iteration4 uses the first.  A run's last two or three steps on a
+// four-step plan take the tail launches.
 static const PairVariant kMultiVariants[] = {
+    {k_erode_quad<3>, 1024},
     {k_erode_multi<1024, false>, 1024}, {k_erode_multi<1024, true>, 1024},
     {k_erode_multi<512, false>, 512},   {k_erode_multi<512, true>, 512},
-    {k_erode_quad<3>, 1024},
 };
-static const PairVariant kMultiRule4 = {k_erode_multi<1024, false, 4>, 1024};
+static const PairVariant kMultiRule4 = {k_erode_quad<4>, 1024};
+static const PairVariant kQuadTails[2][2] = {   // [rule 4][steps 3]
+    {{k_erode_quad<3, 2>, 1024}, {k_erode_quad<3, 3>, 1024}},
+    {{k_erode_quad<4, 2>, 1024}, {k_erode_quad<4, 3>, 1024}},
+};
 
 // erosion.py:76-99 (erosion_iteration1): w[i] = r[i] + amt with amt moved
 // by 0.0005 towards each higher neighbour and away from each lower one, in
@@ -306,8 +314,10 @@ int erosion_load(nx_ctx *ctx, void *state, int64_t V, const double *h, const dou
 
 // Launch n consecutive steps (the launches read the step counter and the
 // current buffer from the workspace) and advance the workspace by n steps.
-// With a tile plan, steps go two per launch and an odd last step runs alone.
-// The last step's rain for a following step is `rain` unless `last`.
+// With a two-step plan, steps go two per launch and an odd last step runs
+// alone; with a four-step plan, four per launch and a last two or three in
+// one tail launch.  The last step's rain for a following step is `rain`
+// unless `last`.
 static void launch_steps(const ErosionBufs &b, int64_t V, const int32_t *col, const double *elen,
                          const nx_tile_plan *plan, int variant, int mvariant, int rule, int n,
                          double rain, bool last, hipStream_t stream)
@@ -323,6 +333,14 @@ static void launch_steps(const ErosionBufs &b, int64_t V, const int32_t *col, co
             const double rain_next = (last && k + kMultiSteps == n) ? 0.0 : rain;
             hipLaunchKernelGGL(mv.fn, dim3((unsigned)tgrid), dim3(mv.threads), 0, stream, b.meta,
                                b.hw[0], b.hw[1], b.s[0], b.s[1], P, j, rain, rain_next);
+        }
+        if (n - k >= 2) {
+            const PairVariant &tv = kQuadTails[rule == 4][n - k - 2];
+            const double rain_next = last ? 0.0 : rain;
+            hipLaunchKernelGGL(tv.fn, dim3((unsigned)tgrid), dim3(tv.threads), 0, stream, b.meta,
+                               b.hw[0], b.hw[1], b.s[0], b.s[1], P, j, rain, rain_next);
+            k = n;
+            ++j;
         }
     } else if (plan) {
         const TilePlanDev P = plan_dev(*plan);
@@ -421,9 +439,10 @@ static int check_plan(const nx_tile_plan *plan, int64_t V)
         return set_error(NX_EINVAL, "tile plan for %d steps per launch (2 or %d)", plan->steps,
                          kMultiSteps);
     const int lmax = plan->steps == 2 ? kLoadMax : kLoadMaxMulti;
-    if (plan->max_tile > kTileMax || plan->max_load > lmax)
-        return set_error(NX_EINVAL, "tile plan tiles exceed %d owned / %d loaded vertices",
-                         kTileMax, lmax);
+    const int tmax = plan->steps == 2 ? kTileMax : kQuadStepOne;
+    if (plan->max_tile > tmax || plan->max_load > lmax)
+        return set_error(NX_EINVAL, "tile plan tiles exceed %d owned / %d loaded vertices", tmax,
+                         lmax);
     const int s1max = plan->steps == 2 ? kStepOneMax : kQuadStepOne;
     if (plan->max_step1 > s1max || (plan->steps != 2 && plan->max_load > kQuadLoadMax))
         return set_error(NX_EINVAL, "tile plan tiles hold %d first-step and %d loaded vertices "

@@ -34,7 +34,7 @@ constexpr int kLoadMaxMulti = 3072; // tile + its kMultiSteps rings
 // two blocks per CU; they start from tiles of kQuadChunk vertices.
 constexpr int kQuadStepOne = 2048;
 constexpr int kQuadLoadMax = 2688;
-constexpr int kQuadChunk = 704;
+constexpr int kQuadChunk = 1152;
 
 // Device view of a plan image (pointers into nx_tile_plan::blob).
 struct TilePlanDev {
@@ -326,8 +326,11 @@ __device__ __forceinline__ void erode_tile_multi(double2 *A, double2 *hw0, doubl
 // 1..2) A, step 3 (tile + ring 1) B, and step 4 writes the tile: one barrier
 // between steps.  Every vertex is computed from the same inputs with the
 // same arithmetic as by one-step launches, so the results are bit for bit
-// theirs.
-template <int RULE = 3>
+// theirs.  NS = 3 or 2 runs the last three or two of those steps (a run's
+// tail of steps that does not fill a four-step launch): the first step then
+// covers tile + rings 1..2 or tile + ring 1, and only the vertices it reads
+// are staged.
+template <int RULE = 3, int NS = 4>
 __device__ __forceinline__ void erode_tile_quad(double2 *A, double2 *B, double2 *hw0, double2 *hw1,
                                                 double *s0, double *s1, int p,
                                                 const TilePlanDev &P, int64_t tile,
@@ -342,13 +345,18 @@ __device__ __forceinline__ void erode_tile_quad(double2 *A, double2 *B, double2 
     const int64_t a = gmem(P.start)[tile];
     const int T = (int)(gmem(P.start)[tile + 1] - a);
     const auto pref = gmem(P.pref) + tile * (kMultiSteps + 1);
+    static_assert(NS >= 2 && NS <= kMultiSteps, "2..4 steps");
     const int c3 = pref[1], c2 = pref[2], c1 = pref[3], L = pref[4];
+    // first-step set and staged set of this launch; slot codes and lengths
+    // keep the plan's layout over the c1 first-step vertices
+    const int cf = NS == 4 ? c1 : NS == 3 ? c2 : c3;
+    const int cl = NS == 4 ? L : NS == 3 ? c1 : c2;
     const auto ring = gmem(P.ring) + gmem(P.rbase)[tile];
     const int64_t cb = gmem(P.cbase)[tile];
     const auto lcol = gmem(P.lcol) + 6 * cb;
     const auto lel = gmem(P.lel) + gmem(P.ebase)[tile];
 
-    for (int li = threadIdx.x; li < L; li += TH)
+    for (int li = threadIdx.x; li < cl; li += TH)
         A[li] = hw_src[li < T ? a + li : ring[li - T]];
     double sed[kPer];
     int code[kPer][6];
@@ -356,7 +364,7 @@ __device__ __forceinline__ void erode_tile_quad(double2 *A, double2 *B, double2 
     for (int m = 0; m < kPer; ++m) {
         const int li = threadIdx.x + m * TH;
         sed[m] = 0.0;
-        if (li < c1) {
+        if (li < cf) {
             sed[m] = s_src[li < T ? a + li : ring[li - T]];
 #pragma unroll
             for (int q = 0; q < 6; ++q) code[m][q] = lcol[q * c1 + li];
@@ -369,7 +377,7 @@ __device__ __forceinline__ void erode_tile_quad(double2 *A, double2 *B, double2 
 #pragma unroll
     for (int m = 0; m < kPer; ++m) {
         const int li = threadIdx.x + m * TH;
-        if (li < c1) {
+        if (li < cf) {
 #pragma unroll
             for (int q = 0; q < 6; ++q) len[m][q] = slot_len(code[m][q], li, c1, T, lel);
 #pragma unroll
@@ -401,16 +409,31 @@ __device__ __forceinline__ void erode_tile_quad(double2 *A, double2 *B, double2 
             }
         }
     };
-    step(A, c1, rain_mid, [&](int li, double h, double w, double) { B[li] = make_double2(h, w); });
-    __syncthreads();
-    step(B, c2, rain_mid, [&](int li, double h, double w, double) { A[li] = make_double2(h, w); });
-    __syncthreads();
-    step(A, c3, rain_mid, [&](int li, double h, double w, double) { B[li] = make_double2(h, w); });
-    __syncthreads();
-    step(B, T, rain_next, [&](int li, double h, double w, double s) {
+    auto to_a = [&](int li, double h, double w, double) { A[li] = make_double2(h, w); };
+    auto to_b = [&](int li, double h, double w, double) { B[li] = make_double2(h, w); };
+    auto out = [&](int li, double h, double w, double s) {
         hw_dst[a + li] = make_double2(h, w);
         s_dst[a + li] = s;
-    });
+    };
+    if (NS == 4) {
+        step(A, c1, rain_mid, to_b);
+        __syncthreads();
+        step(B, c2, rain_mid, to_a);
+        __syncthreads();
+        step(A, c3, rain_mid, to_b);
+        __syncthreads();
+        step(B, T, rain_next, out);
+    } else if (NS == 3) {
+        step(A, c2, rain_mid, to_b);
+        __syncthreads();
+        step(B, c3, rain_mid, to_a);
+        __syncthreads();
+        step(A, T, rain_next, out);
+    } else {
+        step(A, c3, rain_mid, to_b);
+        __syncthreads();
+        step(B, T, rain_next, out);
+    }
 }
 #endif
 

@@ -272,9 +272,11 @@ int tile_plan_build(const int32_t *col, const double *elen, int64_t V, int K, vo
         return set_error(NX_EINVAL, "tile plan needs col, elen, plan and 0 < V < 2^31");
     if (K != 2 && K != kMultiSteps)
         return set_error(NX_EINVAL, "tile plan for %d steps per launch: 2 or %d", K, kMultiSteps);
-    if (chunk_in != 0 && (chunk_in < kTileMin || chunk_in > kTileMax))
+    // four-step kernels take any tile whose first-step set fits kQuadStepOne
+    const int tile_max = K == 2 ? kTileMax : kQuadStepOne;
+    if (chunk_in != 0 && (chunk_in < kTileMin || chunk_in > tile_max))
         return set_error(NX_EINVAL, "tile chunk %d outside %d..%d (0 for the default)", chunk_in,
-                         kTileMin, kTileMax);
+                         kTileMin, tile_max);
     const int limit = K == 2 ? kLoadMax : kQuadLoadMax;
     const int step1_limit = K == 2 ? kStepOneMax : kQuadStepOne;
     const int64_t chunk = chunk_in ? chunk_in : (K == 2 ? kPairChunk : kQuadChunk);

@@ -59,16 +59,18 @@ class ErosionGraph:
     ErosionState use the device numbering (``permute``/``unpermute``).
     With `tiled`, the tile plan is built when the graph cuts into tiles
     (nx_tile_plan_build_chunk); otherwise, or for graphs without locality,
-    every step is its own launch.  `steps` (2 or 4) is the number of steps
-    one launch covers: a 4-step plan carries four rings per tile.
+    every step is its own launch.  `steps` (4 or 2) is the number of steps
+    one launch covers: a 4-step plan carries four rings per tile (a graph
+    whose tiles cannot hold them gets a 2-step plan; ``steps`` tells which).
     `chunk` is the plan's starting tile size (0: the library's default),
-    `variant` the two-step kernel variant and `graph_chunk` the steps per
-    captured graph that runs on this graph use (None: the context's
-    setting); autotune.tune_erosion picks the three per mesh size.  None of
+    `variant` the kernel variant for the plan's step count (the context's
+    NX_OPT_MULTI_KERNEL or NX_OPT_PAIR_KERNEL) and `graph_chunk` the steps
+    per captured graph that runs on this graph use (None: the context's
+    setting); autotune.tune_erosion picks the four per mesh size.  None of
     them changes the results.
     """
 
-    def __init__(self, verts, neighbors, order="sfc", device=None, tiled=True, steps=2,
+    def __init__(self, verts, neighbors, order="sfc", device=None, tiled=True, steps=4,
                  chunk=0, variant=None, graph_chunk=None):
         import torch
         verts = np.ascontiguousarray(verts, dtype=np.float64)
@@ -108,7 +110,7 @@ class ErosionGraph:
         self.graph_chunk = None if graph_chunk is None else int(graph_chunk)
 
     @classmethod
-    def from_device(cls, verts, neighbors, order="sfc", tiled=True, stream=None, steps=2,
+    def from_device(cls, verts, neighbors, order="sfc", tiled=True, stream=None, steps=4,
                     chunk=0, variant=None, graph_chunk=None):
         """The same graph from device tensors (verts float64 [V,3], neighbours
         int32 [V,6]), built on the device (nx_sfc_order_device,
@@ -154,6 +156,10 @@ class ErosionGraph:
         plan = _lib.NxTilePlan()
         rc = L.nx_tile_plan_build_chunk(_lib.ptr(col), _lib.ptr(elen), self.V, self.steps,
                                         self.chunk, None, 0, ctypes.byref(plan))
+        if rc == _lib.NX_ERANGE and self.steps != 2:
+            self.steps = 2          # tiles too wide for four rings: two steps per launch
+            rc = L.nx_tile_plan_build_chunk(_lib.ptr(col), _lib.ptr(elen), self.V, self.steps,
+                                            self.chunk, None, 0, ctypes.byref(plan))
         if rc == _lib.NX_ERANGE:
             return
         _lib.check(rc)
@@ -243,8 +249,11 @@ class ErosionState:
         if graph.V != self.V:
             raise ValueError("graph and state sizes differ")
         plan = ctypes.byref(graph.plan) if graph.plan is not None else None
+        variant = getattr(graph, "variant", None)
+        four = graph.plan is not None and graph.plan.steps == 4
         with self.ctx.options(graph_chunk=getattr(graph, "graph_chunk", None),
-                              pair_kernel=getattr(graph, "variant", None)):
+                              pair_kernel=None if four else variant,
+                              multi_kernel=variant if four else None):
             _lib.check(_lib.lib().nx_erosion_run_rule(
                 self.ctx.handle, _lib.ptr(self.buf), self.V, _lib.ptr(graph.col),
                 _lib.ptr(graph.elen), plan, int(iters), float(rain), int(rule),

@@ -208,15 +208,19 @@ def _run_graph(g, h, iters, rain, chunk=32):
     return [g.unpermute(t.cpu().numpy()) for t in out]
 
 
+@pytest.mark.parametrize("steps", [2, 4])
 @pytest.mark.parametrize("d,iters,chunk", [(20, 101, 32), (24, 64, 0), (20, 75, 7), (2, 9, 32),
-                                          (8, 10, 0)])
-def test_pair_launches_match_single_steps(oracle, d, iters, chunk):
-    """Two steps per launch over the tile plan give the one-step-per-launch
-    results and the FP64 oracle's, bit for bit."""
+                                          (8, 10, 0), (16, 7, 6)])
+def test_pair_launches_match_single_steps(oracle, d, iters, chunk, steps):
+    """Two or four steps per launch over the tile plan (a four-step run's
+    last two or three steps in one tail launch, a last single step alone;
+    graph chunks of 7 and 6 end every chunk that way) give the one-step-per-
+    launch results and the FP64 oracle's, bit for bit."""
     pts, nb, h = _planet(d)
-    gt = erosion.ErosionGraph(pts, nb)
+    gt = erosion.ErosionGraph(pts, nb, steps=steps)
     g1 = erosion.ErosionGraph(pts, nb, tiled=False)
     assert gt.plan is not None and gt.plan.ntiles >= 1 and g1.plan is None
+    assert gt.steps == steps and gt.plan.steps == steps
     a = _run_graph(gt, h, iters, 1.0, chunk)
     b = _run_graph(g1, h, iters, 1.0, chunk)
     href = h.copy()
@@ -234,8 +238,8 @@ def test_pair_kernel_variants_match_oracle(oracle, variant):
     one tile to over a thousand (odd step counts end with a single step)."""
     for d, iters, chunk in ((2, 5, 0), (24, 12, 0), (60, 7, 256), (160, 4, 256), (200, 3, 0)):
         pts, nb, h = _planet(d)
-        g = erosion.ErosionGraph(pts, nb, chunk=chunk, variant=variant)
-        assert g.plan is not None
+        g = erosion.ErosionGraph(pts, nb, chunk=chunk, variant=variant, steps=2)
+        assert g.plan is not None and g.plan.steps == 2
         print(f"variant {variant} d={d} iters={iters} tiles {g.plan.ntiles}", flush=True)
         got = _run_graph(g, h, iters, 1.0)
         href = h.copy()
@@ -246,9 +250,9 @@ def test_pair_kernel_variants_match_oracle(oracle, variant):
 
 @pytest.mark.parametrize("rule", [3, 4])
 def test_four_step_plans_match_oracle(oracle, rule):
-    """Four steps per launch (tile + four rings in LDS) with the remainder
-    as single steps: 1..9 steps equal the oracle bit for bit, for every
-    kernel variant."""
+    """Four steps per launch, a remainder of two or three steps in one tail
+    launch and of one step alone: 1..9 steps equal the oracle bit for bit,
+    for every kernel variant."""
     pts, nb, h = _planet(32)
     V = h.size
     rng = np.random.default_rng(rule)
@@ -260,7 +264,7 @@ def test_four_step_plans_match_oracle(oracle, rule):
     for variant in range(5):
         g_ctx = _lib.Context.get(g.device)
         g_ctx.set_multi_kernel(variant)
-        for n in (1, 4, 5, 9):
+        for n in (1, 2, 3, 4, 5, 6, 7, 9):
             print("rule", rule, "variant", variant, "steps", n, flush=True)
             st = erosion.ErosionState(V)
             st.load(*[torch.from_numpy(g.permute(a)).cuda() for a in (h, w0, s0)])
@@ -277,15 +281,16 @@ def test_four_step_plans_match_oracle(oracle, rule):
 
 
 def test_four_step_launches_equal_two_step_and_single_at_scale():
-    """d=1000 (10,000,002 vertices), 23 steps with rain: four-step plans (the
-    default four-step kernel and variant 4, first-step data in registers),
-    the two-step plan and one step per launch give the same fields bit for
+    """d=1000 (10,000,002 vertices), 23 steps with rain (five four-step
+    launches and a three-step tail): four-step plans (the default kernel,
+    first-step data in registers, and variant 1, slot tables in LDS), the
+    two-step plan and one step per launch give the same fields bit for
     bit."""
     pts, nb, h = _planet(1000, n_oct=8)
     outs = []
     ctx = None
     try:
-        for kw, mv in (({"steps": 4}, 0), ({"steps": 4}, 4), ({"steps": 2}, 0), ({"tiled": False}, 0)):
+        for kw, mv in (({"steps": 4}, 0), ({"steps": 4}, 1), ({"steps": 2}, 0), ({"tiled": False}, 0)):
             g = erosion.ErosionGraph(pts, nb, **kw)
             ctx = _lib.Context.get(g.device)
             ctx.set_multi_kernel(mv)
@@ -452,28 +457,37 @@ def test_erode_terrain3_snapshots(oracle, tmp_path):
 
 
 def test_erosion_autotune_picks_persists_and_keeps_results(oracle, tmp_path):
-    """autotune.tune_erosion times kernel variants, tile chunks and graph
-    chunks on a mesh, stores the best for the mesh's size class, and
-    lookup_erosion returns it.  Erosion on the tuned graph, and on other
-    candidate settings, gives the oracle's heights bit for bit; the tuned
-    options hold only for the run and the context's own come back after it."""
+    """autotune.tune_erosion times four- and two-step plans (kernel
+    variants, then tile chunks), then graph chunks, on a mesh; it stores the
+    best for the mesh's size class with the best two-step tuning beside it,
+    and lookup_erosion returns them.  Erosion on the tuned graph, and on
+    other candidate settings, gives the oracle's heights bit for bit; the
+    tuned options hold only for the run and the context's own come back
+    after it."""
     from nixis_amd import autotune, batch
     pts, nb, h = _planet(48)
     order = util.sfc_order(pts)
     path = str(tmp_path / "tune.json")
     best, table = autotune.tune_erosion(
-        lambda c: erosion.ErosionGraph(pts, nb, order=order, chunk=c), steps=6, reps=1,
-        path=path, variants=(0, 1, 5), chunks=(256, 512), graph_chunks=(8, 32))
-    assert len(table) == 3 + 2 + 2
-    assert best.variant in (0, 1, 5) and best.chunk in (0, 256, 512) and best.graph_chunk in (8, 32)
+        lambda c, k: erosion.ErosionGraph(pts, nb, order=order, chunk=c, steps=k), steps=6,
+        reps=1, path=path, variants={4: (0, 1), 2: (0, 1, 5)},
+        chunks={4: (256, 512), 2: (256, 512)}, graph_chunks=(8, 32))
+    assert len(table) == (2 + 2) + (3 + 2) + 2
+    assert {k for _, k, *_ in table} == {2, 4}
+    assert best.steps in (2, 4) and best.chunk in (0, 256, 512) and best.graph_chunk in (8, 32)
+    assert best.variant in ((0, 1) if best.steps == 4 else (0, 1, 5))
     assert autotune.lookup_erosion(pts.shape[0], path=path) == best
+    two = autotune.lookup_erosion(pts.shape[0], path=path, steps=2)
+    assert two.steps == 2 and two.variant in (0, 1, 5) and two.chunk in (0, 256, 512)
     href = h.copy()
     oracle.erode_terrain3(pts, nb, href, 21, 1.0)
     ctx = _lib.Context.get()
     tiles = set()
-    for kw in (best.graph_kwargs(), {"chunk": 256, "variant": 5, "graph_chunk": 8},
-               {"chunk": 512, "variant": 1, "graph_chunk": 0}):
+    for kw in (best.graph_kwargs(), two.graph_kwargs(),
+               {"chunk": 256, "variant": 5, "graph_chunk": 8, "steps": 2},
+               {"chunk": 512, "variant": 1, "graph_chunk": 0, "steps": 4}):
         g = erosion.ErosionGraph(pts, nb, order=order, **kw)
+        assert g.steps == kw["steps"]
         tiles.add(g.plan.ntiles)
         st = erosion.ErosionState(g.V)
         st.load(torch.from_numpy(g.permute(h)).cuda())
@@ -482,9 +496,17 @@ def test_erosion_autotune_picks_persists_and_keeps_results(oracle, tmp_path):
         st.store(out)
         st.ctx.sync_check()
         assert (g.unpermute(out.cpu().numpy()) == href).all(), kw
-        assert ctx.graph_chunk == 32 and ctx.pair_kernel == 0
+        assert ctx.graph_chunk == 32 and ctx.pair_kernel == 0 and ctx.multi_kernel == 0
     assert len(tiles) >= 2      # the chunks cut different plans
-    # MeshCache cuts its plans with a given tuning
+    # MeshCache cuts two-step plans (what its multi-planet launches run) with a given tuning
     mc = batch.MeshCache()
     g = mc.graph(48, tuning=autotune.ErosionTuning(256, 5, 8))
-    assert (g.chunk, g.variant, g.graph_chunk) == (256, 5, 8)
+    assert (g.chunk, g.variant, g.graph_chunk, g.steps) == (256, 5, 8, 2)
+    # its own search covers two-step plans only and keeps a faster four-step entry first
+    b2, _ = mc.tune_erosion(48, steps=6, reps=1, path=path, variants={2: (0, 5)},
+                            chunks={2: (256,)}, graph_chunks=(8,))
+    assert b2.steps == 2 and mc.graph(48).steps == 2
+    assert autotune.lookup_erosion(pts.shape[0], path=path, steps=2) == b2
+    if best.steps == 4:
+        first = autotune.lookup_erosion(pts.shape[0], path=path)
+        assert first.steps == 4 or first == b2

@@ -29,16 +29,16 @@ def _ell(d, order="sfc"):
     return pts, nb, o, col, elen
 
 
-def _plan(col, elen, steps=2):
+def _plan(col, elen, steps=2, chunk=0):
     V = col.shape[1]
     L = _lib.lib()
     q = _lib.NxTilePlan()
-    rc = L.nx_tile_plan_build_steps(_lib.ptr(col), _lib.ptr(elen), V, steps, None, 0,
+    rc = L.nx_tile_plan_build_chunk(_lib.ptr(col), _lib.ptr(elen), V, steps, chunk, None, 0,
                                     ctypes.byref(q))
     _lib.check(rc)
     img = np.zeros(q.bytes, np.uint8)
-    _lib.check(L.nx_tile_plan_build_steps(_lib.ptr(col), _lib.ptr(elen), V, steps, _lib.ptr(img),
-                                          q.bytes, ctypes.byref(q)))
+    _lib.check(L.nx_tile_plan_build_chunk(_lib.ptr(col), _lib.ptr(elen), V, steps, chunk,
+                                          _lib.ptr(img), q.bytes, ctypes.byref(q)))
     assert q.steps == steps
     nt = q.ntiles
 
@@ -84,8 +84,9 @@ def _decode(codes, lens, c, T):
     return nbr, lens[_where(colm, owner, c, T)]
 
 
-def _tiles(col, elen, steps=2):
-    q, start, cs, load, rbase, ring, cbase, lcol, (lel, ebase), pref = _plan(col, elen, steps)
+def _tiles(col, elen, steps=2, chunk=0):
+    q, start, cs, load, rbase, ring, cbase, lcol, (lel, ebase), pref = _plan(col, elen, steps,
+                                                                             chunk)
     for t in range(q.ntiles):
         a, b = int(start[t]), int(start[t + 1])
         T = b - a
@@ -237,13 +238,17 @@ def test_two_steps_from_the_plan_equal_the_oracle(oracle):
         assert (H2 == hr[a:b]).all() and (W2 == wr[a:b]).all() and (S2 == sr[a:b]).all()
 
 
-@pytest.mark.parametrize("d", [3, 12, 40])
-def test_four_step_rings_are_exact_layers(d):
+@pytest.mark.parametrize("d,chunk", [(3, 0), (12, 0), (40, 0), (40, 1536), (60, 2048)])
+def test_four_step_rings_are_exact_layers(d, chunk):
+    """Four-step tiles from the default chunk and from chunks above the
+    two-step limit of 1024: the rings are the tile's successive layers of
+    outside neighbours, tile + rings 1..3 stay within 2048 vertices and tile
+    + rings within 2688."""
     pts, nb, o, col, elen = _ell(d)
     V = col.shape[1]
     nxt = 0
-    for q, a, b, T, c, L, glob, lc, le, pref in _tiles(col, elen, 4):
-        assert a == nxt and L <= 3072
+    for q, a, b, T, c, L, glob, lc, le, pref in _tiles(col, elen, 4, chunk):
+        assert a == nxt and c <= 2048 and L <= 2688 and T <= max(chunk, 1152)
         nxt = b
         seen = set(range(a, b))
         layer = list(range(a, b))
@@ -278,6 +283,52 @@ def test_four_steps_from_the_plan_equal_the_oracle(oracle):
             Hn, Wn, Sn = _step_rows(n, H, W, S, lc, le)
             H[:n], W[:n], S[:n] = Hn, Wn, Sn
         assert (H[:T] == hr[a:b]).all() and (W[:T] == wr[a:b]).all() and (S[:T] == sr[a:b]).all()
+
+
+@pytest.mark.parametrize("n_steps", [2, 3])
+def test_tail_steps_from_a_four_step_plan_equal_the_oracle(oracle, n_steps):
+    """A run's last two or three steps on a four-step plan: step j covers
+    the first pref[n - j] local vertices (tile + rings 1..n-j), reading only
+    the first pref[n] (the vertices a tail launch stages; the rest are set
+    to NaN here), and equals n steps of the oracle bit for bit."""
+    d = 16
+    pts, nb, o, col, elen = _ell(d)
+    V = col.shape[1]
+    P = pts[o]
+    nbd = col.T.copy()
+    nbd[nbd == np.arange(V)[:, None]] = -1
+    rng = np.random.default_rng(n_steps)
+    h = rng.uniform(-9000, 9000, V)
+    w = rng.uniform(0, 1, V)
+    s = rng.uniform(0, 0.01, V)
+    hr, wr, sr = h.copy(), w.copy(), s.copy()
+    for _ in range(n_steps):
+        oracle.erosion_iteration3(P, nbd, hr, wr, sr)
+    for q, a, b, T, c, L, glob, lc, le, pref in _tiles(col, elen, 4, 1024):
+        H, W, S = h[glob].copy(), w[glob].copy(), s[glob].copy()
+        staged = int(pref[n_steps])
+        H[staged:], W[staged:], S[staged:] = np.nan, np.nan, np.nan
+        for j in range(1, n_steps + 1):
+            n = int(pref[n_steps - j])
+            Hn, Wn, Sn = _step_rows(n, H, W, S, lc, le)
+            H[:n], W[:n], S[:n] = Hn, Wn, Sn
+        assert (H[:T] == hr[a:b]).all() and (W[:T] == wr[a:b]).all() and (S[:T] == sr[a:b]).all()
+
+
+def test_chunk_limits_by_step_count():
+    """Two-step plans start from at most 1024 vertices (the pair kernel's
+    tile limit), four-step plans from at most 2048."""
+    _, _, _, col, elen = _ell(12)
+    V = col.shape[1]
+    q = _lib.NxTilePlan()
+    L = _lib.lib()
+    for steps, chunk, ok in ((2, 1024, True), (2, 1025, False), (4, 2048, True),
+                             (4, 2049, False), (4, 31, False)):
+        rc = L.nx_tile_plan_build_chunk(_lib.ptr(col), _lib.ptr(elen), V, steps, chunk, None, 0,
+                                        ctypes.byref(q))
+        assert (rc == _lib.NX_OK) == ok, (steps, chunk, rc)
+        if not ok:
+            assert rc == _lib.NX_EINVAL
 
 
 def test_numbering_without_locality_is_refused():
