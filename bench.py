@@ -249,6 +249,8 @@ def bench_erosion(args):
     B = 28 * V + 8 * nnz
     per_step_ms = ms / iters
     spl = plan.steps if plan is not None else 1
+    launch_us = spl * per_step_ms * 1e3
+    traffic = profiled_traffic("erosion", V, spl)
     gbs = B / (per_step_ms * 1e-3) / 1e9
     log(f"[erosion] {iters} iters in {ms:.2f} ms ({per_step_ms * 1e3:.1f} us/iter), "
         f"{gvi:.2f} Gvi/s, {gbs:.0f} GB/s algorithmic")
@@ -260,10 +262,16 @@ def bench_erosion(args):
         "us_per_iter": round(per_step_ms * 1e3, 2),
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
-                     "traffic": profiled_traffic("erosion", V, spl),
+                     "traffic": traffic,
                      "algorithmic_bytes_per_launch": spl * B, "steps_per_launch": spl,
                      "plan_stream_bytes_per_launch": getattr(graph, "stream_bytes_per_launch", None),
-                     "launch_us": round(spl * per_step_ms * 1e3, 2)},
+                     "launch_us": round(launch_us, 2),
+                     # B counts a pass over fields and graph per step; a launch of
+                     # several steps makes one, so `frac` can pass 1.  The rate of
+                     # the bytes the launch does move:
+                     "traffic_GBs": round(traffic / (launch_us * 1e3), 1) if traffic else None,
+                     "traffic_frac": (round(traffic / (launch_us * 1e3) / HBM_PEAK_GBS, 4)
+                                      if traffic else None)},
         "tuning": {"steps_per_launch": spl, "chunk": tuning.chunk, "variant": tuning.variant,
                    "graph_chunk": tuning.graph_chunk,
                    "tiles": plan.ntiles if plan is not None else 0,
