@@ -3,8 +3,9 @@
 A job is one planet: fractal noise (terrain.sample_octaves with the
 reference's defaults unless given), an optional rescale to [lower, upper]
 (util.rescale) and an optional number of erosion steps (erode_terrain3).
-``BatchRunner`` runs a list of jobs with one persistent noise launch and one
-launch per erosion step for all planets still eroding; ``run_plain`` runs
+``BatchRunner`` runs a list of jobs with one persistent noise launch and
+erosion launches of four (or two) steps each over the tiles of every planet
+still eroding; ``run_plain`` runs
 the same jobs one planet per call through the single-planet entry points,
 for comparison.  Meshes (vertices and neighbour tables) depend only on the
 subdivision count and are shared between jobs through ``MeshCache``.
@@ -149,22 +150,20 @@ class MeshCache:
         return v
 
     def graph(self, d, tuning=None):
-        """The erosion graph of subdivision d, its two-step tile plan cut
-        with the autotuned two-step settings for its size class when the
-        tuning cache has them (`tuning` overrides; the results are the same
-        either way).  The multi-planet launches run two steps per launch."""
+        """The erosion graph of subdivision d, its tile plan cut with the
+        autotuned settings for its size class when the tuning cache has them
+        (`tuning` overrides; the results are the same either way)."""
         g = self._graphs.get(d)
         if g is None:
             if tuning is None:
                 from . import autotune
-                tuning = autotune.lookup_erosion(10 * d * d + 2, steps=2)
+                tuning = autotune.lookup_erosion(10 * d * d + 2)
             kw = tuning.graph_kwargs() if tuning is not None else {}
             g = self._build_graph(d, **kw)
             self._graphs[d] = g
         return g
 
     def _build_graph(self, d, **kw):
-        kw["steps"] = 2
         if self.on_device:
             pts, cells = _util.create_mesh_device(d, self.device)
             adj = _util.sort_adjacency_device(_util.build_adjacency_device(cells, pts.shape[0]))
@@ -177,8 +176,8 @@ class MeshCache:
         """Autotune erosion for subdivision d's size class (autotune.tune_erosion)
         and keep the tuned graph; returns (tuning, table)."""
         from . import autotune
-        kw.setdefault("step_options", (2,))
-        best, table = autotune.tune_erosion(lambda c, k: self._build_graph(d, chunk=c), **kw)
+        best, table = autotune.tune_erosion(
+            lambda c, k: self._build_graph(d, chunk=c, steps=k), **kw)
         self._graphs[d] = self._build_graph(d, **best.graph_kwargs())
         return best, table
 

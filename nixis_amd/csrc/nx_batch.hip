@@ -10,11 +10,14 @@
 //   * rescale + erosion load: per-planet min/max (one atomic per block per
 //     planet) then the map of util.py:139, which also writes the planet's
 //     FP64 erosion state with the first rain already added.
-//   * erosion: one launch per step covers every planet that still has steps
-//     left (planets ordered by remaining steps, so the active set is a
-//     prefix); each planet keeps its own step counter and buffer index in
+//   * erosion: a tile phase, then a one-step phase.  Launch k of the tile
+//     phase covers every planet with more than k tile launches in this call
+//     (planets ordered by that count, so the active set is a prefix): four
+//     steps of a four-step plan (the last two or three in one tail launch)
+//     or two of a two-step plan; a step left over runs in the one-step
+//     phase.  Each planet keeps its own step counter and buffer index in
 //     its workspace, so a planet's result does not depend on its batch.
-//     The step sequence can be captured into a hipGraph.
+//     The launch sequence can be captured into a hipGraph.
 // Each planet's arithmetic is the single-planet kernels' arithmetic.
 #include <hip/hip_runtime.h>
 
@@ -55,6 +58,10 @@ struct BatchPlanetDev {
     int64_t nblocks;  // ceil(V / 256)
     TilePlanDev plan;
     int32_t has_plan;
+    // this call's tile launches: full_launches of plan.steps steps, then
+    // one of tail_steps (2 or 3; 0: none)
+    int32_t tail_steps;
+    int64_t full_launches;
     OctavePlan oct;
     NoiseTablesPacked tables;
 };
@@ -63,9 +70,13 @@ constexpr int kRsTile = 4096;  // vertices per block in the rescale / load / sto
 
 // Workspace layout (offsets in bytes, all 256-aligned).
 struct BatchLayout {
-    size_t counter, keys, desc, noise_prefix, rs_prefix, pair_prefix, pair_desc, pair_bucket,
-        one_prefix, one_order, one_base, steps, launches, total;
+    size_t counter, keys, desc, noise_prefix, rs_prefix, tile_prefix[4], tile_order[4],
+        tile_bucket[4], one_prefix, one_order, one_base, steps, launches, total;
 };
+
+// Tile phases of an erosion call, in launch order (a planet's tail runs
+// before its four-step launches).
+enum TilePhaseKind { kPhaseTail3, kPhaseTail2, kPhaseQuad, kPhasePair, kTilePhases };
 
 // Planet lookup of the many-planet pair kernel: tile b falls in bucket
 // b >> shift; the bucket holds the planet (in pair order) of its first tile
@@ -92,9 +103,11 @@ static BatchLayout layout(int n)
     L.desc = take(sizeof(BatchPlanetDev) * n);
     L.noise_prefix = take(sizeof(int64_t) * (n + 1));
     L.rs_prefix = take(sizeof(int64_t) * (n + 1));
-    L.pair_prefix = take(sizeof(int64_t) * (n + 1));
-    L.pair_desc = take(sizeof(BatchPlanetDev) * n);
-    L.pair_bucket = take(sizeof(PairBucket) * kPairBuckets);
+    for (int f = 0; f < kTilePhases; ++f) {
+        L.tile_prefix[f] = take(sizeof(int64_t) * (n + 1));
+        L.tile_order[f] = take(sizeof(int32_t) * n);
+        L.tile_bucket[f] = take(sizeof(PairBucket) * kPairBuckets);
+    }
     L.one_prefix = take(sizeof(int64_t) * (n + 1));
     L.one_order = take(sizeof(int32_t) * n);
     L.one_base = take(sizeof(int64_t) * n);
@@ -247,7 +260,7 @@ __global__ __launch_bounds__(256) void k_batch_map(const BatchPlanetDev *__restr
 }
 
 // One erosion step for the first `active` planets of `order`: launch k of
-// the one-step phase, after base[r] two-step launches of planet order[r].
+// the one-step phase, after base[r] tile launches of planet order[r].
 __global__ __launch_bounds__(256) void k_erode_batch(const BatchPlanetDev *__restrict__ P,
                                                      const int32_t *__restrict__ order,
                                                      const int64_t *__restrict__ prefix,
@@ -260,7 +273,7 @@ __global__ __launch_bounds__(256) void k_erode_batch(const BatchPlanetDev *__res
     const BatchPlanetDev &D = P[order[r]];
     const ErosionMeta *meta = D.meta;
     const int64_t launch = base[r] + k;
-    const int64_t t = meta->it + 2 * base[r] + k;
+    const int64_t t = meta->it + D.full_launches * D.plan.steps + D.tail_steps + k;
     const int p = (meta->buf + (int)(launch & 1)) & 1;
     const double2 *hw_src = p ? D.hw1 : D.hw0;
     const double *s_src = p ? D.s1 : D.s0;
@@ -271,60 +284,91 @@ __global__ __launch_bounds__(256) void k_erode_batch(const BatchPlanetDev *__res
     if (i < D.V) erode_vertex_step(i, D.V, hw_src, s_src, hw_dst, s_dst, D.col, D.elen, rain);
 }
 
-// Launch k of the two-step phase runs steps it + 2k and it + 2k + 1 of every
-// tile of the first `active` planets in pair order (one block per tile), in
-// one of three kernels by the number of planets it covers.
-//
-// A two-step launch that covers one planet takes the single-planet kernel's
-// arguments (buffers and plan by value) with the planet's step count and
-// rain; the step counter and buffer index come from one load of the
+// The tile phases.  A planet with a four-step plan takes, in this order,
+// one tail launch of 2 or 3 steps (when its step count leaves one) and then
+// its four-step launches; a planet with a two-step plan takes its two-step
+// launches; a step left over runs in the one-step phase.  Each phase's
+// launches run one tile kernel, a template on the plan's steps per launch K
+// and the launch's steps NS: (4, 3) and (4, 2) are the tails, (4, 4) the
+// four-step phase, (2, 2) the two-step phase.  Launch k of a phase covers
+// every tile of the first `active` planets in the phase's order (planets
+// ordered by their launch count in the phase, one block per tile) and runs
+// steps t .. t + NS - 1, t = it + (tail before it) + NS k, reading buffer
+// (buf + launches before it) & 1.
+template <int K, int NS>
+__device__ __forceinline__ void batch_tile(double2 *A, double2 *B, double2 *hw0, double2 *hw1,
+                                           double *s0, double *s1, int64_t it, int buf,
+                                           const TilePlanDev &plan, int tail, int64_t total_iters,
+                                           double rain, int64_t tile, int64_t k)
+{
+    constexpr bool after_tail = K == 4 && NS == 4;
+    const int64_t t = it + (after_tail ? tail : 0) + NS * k;
+    const int64_t launch = (after_tail && tail ? 1 : 0) + k;
+    const int p = (buf + (int)(launch & 1)) & 1;
+    const double rain_mid = t + 1 < total_iters ? rain : 0.0;
+    const double rain_next = t + NS < total_iters ? rain : 0.0;
+    if constexpr (K == 2)
+        erode_tile_pair<1024, true>(A, B, hw0, hw1, s0, s1, p, plan, tile, rain_mid, rain_next);
+    else
+        erode_tile_quad<3, NS>(A, B, hw0, hw1, s0, s1, p, plan, tile, rain_mid, rain_next);
+}
+
+template <int K>
+struct TileLds {
+    static constexpr int kA = K == 2 ? kLoadMax : kQuadLoadMax;
+    static constexpr int kB = K == 2 ? kStepOneMax : kQuadStepOne;
+};
+
+// A launch that covers one planet takes the single-planet kernel's
+// arguments (buffers and plan by value) with the planet's step count, rain
+// and tail; the step counter and buffer index come from one load of the
 // planet's meta, as in the single-planet kernel.
-__global__ __launch_bounds__(1024) void k_erode_batch_pair_one(const ErosionMeta *__restrict__ meta,
+template <int K, int NS>
+__global__ __launch_bounds__(1024) void k_erode_batch_tile_one(const ErosionMeta *__restrict__ meta,
                                                                double2 *__restrict__ hw0,
                                                                double2 *__restrict__ hw1,
                                                                double *__restrict__ s0,
                                                                double *__restrict__ s1,
-                                                               TilePlanDev plan, int64_t k,
+                                                               TilePlanDev plan, int tail, int64_t k,
                                                                int64_t total_iters, double rain)
 {
-    __shared__ double2 A[kLoadMax];
-    __shared__ double2 B[kStepOneMax];
+    __shared__ double2 A[TileLds<K>::kA];
+    __shared__ double2 B[TileLds<K>::kB];
     const int64_t tile = xcd_tile(blockIdx.x, plan.ntiles);
     if (tile >= plan.ntiles) return;
     const ErosionMeta m = *meta;
-    const int64_t t = m.it + 2 * k;
-    const double rain_mid = t + 1 < total_iters ? rain : 0.0;
-    const double rain_next = t + 2 < total_iters ? rain : 0.0;
-    erode_tile_pair<1024, true>(A, B, hw0, hw1, s0, s1, (m.buf + (int)(k & 1)) & 1, plan, tile,
-                                rain_mid, rain_next);
+    batch_tile<K, NS>(A, B, hw0, hw1, s0, s1, m.it, m.buf, plan, tail, total_iters, rain, tile, k);
 }
 
-// A two-step launch over at most kArgPlanets planets takes what each block
-// needs in its kernel arguments (tile prefix, plan, buffers, step counter,
-// rain), so a block finds its planet among scalar argument loads instead of
-// a search over device arrays followed by the descriptor.  The prefix is
-// int32 and padded past the launch's planets with values no tile index
-// reaches, so the planet is a count of prefix entries <= the tile: fixed
-// argument offsets, loaded together and counted with scalar arithmetic.
+// A launch over at most kArgPlanets planets takes what each block needs in
+// its kernel arguments (tile prefix, plan, buffers, step counter, rain), so
+// a block finds its planet among scalar argument loads instead of a search
+// over device arrays followed by the descriptor.  The prefix is int32 and
+// padded past the launch's planets with values no tile index reaches, so the
+// planet is a count of prefix entries <= the tile: fixed argument offsets,
+// loaded together and counted with scalar arithmetic.
 constexpr int kArgPlanets = 16;
-struct PairPlanetArgs {
+struct TilePlanetArgs {
     TilePlanDev plan;
     double2 *hw0, *hw1;
     double *s0, *s1;
     const ErosionMeta *meta;
     int64_t total_iters;
     double rain;
+    int32_t tail_steps;
 };
-struct PairLaunchArgs {
+struct TileLaunchArgs {
     int32_t prefix[kArgPlanets];   // prefix[q] = first tile of planet q (q < active)
-    PairPlanetArgs pl[kArgPlanets];
+    TilePlanetArgs pl[kArgPlanets];
 };
+static_assert(sizeof(TileLaunchArgs) + 16 <= 4096, "kernel arguments within 4 KiB");
 
-__global__ __launch_bounds__(1024) void k_erode_batch_pair_args(const PairLaunchArgs a,
+template <int K, int NS>
+__global__ __launch_bounds__(1024) void k_erode_batch_tile_args(const TileLaunchArgs a,
                                                                 int32_t nt_total, int64_t k)
 {
-    __shared__ double2 A[kLoadMax];
-    __shared__ double2 B[kStepOneMax];
+    __shared__ double2 A[TileLds<K>::kA];
+    __shared__ double2 B[TileLds<K>::kB];
     const int b = (int)xcd_tile(blockIdx.x, nt_total);
     if (b >= nt_total) return;
     // b and the prefix entries lie in [0, 2^31): b - prefix[q] is negative,
@@ -333,26 +377,25 @@ __global__ __launch_bounds__(1024) void k_erode_batch_pair_args(const PairLaunch
 #pragma unroll
     for (int q = 1; q < kArgPlanets; ++q) after += (int)(((uint32_t)b - (uint32_t)a.prefix[q]) >> 31);
     const int r = (kArgPlanets - 1) - after;
-    const PairPlanetArgs &Q = a.pl[r];
-    const int64_t t = Q.meta->it + 2 * k;
-    const int p = (Q.meta->buf + (int)(k & 1)) & 1;
-    const double rain_mid = t + 1 < Q.total_iters ? Q.rain : 0.0;
-    const double rain_next = t + 2 < Q.total_iters ? Q.rain : 0.0;
-    erode_tile_pair<1024, true>(A, B, Q.hw0, Q.hw1, Q.s0, Q.s1, p, Q.plan, b - a.prefix[r],
-                                rain_mid, rain_next);
+    const TilePlanetArgs &Q = a.pl[r];
+    batch_tile<K, NS>(A, B, Q.hw0, Q.hw1, Q.s0, Q.s1, Q.meta->it, Q.meta->buf, Q.plan, Q.tail_steps,
+                      Q.total_iters, Q.rain, b - a.prefix[r], k);
 }
 
-// A launch over more planets reads their descriptors (PD, in pair order;
-// prefix[r] = first tile of planet r) and finds its planet in the buckets.
-__global__ __launch_bounds__(1024) void k_erode_batch_pair(const BatchPlanetDev *__restrict__ PD,
+// A launch over more planets reads their descriptors (PD[order[r]] for
+// the phase's planet r; prefix[r] = its first tile) and finds its planet in
+// the buckets.
+template <int K, int NS>
+__global__ __launch_bounds__(1024) void k_erode_batch_tile(const BatchPlanetDev *__restrict__ PD,
+                                                           const int32_t *__restrict__ order,
                                                            const PairBucket *__restrict__ bucket,
                                                            int shift,
                                                            const int64_t *__restrict__ prefix,
                                                            int active, int64_t nt_total,
                                                            int64_t k)
 {
-    __shared__ double2 A[kLoadMax];
-    __shared__ double2 B[kStepOneMax];
+    __shared__ double2 A[TileLds<K>::kA];
+    __shared__ double2 B[TileLds<K>::kB];
     const int64_t b = xcd_tile(blockIdx.x, nt_total);
     if (b >= nt_total) return;
     // The planet is the same for the whole block: keep its descriptor in
@@ -366,17 +409,21 @@ __global__ __launch_bounds__(1024) void k_erode_batch_pair(const BatchPlanetDev 
         while (r + 1 < active && prefix[r + 1] <= b) ++r;
     }
     r = __builtin_amdgcn_readfirstlane(r);
-    const BatchPlanetDev &D = PD[r];
+    const BatchPlanetDev &D = PD[order[r]];
     const TilePlanDev plan = D.plan;
     double2 *hw0 = D.hw0, *hw1 = D.hw1;
     double *s0 = D.s0, *s1 = D.s1;
-    const int64_t t = D.meta->it + 2 * k;
-    const int p = (D.meta->buf + (int)(k & 1)) & 1;
-    const double rain_mid = t + 1 < D.total_iters ? D.rain : 0.0;
-    const double rain_next = t + 2 < D.total_iters ? D.rain : 0.0;
-    erode_tile_pair<1024, true>(A, B, hw0, hw1, s0, s1, p, plan, b - prefix[r], rain_mid,
-                                rain_next);
+    batch_tile<K, NS>(A, B, hw0, hw1, s0, s1, D.meta->it, D.meta->buf, plan, D.tail_steps,
+                      D.total_iters, D.rain, b - prefix[r], k);
 }
+
+// The three launch forms of one tile phase.
+template <int K, int NS>
+struct TileKernels {
+    decltype(&k_erode_batch_tile_one<K, NS>) one = k_erode_batch_tile_one<K, NS>;
+    decltype(&k_erode_batch_tile_args<K, NS>) args = k_erode_batch_tile_args<K, NS>;
+    decltype(&k_erode_batch_tile<K, NS>) many = k_erode_batch_tile<K, NS>;
+};
 
 // After both phases: planet p took steps[p] steps in launches[p] launches.
 __global__ void k_batch_advance(const BatchPlanetDev *__restrict__ P,
@@ -446,10 +493,13 @@ static int fill_desc(const nx_planet &q, BatchPlanetDev &d)
         d.rain = q.rain;
         if (q.plan) {
             const nx_tile_plan &pl = *q.plan;
+            const bool two = pl.steps == 2;
             if (!pl.blob || ((uintptr_t)pl.blob & 255) || pl.V != q.V || pl.ntiles <= 0 ||
-                pl.ntiles > INT32_MAX || pl.steps != 2 || pl.max_tile > kTileMax ||
-                pl.max_load > kLoadMax || pl.max_step1 > kStepOneMax)
-                return set_error(NX_EINVAL, "planet tile plan is not a 2-step plan for V=%lld within the kernel limits",
+                pl.ntiles > INT32_MAX || (!two && pl.steps != kMultiSteps) ||
+                pl.max_tile > (two ? kTileMax : kQuadStepOne) ||
+                pl.max_load > (two ? kLoadMax : kQuadLoadMax) ||
+                pl.max_step1 > (two ? kStepOneMax : kQuadStepOne))
+                return set_error(NX_EINVAL, "planet tile plan is not a 2- or 4-step plan for V=%lld within the kernel limits",
                                  (long long)q.V);
             d.plan = plan_dev(pl);
             d.has_plan = 1;
@@ -466,8 +516,9 @@ struct Part {
     size_t bytes;
 };
 
-static int upload(nx_ctx *ctx, char *w, const BatchLayout &L, std::initializer_list<Part> parts,
-                  hipStream_t stream)
+template <class Parts>
+static int upload_parts(nx_ctx *ctx, char *w, const BatchLayout &L, const Parts &parts,
+                        hipStream_t stream)
 {
     const size_t bytes = L.total - L.desc;
     int slot = 0;
@@ -475,6 +526,12 @@ static int upload(nx_ctx *ctx, char *w, const BatchLayout &L, std::initializer_l
     if (!h) return set_error(NX_EHIP, "pinned staging buffer unavailable");
     for (const Part &p : parts) std::memcpy(h + (p.off - L.desc), p.src, p.bytes);
     return stage_commit(ctx, slot, w + L.desc, bytes, stream);
+}
+
+static int upload(nx_ctx *ctx, char *w, const BatchLayout &L, std::initializer_list<Part> parts,
+                  hipStream_t stream)
+{
+    return upload_parts(ctx, w, L, parts, stream);
 }
 
 int batch_noise(nx_ctx *ctx, nx_planet *planets, int n, void *ws, int tile, int blocks_per_cu,
@@ -544,10 +601,14 @@ int batch_erode(nx_ctx *ctx, nx_planet *planets, int n, int max_steps, void *ws,
         int rc = fill_desc(planets[p], desc[p]);
         if (rc) return rc;
     }
-    // Steps of each planet in this call: pairs over its tiles if it has a
-    // plan, then single steps (an odd last one, or all without a plan).
-    std::vector<int64_t> steps(n, 0), pairs(n, 0), ones(n, 0);
+    // Steps of each planet in this call: a tail launch of 2 or 3 steps and
+    // then four-step launches with a four-step plan, two-step launches with
+    // a two-step plan, and a last step (or every step, without a plan)
+    // alone.  Launch counts per tile phase (TilePhaseKind order).
+    std::vector<int64_t> steps(n, 0), tile_launches(n, 0), ones(n, 0);
     std::vector<int32_t> launches(n, 0);
+    std::vector<int64_t> cnt[kTilePhases];
+    for (auto &c : cnt) c.assign(n, 0);
     for (int p = 0; p < n; ++p) {
         const nx_planet &q = planets[p];
         if (q.erosion_state) {
@@ -555,94 +616,133 @@ int batch_erode(nx_ctx *ctx, nx_planet *planets, int n, int max_steps, void *ws,
             if (q.step_cap > 0) cap = std::min<int64_t>(cap, q.step_cap);
             steps[p] = std::max<int64_t>(0, std::min<int64_t>(cap, q.erosion_iters - q.steps_done));
         }
-        pairs[p] = desc[p].has_plan ? steps[p] / 2 : 0;
-        ones[p] = steps[p] - 2 * pairs[p];
-        launches[p] = (int32_t)(pairs[p] + ones[p]);
+        const int K = desc[p].has_plan ? desc[p].plan.steps : 0;
+        const int64_t full = K ? steps[p] / K : 0;
+        const int64_t rem = steps[p] - K * full;
+        const int tail = (K == kMultiSteps && rem >= 2) ? (int)rem : 0;
+        desc[p].full_launches = full;
+        desc[p].tail_steps = tail;
+        cnt[kPhaseTail3][p] = tail == 3;
+        cnt[kPhaseTail2][p] = tail == 2;
+        cnt[kPhaseQuad][p] = K == kMultiSteps ? full : 0;
+        cnt[kPhasePair][p] = K == 2 ? full : 0;
+        tile_launches[p] = full + (tail ? 1 : 0);
+        ones[p] = rem - tail;
+        launches[p] = (int32_t)(tile_launches[p] + ones[p]);
     }
-    auto ordered = [&](const std::vector<int64_t> &cnt, std::vector<int32_t> &ord) {
+    auto ordered = [&](const std::vector<int64_t> &c, std::vector<int32_t> &ord) {
         ord.resize(n);
         std::iota(ord.begin(), ord.end(), 0);
-        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return cnt[a] > cnt[b]; });
+        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return c[a] > c[b]; });
         int act = 0;
-        while (act < n && cnt[ord[act]] > 0) ++act;
+        while (act < n && c[ord[act]] > 0) ++act;
         return act;
     };
-    std::vector<int32_t> ordP, ordS;
-    const int activeP0 = ordered(pairs, ordP), activeS0 = ordered(ones, ordS);
-    if (activeP0 == 0 && activeS0 == 0) return NX_OK;
-    const int64_t KP = activeP0 ? pairs[ordP[0]] : 0, KS = activeS0 ? ones[ordS[0]] : 0;
-    std::vector<int64_t> tpre(n + 1, 0), bpre(n + 1, 0), sbase(n, 0);
-    for (int r = 0; r < n; ++r) {
-        tpre[r + 1] = tpre[r] + (r < activeP0 ? desc[ordP[r]].plan.ntiles : 0);
-        bpre[r + 1] = bpre[r] + (r < activeS0 ? desc[ordS[r]].nblocks : 0);
-        sbase[r] = pairs[ordS[r]];
+    // Each tile phase: planets ordered by launch count (the active set of
+    // launch k is a prefix), their tile prefix, the bucket table of the
+    // many-planet kernel (tile b in bucket b >> shift) and the kernel-
+    // argument form of its first kArgPlanets planets.
+    struct Phase {
+        std::vector<int32_t> ord;
+        std::vector<int64_t> tpre;
+        std::vector<PairBucket> buckets;
+        int active0 = 0, shift = 0;
+        int64_t count0 = 0;
+        TileLaunchArgs args;
+    };
+    std::vector<Phase> ph(kTilePhases);
+    bool any = false;
+    for (int f = 0; f < kTilePhases; ++f) {
+        Phase &P = ph[f];
+        P.active0 = ordered(cnt[f], P.ord);
+        P.count0 = P.active0 ? cnt[f][P.ord[0]] : 0;
+        any = any || P.active0 > 0;
+        P.tpre.assign(n + 1, 0);
+        for (int r = 0; r < n; ++r)
+            P.tpre[r + 1] = P.tpre[r] + (r < P.active0 ? desc[P.ord[r]].plan.ntiles : 0);
+        const int64_t nt = P.tpre[P.active0];
+        if (nt > INT32_MAX) return set_error(NX_EINVAL, "%lld tiles in one batch exceed 2^31 - 1", (long long)nt);
+        while (nt > 0 && ((nt - 1) >> P.shift) >= kPairBuckets) ++P.shift;
+        P.buckets.assign(kPairBuckets, PairBucket{0, INT32_MAX});
+        for (int j = 0, r = 0; j < kPairBuckets && ((int64_t)j << P.shift) < nt; ++j) {
+            const int64_t first = (int64_t)j << P.shift;
+            while (r + 1 < P.active0 && P.tpre[r + 1] <= first) ++r;
+            P.buckets[j] = PairBucket{r, r + 1 < P.active0 ? (int32_t)P.tpre[r + 1] : INT32_MAX};
+        }
+        std::memset(&P.args, 0, sizeof(P.args));
+        for (int r = 0; r < kArgPlanets; ++r) P.args.prefix[r] = INT32_MAX;
+        for (int r = 0; r < std::min(P.active0, kArgPlanets); ++r) {
+            const BatchPlanetDev &d = desc[P.ord[r]];
+            P.args.prefix[r] = (int32_t)P.tpre[r];   // <= nt <= INT32_MAX, checked above
+            P.args.pl[r] = TilePlanetArgs{d.plan, d.hw0,          d.hw1,  d.s0,        d.s1,
+                                          d.meta, d.total_iters, d.rain, d.tail_steps};
+        }
     }
-    // Pair-ordered descriptors and the bucket table of the many-planet pair
-    // kernel (tile b is in bucket b >> shift, at most kPairBuckets of them).
-    std::vector<BatchPlanetDev> descP(n);
-    for (int r = 0; r < n; ++r) descP[r] = desc[ordP[r]];
-    const int64_t ntP = tpre[activeP0];
-    if (ntP > INT32_MAX) return set_error(NX_EINVAL, "%lld tiles in one batch exceed 2^31 - 1", (long long)ntP);
-    int shift = 0;
-    while (((ntP - 1) >> shift) >= kPairBuckets) ++shift;
-    std::vector<PairBucket> buckets(kPairBuckets, PairBucket{0, INT32_MAX});
-    for (int j = 0, r = 0; j < kPairBuckets && ((int64_t)j << shift) < ntP; ++j) {
-        const int64_t first = (int64_t)j << shift;
-        while (r + 1 < activeP0 && tpre[r + 1] <= first) ++r;
-        buckets[j] = PairBucket{r, r + 1 < activeP0 ? (int32_t)tpre[r + 1] : INT32_MAX};
+    std::vector<int32_t> ordS;
+    const int activeS0 = ordered(ones, ordS);
+    if (!any && activeS0 == 0) return NX_OK;
+    const int64_t KS = activeS0 ? ones[ordS[0]] : 0;
+    std::vector<int64_t> bpre(n + 1, 0), sbase(n, 0);
+    for (int r = 0; r < n; ++r) {
+        bpre[r + 1] = bpre[r] + (r < activeS0 ? desc[ordS[r]].nblocks : 0);
+        sbase[r] = tile_launches[ordS[r]];
     }
     char *w = (char *)ws;
     NX_HIP(hipSetDevice(ctx->device));
-    int rc = upload(ctx, w, L,
-                    {{L.desc, desc.data(), sizeof(BatchPlanetDev) * n},
-                     {L.pair_prefix, tpre.data(), sizeof(int64_t) * (n + 1)},
-                     {L.pair_desc, descP.data(), sizeof(BatchPlanetDev) * n},
-                     {L.pair_bucket, buckets.data(), sizeof(PairBucket) * kPairBuckets},
-                     {L.one_prefix, bpre.data(), sizeof(int64_t) * (n + 1)},
-                     {L.one_order, ordS.data(), sizeof(int32_t) * n},
-                     {L.one_base, sbase.data(), sizeof(int64_t) * n},
-                     {L.steps, steps.data(), sizeof(int64_t) * n},
-                     {L.launches, launches.data(), sizeof(int32_t) * n}},
-                    stream);
+    std::vector<Part> parts = {{L.desc, desc.data(), sizeof(BatchPlanetDev) * n},
+                               {L.one_prefix, bpre.data(), sizeof(int64_t) * (n + 1)},
+                               {L.one_order, ordS.data(), sizeof(int32_t) * n},
+                               {L.one_base, sbase.data(), sizeof(int64_t) * n},
+                               {L.steps, steps.data(), sizeof(int64_t) * n},
+                               {L.launches, launches.data(), sizeof(int32_t) * n}};
+    for (int f = 0; f < kTilePhases; ++f) {
+        parts.push_back({L.tile_prefix[f], ph[f].tpre.data(), sizeof(int64_t) * (n + 1)});
+        parts.push_back({L.tile_order[f], ph[f].ord.data(), sizeof(int32_t) * n});
+        parts.push_back({L.tile_bucket[f], ph[f].buckets.data(), sizeof(PairBucket) * kPairBuckets});
+    }
+    int rc = upload_parts(ctx, w, L, parts, stream);
     if (rc) return rc;
     const auto *D = (const BatchPlanetDev *)(w + L.desc);
-    const auto *PDv = (const BatchPlanetDev *)(w + L.pair_desc);
-    const auto *PB = (const PairBucket *)(w + L.pair_bucket);
-    const auto *TP = (const int64_t *)(w + L.pair_prefix);
     const auto *OS = (const int32_t *)(w + L.one_order);
     const auto *BP = (const int64_t *)(w + L.one_prefix);
     const auto *SB = (const int64_t *)(w + L.one_base);
     const auto *ST = (const int64_t *)(w + L.steps);
     const auto *LN = (const int32_t *)(w + L.launches);
-    // Kernel-argument form of the first kArgPlanets planets in pair order.
-    PairLaunchArgs pargs;
-    std::memset(&pargs, 0, sizeof(pargs));
-    for (int r = 0; r < kArgPlanets; ++r) pargs.prefix[r] = INT32_MAX;
-    for (int r = 0; r < std::min(activeP0, kArgPlanets); ++r) {
-        const BatchPlanetDev &d = desc[ordP[r]];
-        pargs.prefix[r] = (int32_t)tpre[r];   // <= ntP <= INT32_MAX, checked above
-        pargs.pl[r] = PairPlanetArgs{d.plan, d.hw0, d.hw1, d.s0, d.s1, d.meta, d.total_iters, d.rain};
-    }
+    // Launch k of tile phase f: one planet, up to kArgPlanets, or more.
+    auto tile_launch = [&](auto kernels, int f, int active, int64_t k, hipStream_t s) {
+        const Phase &P = ph[f];
+        const int64_t nt = P.tpre[active];
+        const dim3 grid((unsigned)(((nt + 7) / 8) * 8));
+        if (active == 1) {
+            const BatchPlanetDev &d = desc[P.ord[0]];
+            hipLaunchKernelGGL(kernels.one, grid, dim3(1024), 0, s, d.meta, d.hw0, d.hw1, d.s0,
+                               d.s1, d.plan, d.tail_steps, k, d.total_iters, d.rain);
+        } else if (active <= kArgPlanets) {
+            // entries past the active planets start at or after nt
+            hipLaunchKernelGGL(kernels.args, grid, dim3(1024), 0, s, P.args, (int32_t)nt, k);
+        } else {
+            hipLaunchKernelGGL(kernels.many, grid, dim3(1024), 0, s, D,
+                               (const int32_t *)(w + L.tile_order[f]),
+                               (const PairBucket *)(w + L.tile_bucket[f]), P.shift,
+                               (const int64_t *)(w + L.tile_prefix[f]), active, nt, k);
+        }
+    };
     auto launch_all = [&](hipStream_t s) {
-        int active = activeP0;
-        for (int64_t k = 0; k < KP; ++k) {
-            while (active > 0 && pairs[ordP[active - 1]] <= k) --active;
-            const int64_t nt = tpre[active];
-            const dim3 grid((unsigned)(((nt + 7) / 8) * 8));
-            if (active == 1) {
-                const BatchPlanetDev &d = desc[ordP[0]];
-                hipLaunchKernelGGL(k_erode_batch_pair_one, grid, dim3(1024), 0, s, d.meta, d.hw0,
-                                   d.hw1, d.s0, d.s1, d.plan, k, d.total_iters, d.rain);
-            } else if (active <= kArgPlanets) {
-                // entries past the active planets start at or after nt
-                hipLaunchKernelGGL(k_erode_batch_pair_args, grid, dim3(1024), 0, s, pargs,
-                                   (int32_t)nt, k);
-            } else {
-                hipLaunchKernelGGL(k_erode_batch_pair, grid, dim3(1024), 0, s, PDv, PB, shift, TP,
-                                   active, nt, k);
+        // tails before the four-step launches of the same planet
+        for (int f = 0; f < kTilePhases; ++f) {
+            const Phase &P = ph[f];
+            int active = P.active0;
+            for (int64_t k = 0; k < P.count0; ++k) {
+                while (active > 0 && cnt[f][P.ord[active - 1]] <= k) --active;
+                switch (f) {
+                case kPhaseTail3: tile_launch(TileKernels<4, 3>(), f, active, k, s); break;
+                case kPhaseTail2: tile_launch(TileKernels<4, 2>(), f, active, k, s); break;
+                case kPhaseQuad: tile_launch(TileKernels<4, 4>(), f, active, k, s); break;
+                default: tile_launch(TileKernels<2, 2>(), f, active, k, s); break;
+                }
             }
         }
-        active = activeS0;
+        int active = activeS0;
         for (int64_t k = 0; k < KS; ++k) {
             while (active > 0 && ones[ordS[active - 1]] <= k) --active;
             const int64_t nb = bpre[active];
@@ -658,9 +758,11 @@ int batch_erode(nx_ctx *ctx, nx_planet *planets, int n, int max_steps, void *ws,
         // Cache key: workspace, the descriptor contents and the schedule.
         uint64_t h = 1469598103934665603ull;
         h = fnv(desc.data(), sizeof(BatchPlanetDev) * n, h);
-        h = fnv(pairs.data(), sizeof(int64_t) * n, h);
+        for (const auto &c : cnt) h = fnv(c.data(), sizeof(int64_t) * n, h);
         h = fnv(ones.data(), sizeof(int64_t) * n, h);
-        GraphKey key{ws, (int64_t)n, nullptr, nullptr, (int)(KP + KS), 0.0, true, h};
+        int64_t nl = KS;
+        for (const Phase &P : ph) nl += P.count0;
+        GraphKey key{ws, (int64_t)n, nullptr, nullptr, (int)nl, 0.0, true, h};
         hipGraphExec_t ex = nullptr;
         {
             std::lock_guard<std::mutex> lk(ctx->mu);

@@ -16,7 +16,8 @@ so jobs that arrive while others are on the GPU join the next tick
 jobs erode, noise passes may use only part of the CUs (noise_cu_share).  An
 erosion tick hands out a budget of vertex-steps to the eroding jobs oldest
 first, each job taking as many of its remaining steps as the budget left
-covers (at least two): a small planet finishes all its steps in one tick
+covers, in whole launches of its tile plan (four steps, or two; at least
+one launch): a small planet finishes all its steps in one tick
 while a large one advances a few steps per tick in arrival order.  By
 default the oldest eroding job has each tick to itself (young_share) unless
 its planet is too small to fill the GPU (fill_vertices), and noise passes may
@@ -36,6 +37,15 @@ import numpy as np
 
 from . import _lib
 from . import batch as _batch
+
+
+def _launch_steps(run):
+    """Steps per tile launch of a run's erosion graph (4 or 2; 2 without a
+    plan or a graph)."""
+    g = getattr(run, "graph", None)
+    if g is None or getattr(g, "plan", None) is None:
+        return 2
+    return int(g.steps)
 
 
 def job_bytes(job):
@@ -207,7 +217,8 @@ class PlanetServer:
 
     def _tick(self, eroding):
         """(jobs, step caps) of the next erosion tick: oldest first, each
-        taking the steps the remaining budget covers (at least two).  Younger
+        taking the steps the remaining budget covers, in whole launches of
+        its tile plan's steps (at least one launch).  Younger
         jobs share the tick's budget up to young_share of it, and beyond that
         while the tick's planets together have fewer than fill_vertices
         vertices (too few tiles to fill the GPU), for at most the oldest
@@ -226,10 +237,11 @@ class PlanetServer:
             remaining = r.job.erosion_iters - r.steps_done
             n = remaining if self.steps_per_tick == 0 else min(remaining, self.steps_per_tick)
             n = min(n, max(2, allow // r.V))
-            if n < remaining and n % 2:
-                # whole pairs; an odd (single-step) launch only at the end,
-                # and never fewer than one pair, so every tick advances
-                n = max(n - 1, min(2, remaining))
+            unit = _launch_steps(r)
+            if n < remaining and n % unit:
+                # whole launches; a shorter one only at the end, and never
+                # fewer than one launch, so every tick advances
+                n = max(n - n % unit, min(unit, remaining))
             if tick:
                 young_left -= n * r.V
             tick.append(r)

@@ -91,18 +91,25 @@ def test_batch_one_planet_chunks_under_graphs(meshes, plain, chunk):
         assert (_heights(runs)[0] == plain[2]).all(), f"pass {rep}"
 
 
-@pytest.mark.parametrize("use_graph", [False, True])
-def test_batch_many_planets_equals_plain(meshes, use_graph):
-    """More than 16 eroding planets take the many-planet pair kernel, which
-    finds a tile's planet through a table of buckets over the tiles (two
-    tiles per bucket here, so some buckets hold the start of a second
-    planet and one-tile planets share buckets): every planet equals its
-    single-planet result, with and without graphs."""
+@pytest.mark.parametrize("use_graph", [True, False])
+@pytest.mark.parametrize("steps", [4, 2])
+def test_batch_many_planets_equals_plain(use_graph, steps):
+    """More than 16 eroding planets take the many-planet tile kernels, which
+    find a tile's planet through a table of buckets over the tiles (tiles
+    from chunks of 256 vertices: two tiles per bucket here, so some buckets
+    hold the start of a second planet and one-tile planets share buckets);
+    with four-step plans the step counts 2..11 give tails of two and three
+    steps and single last steps.  Every planet equals its single-planet
+    result, with and without graphs."""
     rng = np.random.default_rng(8)
     ds = rng.choice([6, 8, 12, 40, 64, 90, 120], size=24)
     its = rng.integers(2, 12, size=24)
     jobs = [batch.Job(d=int(d), seed=k, n_octaves=4, erosion_iters=int(it), rain=1.0)
             for k, (d, it) in enumerate(zip(ds, its))]
+    meshes = batch.MeshCache()
+    for d in sorted(set(int(d) for d in ds)):
+        g = meshes.graph(d, tuning=autotune.ErosionTuning(256, 0, 32, steps))
+        assert g.plan is not None and g.steps == steps
     ref = _runs(jobs, meshes)
     batch.run_plain(ref)
     want = _heights(ref)
@@ -110,12 +117,35 @@ def test_batch_many_planets_equals_plain(meshes, use_graph):
     br = batch.BatchRunner()
     br.run(runs, use_graph=use_graph)
     br.ctx.sync_check()
-    plans = [meshes.graph(j.d).plan for j in jobs]
-    assert all(q is not None for q in plans)
-    tiles = sum(q.ntiles for q in plans)
+    tiles = sum(meshes.graph(j.d).plan.ntiles for j in jobs)
     assert tiles > 1024, tiles
     for k, (a, b) in enumerate(zip(_heights(runs), want)):
         assert (a == b).all(), f"planet {k} (d={jobs[k].d}) differs from the single-planet path"
+
+
+def test_batch_mixed_plan_steps_equals_plain():
+    """One batch holding planets with four-step plans (tails of two and
+    three steps, single last steps) and planets with two-step plans: the
+    four tile phases and the one-step phase run in one call, and every
+    planet equals its single-planet result."""
+    meshes = batch.MeshCache()
+    four, two = (10, 24, 48), (12, 20, 36)
+    for d in four:
+        assert meshes.graph(d, tuning=autotune.ErosionTuning(256, 0, 32, 4)).steps == 4
+    for d in two:
+        assert meshes.graph(d, tuning=autotune.ErosionTuning(256, 0, 32, 2)).steps == 2
+    jobs = [batch.Job(d=d, seed=k, n_octaves=5, erosion_iters=it, rain=1.0)
+            for k, (d, it) in enumerate(zip(four + two + four, (9, 6, 7, 9, 6, 7, 13, 2, 1)))]
+    ref = _runs(jobs, meshes)
+    batch.run_plain(ref)
+    want = _heights(ref)
+    for use_graph in (True, False):
+        runs = _runs(jobs, meshes)
+        br = batch.BatchRunner()
+        br.run(runs, use_graph=use_graph)
+        br.ctx.sync_check()
+        for k, (a, b) in enumerate(zip(_heights(runs), want)):
+            assert (a == b).all(), f"planet {k} (d={jobs[k].d}) differs, graph={use_graph}"
 
 
 def test_batch_alone_equals_in_mix(meshes, plain):

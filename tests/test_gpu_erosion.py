@@ -498,13 +498,16 @@ def test_erosion_autotune_picks_persists_and_keeps_results(oracle, tmp_path):
         assert (g.unpermute(out.cpu().numpy()) == href).all(), kw
         assert ctx.graph_chunk == 32 and ctx.pair_kernel == 0 and ctx.multi_kernel == 0
     assert len(tiles) >= 2      # the chunks cut different plans
-    # MeshCache cuts two-step plans (what its multi-planet launches run) with a given tuning
+    # MeshCache cuts its plans with a given tuning
     mc = batch.MeshCache()
-    g = mc.graph(48, tuning=autotune.ErosionTuning(256, 5, 8))
+    g = mc.graph(48, tuning=autotune.ErosionTuning(256, 5, 8, 2))
     assert (g.chunk, g.variant, g.graph_chunk, g.steps) == (256, 5, 8, 2)
-    # its own search covers two-step plans only and keeps a faster four-step entry first
+    mc.evict(48)
+    g = mc.graph(48, tuning=autotune.ErosionTuning(512, 0, 16, 4))
+    assert (g.chunk, g.variant, g.graph_chunk, g.steps) == (512, 0, 16, 4)
+    # a search of two-step plans alone keeps a faster four-step entry first
     b2, _ = mc.tune_erosion(48, steps=6, reps=1, path=path, variants={2: (0, 5)},
-                            chunks={2: (256,)}, graph_chunks=(8,))
+                            chunks={2: (256,)}, graph_chunks=(8,), step_options=(2,))
     assert b2.steps == 2 and mc.graph(48).steps == 2
     assert autotune.lookup_erosion(pts.shape[0], path=path, steps=2) == b2
     if best.steps == 4:

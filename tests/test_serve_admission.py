@@ -62,32 +62,45 @@ def test_served_trace_shape():
     assert abs(sum(gaps) / len(gaps) - 1 / 50.0) < 0.005
 
 
+class _Plan:
+    pass
+
+
+class _Graph:
+    def __init__(self, steps):
+        self.steps = steps
+        self.plan = _Plan()
+
+
 class _Run:
-    def __init__(self, V, iters):
+    def __init__(self, V, iters, steps=None):
         self.V = V
         self.steps_done = 0
         self.job = Job(d=2, seed=0, n_octaves=1, erosion_iters=iters)
+        if steps is not None:
+            self.graph = _Graph(steps)
 
 
+@pytest.mark.parametrize("unit", [2, 4])
 @pytest.mark.parametrize("budget, per_tick", [(9_000, 0), (4_000, 0), (50_000, 5), (50_000, 1),
                                               (50_000, 3)])
-def test_erosion_ticks_hand_out_whole_pairs(budget, per_tick):
-    """Every tick gives a job an even number of steps (two-step launches)
-    except the tick that finishes it, and the steps add up to the job's.
-    A per-job cap of one or three steps still advances every job by a pair
-    per tick (rounding down to whole pairs never reaches zero steps)."""
+def test_erosion_ticks_hand_out_whole_launches(budget, per_tick, unit):
+    """Every tick gives a job a whole number of its plan's launches (two
+    or four steps) except the tick that finishes it, and the steps add up to
+    the job's.  A per-job cap below one launch still advances every job by
+    a launch per tick (rounding down never reaches zero steps)."""
     from nixis_amd.serve import PlanetServer
     srv = PlanetServer.__new__(PlanetServer)
     srv.tick_vertex_steps, srv.steps_per_tick, srv.young_share = budget, per_tick, 1.0
     srv.fill_vertices = 0
-    runs = [_Run(1000, 31), _Run(400, 7), _Run(3000, 10)]
+    runs = [_Run(1000, 31, unit), _Run(400, 7, unit), _Run(3000, 10, unit)]
     eroding, ticks = list(runs), 0
     while eroding:
         tick, caps = srv._tick(eroding)
         assert tick and all(n >= 1 for n in caps)
         for r, n in zip(tick, caps):
             left = r.job.erosion_iters - r.steps_done
-            assert n <= left and (n % 2 == 0 or n == left)
+            assert n <= left and (n % unit == 0 or n == left)
             r.steps_done += n
         eroding = [r for r in eroding if r.steps_done < r.job.erosion_iters]
         ticks += 1
