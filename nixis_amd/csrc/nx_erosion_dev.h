@@ -56,22 +56,16 @@ NX_EHD void erode_accumulate(double h, double hn, double wn, double d, double &S
 {
     double slope = hn - h;
     if (__builtin_expect(fabs(slope) < 0x1p-1000, 0)) slope = slope / (d + 0.00001);
+    // Both branches add or subtract the same two products: computed once
+    // here, so lanes that take different branches share them.
+    const double ts = RULE == 4 ? max0(kSol * wn) : kSol * wn;
+    const double tw = RULE == 4 ? max0(wn * d) : wn * d;
     if (slope > 0) {
-        if (RULE == 4) {
-            S += max0(kSol * wn);
-            W += max0(wn * d);
-        } else {
-            S += kSol * wn;
-            W += wn * d;
-        }
+        S += ts;
+        W += tw;
     } else if (slope < 0) {
-        if (RULE == 4) {
-            S -= max0(kSol * wn);
-            W -= max0(wn * d);
-        } else {
-            S -= kSol * wn;
-            W -= wn * d;
-        }
+        S -= ts;
+        W -= tw;
     }
 }
 
