@@ -248,6 +248,34 @@ def test_pair_kernel_variants_match_oracle(oracle, variant):
             assert np.array_equal(x.view(np.int64), r.view(np.int64))
 
 
+def test_graph_without_room_for_four_rings_takes_a_two_step_plan(oracle, monkeypatch):
+    """When the planner refuses four rings (NX_ERANGE, forced here for every
+    four-step request) the graph cuts a two-step plan instead, and erosion
+    on it equals the oracle bit for bit."""
+    pts, nb, h = _planet(24)
+    real = _lib.lib()
+
+    class FourStepsRefused:
+        def __getattr__(self, name):
+            return getattr(real, name)
+
+        @staticmethod
+        def nx_tile_plan_build_chunk(col, elen, V, steps, chunk, out, cap, plan):
+            if steps == 4:
+                return _lib.NX_ERANGE
+            return real.nx_tile_plan_build_chunk(col, elen, V, steps, chunk, out, cap, plan)
+
+    monkeypatch.setattr(_lib, "lib", lambda: FourStepsRefused())
+    g = erosion.ErosionGraph(pts, nb)
+    monkeypatch.undo()
+    assert g.steps == 2 and g.plan is not None and g.plan.steps == 2
+    got = _run_graph(g, h, 9, 1.0)
+    href = h.copy()
+    w, s = oracle.erode_terrain3(pts, nb, href, 9, 1.0, return_fields=True)
+    for x, r in zip(got, (href, w, s)):
+        assert np.array_equal(x, r)
+
+
 @pytest.mark.parametrize("rule", [3, 4])
 def test_four_step_plans_match_oracle(oracle, rule):
     """Four steps per launch, a remainder of two or three steps in one tail
