@@ -78,8 +78,8 @@ struct BatchLayout {
 // before its four-step launches).
 enum TilePhaseKind { kPhaseTail3, kPhaseTail2, kPhaseQuad, kPhasePair, kTilePhases };
 
-// Planet lookup of the many-planet pair kernel: tile b falls in bucket
-// b >> shift; the bucket holds the planet (in pair order) of its first tile
+// Planet lookup of the many-planet tile kernels: tile b falls in bucket
+// b >> shift; the bucket holds the planet (in the phase's order) of its first tile
 // and the first tile of the next planet.
 constexpr int kPairBuckets = 1024;
 struct PairBucket {
