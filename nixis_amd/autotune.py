@@ -117,8 +117,8 @@ def _tuning(e):
 
 def lookup_erosion(V, path=CACHE_PATH, steps=None):
     """Cached erosion tuning for meshes of V vertices' size class, or None;
-    with steps=2, the best two-step tuning (what the multi-planet launches,
-    which run two steps per launch, can use)."""
+    with steps=2 (or 4), the best tuning of that many steps per launch when
+    the cache holds one."""
     e = load_cache(path).get(erosion_key(V))
     if not e:
         return None
@@ -221,8 +221,7 @@ def merge_erosion_entry(prev, entry, searched):
     """The cache entry to store after a search of the step counts in
     `searched` found `entry`: a cached tuning of a step count not searched
     now that is at least as fast stays first, with the new one beside it
-    (a search of two-step plans alone, for the multi-planet launches, keeps
-    a faster four-step tuning of the single-planet form)."""
+    (a search of two-step plans alone keeps a faster four-step tuning)."""
     if prev and prev.get("steps", 2) not in searched \
             and prev.get("us_per_step", math.inf) <= entry["us_per_step"]:
         return dict(prev, **{f"steps{entry['steps']}": entry})

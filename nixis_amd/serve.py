@@ -149,10 +149,10 @@ def _prepare_meshes(meshes, jobs):
 
 
 # Rates of the device kernels used to size a tick's noise admissions (MI355X:
-# fused noise about 2.2e11 vertex-octaves/s, two-step erosion about 8e10
+# fused noise about 2.3e11 vertex-octaves/s, four-step erosion about 1.2e11
 # vertex-steps/s); only their ratio matters.
-NOISE_RATE = 2.2e11
-EROSION_RATE = 8.0e10
+NOISE_RATE = 2.3e11
+EROSION_RATE = 1.2e11
 
 
 class PlanetServer:
